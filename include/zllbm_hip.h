@@ -1,0 +1,133 @@
+/*
+ * zllbm_hip.h -- C ABI of the MI355X (gfx950) lattice Boltzmann time step on ZLLBM's
+ * Morton-keyed AMR grid.
+ *
+ * One context owns all device memory of one solver instance on the current HIP device.  Host
+ * arrays passed in are borrowed for the duration of the call; output arrays are caller owned.
+ * Calls are not thread safe on one context; every device operation is enqueued on the
+ * context's stream and only the download / conservation / timing calls synchronise.
+ * No function aborts or throws: errors are returned as a negative status and described by
+ * zlh_last_error().
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the ZLLBM
+ * source tree).
+ */
+#ifndef ZLLBM_HIP_H_
+#define ZLLBM_HIP_H_
+#include <stdint.h>
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct zlh_ctx zlh_ctx;
+typedef int32_t zlh_status;
+
+enum {
+    ZLH_OK = 0,
+    ZLH_EINVAL = -1,    /* bad argument */
+    ZLH_EDEVICE = -2,   /* HIP runtime error */
+    ZLH_ENOMEM = -3,    /* device allocation failed */
+    ZLH_ESTATE = -4     /* call out of order (e.g. step before build) */
+};
+
+/* storage / arithmetic form of the populations */
+enum {
+    ZLH_FP64 = 0,           /* f stored and computed in double */
+    ZLH_FP32_SHIFTED = 1    /* f - w_i stored and computed in float */
+};
+
+/* ELbmBoundaryConditionScheme (boundary_conditions/lbm_boundary_conditions.h) */
+enum { ZLH_BC_NONE = 0, ZLH_BC_BOUNCE_BACK = 1, ZLH_BC_PERIODIC = 2, ZLH_BC_NEE = 3 };
+/* EDomainBoundaryDirection (grid/grid_enumerates.h:37-45) */
+enum { ZLH_XMIN = 1, ZLH_XMAX = 2, ZLH_YMIN = 3, ZLH_YMAX = 4, ZLH_ZMIN = 5, ZLH_ZMAX = 6 };
+/* InterfaceLayerInfo maps (grid/grid_info_interface.h:117-127) */
+enum { ZLH_OUTER_F2C = 0, ZLH_OUTER_C2F = 1, ZLH_INNER_F2C = 2, ZLH_INNER_C2F = 3 };
+/* collision operators (ELbmCollisionOperatorType, collision_model/lbm_collision_models.h) */
+enum { ZLH_SRT = 0, ZLH_MRT = 1 };
+/* sub-grid closures */
+enum { ZLH_LES_NONE = 0, ZLH_LES_SMAGORINSKY = 1, ZLH_LES_MLP = 2 };
+/* Smagorinsky relaxation time: reference = tau_eff is the SGS increment alone (the shipped
+ * behaviour of lbm_solver_interface.cpp:276-295), physical = tau0 + increment */
+enum { ZLH_LES_TAU_REFERENCE = 0, ZLH_LES_TAU_PHYSICAL = 1 };
+
+/* Solver settings: SolverLbmInterface members set by the drivers (lbm_interface.h:288-474)
+ * and the per-direction boundary settings (SetDomainBoundaryCondition, SetValues). */
+typedef struct zlh_solver_desc {
+    int32_t dims;               /* 2 (D2Q9) or 3 (D3Q19) */
+    int32_t max_level;          /* finest level, 0 = uniform */
+    int32_t precision;          /* ZLH_FP64 / ZLH_FP32_SHIFTED */
+    int32_t compressible;       /* k0BoolCompressible_ */
+    double nu0;                 /* k0LbmViscosity_: viscosity at level 0 */
+    double rho0;                /* k0Rho_ */
+    double u0[3];               /* k0Velocity_ */
+    int32_t les;                /* ZLH_LES_* */
+    int32_t les_tau_mode;       /* ZLH_LES_TAU_* */
+    double cs;                  /* Smagorinsky constant */
+    int32_t has_force;          /* bool_forces_ */
+    int32_t pad0;
+    double const_force[3];      /* k0ConstForce_ */
+    int32_t collision[16];      /* per level ZLH_SRT / ZLH_MRT */
+    int32_t bc_scheme[7];       /* per direction 1..6 */
+    int32_t pad1;
+    double bc_value[7][3];      /* wall / boundary velocity per direction */
+    uint32_t dom_min[3];        /* background node index bounds */
+    uint32_t dom_max[3];
+} zlh_solver_desc;
+
+/* SolverCreatorLbmD2Q9/D3Q19::CreateSolver + SolverLbmInterface::SolverInitial
+ * (lbm_solver_interface.cpp:19-44) */
+zlh_status zlh_create(const zlh_solver_desc *desc, zlh_ctx **out);
+/* nodes of one level: map_grid_node_ keys and flag_status_ (grid/grid_info_interface.h:258);
+ * keys sorted ascending and unique, no axis index 0 or at the top of the 64-bit code */
+zlh_status zlh_set_level_nodes(zlh_ctx *ctx, int level, const uint64_t *keys, const int32_t *flags,
+                               int64_t n);
+/* one interface layer map (InterfaceLayerInfo::vec_{outer,inner}_{coarse2fine,fine2coarse}_) */
+zlh_status zlh_set_interface_layer(zlh_ctx *ctx, int level, int kind, int layer,
+                                   const uint64_t *keys, int64_t n);
+/* immersed-boundary markers of one geometry at one level (GeometryVertexImmersedBoundary
+ * coordinate_/velocity_/area_, immersed_boundary.h:23-28); xyz and vel are [n][3] */
+zlh_status zlh_ib_set_markers(zlh_ctx *ctx, int level, const double *xyz, const double *area,
+                              const double *vel, int64_t n, int stencil_dis, double stiffness);
+/* device slot and neighbour tables, boundary / interface / IB tables
+ * (replaces the per-access unordered_map lookups of lbm_d3q19_solver.cpp:20-135) */
+zlh_status zlh_build(zlh_ctx *ctx);
+/* every node at feq(rho0, u0) (SetInitialDisFuncBasedOnReferenceMacros, lbm_solver_interface.cpp:344-367) */
+zlh_status zlh_init_feq(zlh_ctx *ctx);
+/* per node initial macros in key order, f = f_collide = feq(rho, u); u is [n][dims] */
+zlh_status zlh_init_macros(zlh_ctx *ctx, int level, const double *rho, const double *u);
+/* post-stream populations in key order [n][q] (GridNodeLbm::f_); also sets f_collide = f */
+zlh_status zlh_upload_populations(zlh_ctx *ctx, int level, const double *f);
+/* one level substep: IB forcing, collision, streaming, domain boundaries, and on even substeps
+ * of a refined level the coarse/fine transfer (GridInfoLbmInteface::AdvancingAtCurrentTime,
+ * lbm_grid_interface.cpp:104-153) */
+zlh_status zlh_step_level(zlh_ctx *ctx, int level, int time_step_level);
+/* background steps in the coarse-to-fine multi time stepping order (AmrManager::TimeMarching,
+ * amr_manager.cpp:279-300; MultiTimeSteppingC2F, grid/grid_manager.cpp:997-1024) */
+zlh_status zlh_time_marching(zlh_ctx *ctx, int nsteps);
+/* post-stream populations GridNodeLbm::f_ in key order [n][q] (synchronising) */
+zlh_status zlh_download_populations(zlh_ctx *ctx, int level, double *f);
+/* density and velocity of the post-stream populations, key order; u is [n][dims] (synchronising) */
+zlh_status zlh_download_macros(zlh_ctx *ctx, int level, double *rho, double *u);
+/* pull-source table [n][q]: key-order index of the node x - c_q, -1 when absent; the device
+ * replacement of the FindX/Y/Z{Pos,Neg} + map lookups (synchronising) */
+zlh_status zlh_download_source_table(zlh_ctx *ctx, int level, int32_t *src);
+/* Lagrangian marker forces of the last step [n][3] (synchronising) */
+zlh_status zlh_ib_download_forces(zlh_ctx *ctx, int level, double *force);
+/* FP64 mass and momentum sums of the post-stream populations of a level: out = {m, jx, jy, jz}
+ * (no reference counterpart; synchronising) */
+zlh_status zlh_conservation(zlh_ctx *ctx, int level, double *out4);
+/* node counts: total nodes of a level, and how many take the deferred (aux) path */
+zlh_status zlh_level_info(zlh_ctx *ctx, int level, int64_t *n_nodes, int64_t *n_aux, int64_t *n_bricks);
+zlh_status zlh_synchronize(zlh_ctx *ctx);
+/* device time of the fused stream-collide kernel: enable recording, then read the total in
+ * milliseconds and the number of launches since enabling (synchronising) */
+zlh_status zlh_timing_enable(zlh_ctx *ctx, int enable);
+zlh_status zlh_timing_read(zlh_ctx *ctx, double *total_ms, int64_t *launches);
+const char *zlh_last_error(const zlh_ctx *ctx);
+void zlh_destroy(zlh_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,233 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/zlbm_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of bench.py, where it serves as the checker.  The product path
+(``zllbm_amd``) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libzlbm_oracle.so")
+_lib = None
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("dims", C.c_int32), ("max_level", C.c_int32), ("nu0", C.c_double), ("rho0", C.c_double),
+        ("u0", C.c_double * 3), ("compressible", C.c_int32), ("les", C.c_int32),
+        ("les_tau_mode", C.c_int32), ("cs", C.c_double), ("has_force", C.c_int32),
+        ("const_force", C.c_double * 3), ("collision", C.c_int32 * 16),
+        ("bc_scheme", C.c_int32 * 7), ("bc_value", (C.c_double * 3) * 7),
+        ("dom_min", C.c_uint32 * 3), ("dom_max", C.c_uint32 * 3),
+    ]
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        P = C.c_void_p
+        D = np.ctypeslib.ndpointer
+        L.zo_create.restype = P
+        L.zo_create.argtypes = [C.POINTER(Params)]
+        L.zo_destroy.argtypes = [P]
+        L.zo_last_error.restype = C.c_char_p
+        L.zo_last_error.argtypes = [P]
+        L.zo_set_level.argtypes = [P, C.c_int, C.c_int64, D(np.uint64), D(np.int32)]
+        L.zo_set_layer.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int64, D(np.uint64)]
+        L.zo_finalize.argtypes = [P]
+        L.zo_init_feq.argtypes = [P]
+        L.zo_init_macro.argtypes = [P, C.c_int, D(np.float64), D(np.float64)]
+        for nm in ("zo_set_f", "zo_get_f", "zo_get_fc", "zo_set_node_force", "zo_get_node_force"):
+            getattr(L, nm).argtypes = [P, C.c_int, D(np.float64)]
+        L.zo_get_node_macro.argtypes = [P, C.c_int, D(np.float64), D(np.float64)]
+        L.zo_level_size.restype = C.c_int64
+        L.zo_level_size.argtypes = [P, C.c_int]
+        L.zo_get_source_table.argtypes = [P, C.c_int, D(np.int32)]
+        L.zo_get_boundary.restype = C.c_int64
+        L.zo_get_boundary.argtypes = [P, C.c_int, C.c_int, C.c_void_p]
+        L.zo_advance_level.argtypes = [P, C.c_int, C.c_int]
+        L.zo_time_marching.argtypes = [P, C.c_int]
+        L.zo_time_stepping_order.argtypes = [C.c_int, D(np.int32), C.c_int]
+        L.zo_set_ib.argtypes = [P, C.c_int, C.c_int64, D(np.float64), D(np.float64), D(np.float64),
+                                C.c_int, C.c_double, C.c_double]
+        L.zo_get_ib_force.argtypes = [P, C.c_int, D(np.float64)]
+        L.zo_conservation.argtypes = [P, C.c_int, D(np.float64)]
+        L.zo_lattice.argtypes = [C.c_int, D(np.float64), D(np.float64), D(np.float64), D(np.float64),
+                                 D(np.int32)]
+        L.zo_mrt_matrices.argtypes = [C.c_int, D(np.float64), D(np.float64), D(np.float64)]
+        L.zo_encode.restype = C.c_uint64
+        L.zo_encode.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.zo_decode.argtypes = [C.c_int, C.c_uint64, D(np.uint32)]
+        L.zo_find_neighbor.restype = C.c_uint64
+        L.zo_find_neighbor.argtypes = [C.c_int, C.c_uint64, C.c_int, C.c_int]
+        L.zo_set_threads.argtypes = [C.c_int]
+        L.zo_debug_set_s_base.argtypes = [P, D(np.float64)]
+        _lib = L
+    return _lib
+
+
+def lattice(dims: int):
+    q = 9 if dims == 2 else 19
+    cx, cy, cz, w = (np.zeros(q) for _ in range(4))
+    opp = np.zeros(q, dtype=np.int32)
+    lib().zo_lattice(dims, cx, cy, cz, w, opp)
+    return cx, cy, cz, w, opp
+
+
+def mrt_matrices(dims: int):
+    q = 9 if dims == 2 else 19
+    m, minv = np.zeros(q * q), np.zeros(q * q)
+    s = np.zeros(q)
+    lib().zo_mrt_matrices(dims, m, minv, s)
+    return m.reshape(q, q), minv.reshape(q, q), s
+
+
+def time_stepping_order(max_level: int) -> np.ndarray:
+    buf = np.zeros(1 << 12, dtype=np.int32)
+    n = lib().zo_time_stepping_order(max_level, buf, buf.size)
+    return buf[:n].copy()
+
+
+class Oracle:
+    """One oracle simulation: a set of levels with their populations."""
+
+    def __init__(self, grid, *, nu0: float, rho0: float = 1.0, u0=(0., 0., 0.),
+                 collision: Sequence[int] = (0,), les: int = 0, les_tau_mode: int = 1,
+                 cs: float = 0.1, has_force: bool = False, const_force=(0., 0., 0.),
+                 bc: Optional[Dict[int, tuple]] = None, compressible: bool = True):
+        self.grid = grid
+        self.dims = grid.dims
+        self.q = 9 if grid.dims == 2 else 19
+        p = Params()
+        p.dims = grid.dims
+        p.max_level = grid.max_level
+        p.nu0, p.rho0 = nu0, rho0
+        for a in range(3):
+            p.u0[a] = u0[a] if a < len(u0) else 0.
+            p.const_force[a] = const_force[a] if a < len(const_force) else 0.
+        p.compressible = 1 if compressible else 0
+        p.les, p.les_tau_mode, p.cs = les, les_tau_mode, cs
+        p.has_force = 1 if has_force else 0
+        for l in range(16):
+            p.collision[l] = collision[min(l, len(collision) - 1)]
+        bc = bc or {}
+        for d, (scheme, val) in bc.items():
+            p.bc_scheme[d] = scheme
+            for a in range(3):
+                p.bc_value[d][a] = val[a] if a < len(val) else 0.
+        for a in range(grid.dims):
+            p.dom_min[a] = grid.dom_min[a]
+            p.dom_max[a] = grid.dom_max[a]
+        self.params = p
+        L = lib()
+        self.h = L.zo_create(C.byref(p))
+        if not self.h:
+            raise RuntimeError("zo_create failed")
+        for l, lev in enumerate(grid.levels):
+            self._chk(L.zo_set_level(self.h, l, lev.n, np.ascontiguousarray(lev.keys, np.uint64),
+                                     np.ascontiguousarray(lev.flags, np.int32)))
+        for l, lev in enumerate(grid.levels):
+            for (kind, layer), keys in lev.layers.items():
+                self._chk(L.zo_set_layer(self.h, l, kind, layer, keys.size,
+                                         np.ascontiguousarray(keys, np.uint64)))
+        self._chk(L.zo_finalize(self.h))
+        self._chk(L.zo_init_feq(self.h))
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise RuntimeError("oracle: " + lib().zo_last_error(self.h).decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().zo_destroy(self.h)
+            self.h = None
+
+    def n(self, level=0) -> int:
+        return int(lib().zo_level_size(self.h, level))
+
+    def init_macro(self, level, rho, u):
+        self._chk(lib().zo_init_macro(self.h, level, np.ascontiguousarray(rho, np.float64),
+                                      np.ascontiguousarray(u, np.float64).ravel()))
+
+    def set_f(self, level, f):
+        self._chk(lib().zo_set_f(self.h, level, np.ascontiguousarray(f, np.float64).ravel()))
+
+    def f(self, level=0) -> np.ndarray:
+        out = np.zeros(self.n(level) * self.q)
+        self._chk(lib().zo_get_f(self.h, level, out))
+        return out.reshape(-1, self.q)
+
+    def fc(self, level=0) -> np.ndarray:
+        out = np.zeros(self.n(level) * self.q)
+        self._chk(lib().zo_get_fc(self.h, level, out))
+        return out.reshape(-1, self.q)
+
+    def node_macro(self, level=0):
+        rho = np.zeros(self.n(level))
+        u = np.zeros(self.n(level) * self.dims)
+        self._chk(lib().zo_get_node_macro(self.h, level, rho, u))
+        return rho, u.reshape(-1, self.dims)
+
+    def set_node_force(self, level, force):
+        self._chk(lib().zo_set_node_force(self.h, level, np.ascontiguousarray(force, np.float64).ravel()))
+
+    def node_force(self, level=0):
+        out = np.zeros(self.n(level) * self.dims)
+        self._chk(lib().zo_get_node_force(self.h, level, out))
+        return out.reshape(-1, self.dims)
+
+    def source_table(self, level=0) -> np.ndarray:
+        out = np.zeros(self.n(level) * self.q, dtype=np.int32)
+        self._chk(lib().zo_get_source_table(self.h, level, out))
+        return out.reshape(-1, self.q)
+
+    def boundary(self, level, direction) -> np.ndarray:
+        n = lib().zo_get_boundary(self.h, level, direction, None)
+        out = np.zeros(max(n, 1), dtype=np.int32)
+        lib().zo_get_boundary(self.h, level, direction, out.ctypes.data_as(C.c_void_p))
+        return out[:n]
+
+    def advance_level(self, level, time_step_level):
+        self._chk(lib().zo_advance_level(self.h, level, time_step_level))
+
+    def time_marching(self, nsteps=1):
+        self._chk(lib().zo_time_marching(self.h, nsteps))
+
+    def set_ib(self, level, xyz, area, vel, stencil_dis=2, stiffness=2.0):
+        """Markers of one immersed boundary at ``level``: coordinates and velocities (n, dims),
+        areas (n,), in background-spacing units with node index i at coordinate i * dx_l."""
+        def pad3(a):
+            a = np.asarray(a, np.float64).reshape(len(area), -1)
+            return np.ascontiguousarray(np.pad(a, ((0, 0), (0, 3 - a.shape[1]))))
+        self._ib_n = len(area)
+        self._chk(lib().zo_set_ib(self.h, level, len(area), pad3(xyz).ravel(),
+                                  np.ascontiguousarray(area, np.float64), pad3(vel).ravel(),
+                                  stencil_dis, stiffness, 1.0))
+
+    def ib_force(self, level) -> np.ndarray:
+        out = np.zeros(self._ib_n * 3)
+        self._chk(lib().zo_get_ib_force(self.h, level, out))
+        return out.reshape(-1, 3)[:, :self.dims]
+
+    def debug_set_s_base(self, s_base):
+        self._chk(lib().zo_debug_set_s_base(self.h, np.ascontiguousarray(s_base, np.float64)))
+
+    def conservation(self, level=0) -> np.ndarray:
+        out = np.zeros(4)
+        self._chk(lib().zo_conservation(self.h, level, out))
+        return out

@@ -1,0 +1,1100 @@
+// zlh_api.hip -- host side of the C ABI declared in include/zllbm_hip.h.
+//
+// The context keeps, per level, the sorted Morton keys on the host (for table construction) and
+// the brick-layout device state described in kernels.hpp.  Device tables are built by kernels
+// (slot lookup by binary search over sorted brick keys, neighbour bricks from Morton neighbour
+// arithmetic); the host only classifies which nodes take the deferred path and lays out the
+// boundary / transfer / forcing tables that address them.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+#include "zllbm_hip.h"
+#include <functional>
+#include "kernels.hpp"
+#include "amr_ib_types.hpp"
+
+namespace zlh {
+
+// ----------------------------------------------------------------------------------------------
+// Morton helpers (host and device): same bit layout as SFBitsetAux2D/3D (grid/morton_aux.cpp)
+// ----------------------------------------------------------------------------------------------
+__host__ __device__ inline uint64_t axis_mask(int dims, int axis) {
+    uint64_t m = 0;
+    const int bits = 64 / dims;
+    for (int i = 0; i < bits; ++i) m |= 1ull << (i * dims + axis);
+    return m;
+}
+
+__host__ __device__ inline uint32_t axis_coord(int dims, uint64_t key, int axis) {
+    uint32_t v = 0;
+    const int bits = 64 / dims;
+    for (int i = 0; i < bits && i < 32; ++i) v |= (uint32_t)((key >> (i * dims + axis)) & 1ull) << i;
+    return v;
+}
+
+__host__ __device__ inline uint64_t spread(int dims, uint32_t v, int axis) {
+    uint64_t k = 0;
+    const int bits = 64 / dims;
+    for (int i = 0; i < bits && i < 32; ++i) k |= (uint64_t)((v >> i) & 1u) << (i * dims + axis);
+    return k;
+}
+
+// neighbour along one axis without wrap-around: returns false past either end of the code space
+__host__ __device__ inline bool step_axis(int dims, uint64_t key, int axis, int dir, uint64_t *out) {
+    const uint64_t cur = axis_mask(dims, axis);
+    const uint64_t a = key & cur;
+    if (dir < 0) {
+        if (a == 0) return false;
+        *out = ((a - 1) & cur) | (key & ~cur);       // FindXNeg
+        return true;
+    }
+    if (a == cur) return false;
+    *out = (((key | ~cur) + 1) & cur) | (key & ~cur);  // FindXPos
+    return true;
+}
+
+__host__ __device__ inline int64_t bsearch_u64(const uint64_t *a, int64_t n, uint64_t v) {
+    int64_t lo = 0, hi = n - 1;
+    while (lo <= hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        const uint64_t x = a[mid];
+        if (x == v) return mid;
+        if (x < v) lo = mid + 1; else hi = mid - 1;
+    }
+    return -1;
+}
+
+// ----------------------------------------------------------------------------------------------
+// K0 / K1: slot lookup and neighbour tables
+// ----------------------------------------------------------------------------------------------
+__global__ void k_node_slots(const uint64_t *keys, int64_t n, const uint64_t *bkeys, int64_t nbrick,
+                             int32_t *node_slot, unsigned long long *occ, int32_t *slot_node) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = keys[i];
+    const int64_t b = bsearch_u64(bkeys, nbrick, k >> 6);
+    if (b < 0) { node_slot[i] = -1; return; }
+    const int lane = (int)(k & 63ull);
+    const int64_t slot = b * 64 + lane;
+    node_slot[i] = (int32_t)slot;
+    slot_node[slot] = (int32_t)i;
+    atomicOr(&occ[b], 1ull << lane);
+}
+
+__global__ void k_brick_nbrs(int dims, const uint64_t *bkeys, int64_t nbrick, int32_t *nbr) {
+    const int nn = dims == 3 ? 27 : 9;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nbrick * nn) return;
+    const int64_t b = t / nn;
+    const int k = (int)(t % nn);
+    int o[3] = {k % 3 - 1, (k / 3) % 3 - 1, dims == 3 ? k / 9 - 1 : 0};
+    uint64_t key = bkeys[b];
+    bool ok = true;
+    for (int a = 0; a < dims && ok; ++a)
+        if (o[a] != 0) ok = step_axis(dims, key, a, o[a], &key);
+    nbr[t] = ok ? (int32_t)bsearch_u64(bkeys, nbrick, key) : -1;
+}
+
+template <int D>
+__global__ void k_missing(const int32_t *node_slot, int64_t n, const unsigned long long *occ,
+                          const int32_t *nbr, uint32_t *missing) {
+    using L = Lattice<D>;
+    using B = Brick<D>;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t slot = node_slot[i];
+    if (slot < 0) { missing[i] = 0xffffffffu; return; }
+    const int64_t b = slot >> 6;
+    const int lane = (int)(slot & 63);
+    const int lx = B::lane_coord(lane, 0), ly = B::lane_coord(lane, 1), lz = D == 3 ? B::lane_coord(lane, 2) : 0;
+    uint32_t m = 0;
+    for (int q = 1; q < L::Q; ++q) {
+        const int sx = lx - L::cx[q], sy = ly - L::cy[q], sz = lz - L::cz[q];
+        const int ox = sx < 0 ? -1 : (sx >= B::edge ? 1 : 0);
+        const int oy = sy < 0 ? -1 : (sy >= B::edge ? 1 : 0);
+        const int oz = D == 3 ? (sz < 0 ? -1 : (sz >= B::edge ? 1 : 0)) : 0;
+        const int k = D == 3 ? (oz + 1) * 9 + (oy + 1) * 3 + (ox + 1) : (oy + 1) * 3 + (ox + 1);
+        const int sb = nbr[b * B::nnbr + k];
+        const int sl = B::lane_of(sx & (B::edge - 1), sy & (B::edge - 1), sz & (B::edge - 1));
+        if (sb < 0 || !((occ[sb] >> sl) & 1ull)) m |= 1u << q;
+    }
+    missing[i] = m;
+}
+
+template <int D>
+__global__ void k_source_table(const int32_t *node_slot, int64_t n, const unsigned long long *occ,
+                               const int32_t *nbr, const int32_t *slot_node, int32_t *src) {
+    using L = Lattice<D>;
+    using B = Brick<D>;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t slot = node_slot[i];
+    const int64_t b = slot >> 6;
+    const int lane = (int)(slot & 63);
+    const int lx = B::lane_coord(lane, 0), ly = B::lane_coord(lane, 1), lz = D == 3 ? B::lane_coord(lane, 2) : 0;
+    src[i * L::Q] = (int32_t)i;
+    for (int q = 1; q < L::Q; ++q) {
+        const int sx = lx - L::cx[q], sy = ly - L::cy[q], sz = lz - L::cz[q];
+        const int ox = sx < 0 ? -1 : (sx >= B::edge ? 1 : 0);
+        const int oy = sy < 0 ? -1 : (sy >= B::edge ? 1 : 0);
+        const int oz = D == 3 ? (sz < 0 ? -1 : (sz >= B::edge ? 1 : 0)) : 0;
+        const int k = D == 3 ? (oz + 1) * 9 + (oy + 1) * 3 + (ox + 1) : (oy + 1) * 3 + (ox + 1);
+        const int sb = nbr[b * B::nnbr + k];
+        const int sl = B::lane_of(sx & (B::edge - 1), sy & (B::edge - 1), sz & (B::edge - 1));
+        src[i * L::Q + q] = (sb < 0 || !((occ[sb] >> sl) & 1ull)) ? -1 : slot_node[(int64_t)sb * 64 + sl];
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// state upload / materialisation
+// ----------------------------------------------------------------------------------------------
+// f_in is [n][Q] double in key order (unshifted).  Aux nodes store it; bulk nodes store it in
+// the "previous" buffer (for materialisation before the first step) and their first collision
+// into the buffer the first fused kernel reads.
+template <int D, typename T, typename C, bool SHIFT, int COLL>
+__global__ void k_upload(BrickLevel<T> lv, const double *f_in, int64_t n, const int32_t *node_slot,
+                         const int32_t *node_aux, T *fa, T *fb, T *aux_f, StepParams p) {
+    constexpr int Q = Lattice<D>::Q;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    C f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) f[q] = C(f_in[i * Q + q] - (SHIFT ? Lattice<D>::w[q] : 0.));
+    const int32_t ai = node_aux[i];
+    if (ai >= 0) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) aux_f[q * lv.naux + ai] = T(f[q]);
+        return;
+    }
+    const int64_t slot = node_slot[i];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) fb[q * lv.nslot + slot] = T(f[q]);
+    C F[3] = {C(p.const_force[0]), C(p.const_force[1]), C(p.const_force[2])};
+    C rho, u[3];
+    collide<D, SHIFT, COLL, C>(f, F, p, rho, u);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) fa[q * lv.nslot + slot] = T(f[q]);
+}
+
+// post-stream populations of every node, [Q][n] double in key order (unshifted)
+template <int D, typename T, bool SHIFT>
+__global__ void k_materialize(BrickLevel<T> lv, const int32_t *node_slot, const int32_t *node_aux, int64_t n,
+                              const T *last, const T *aux_f, int fresh, double *out) {
+    using L = Lattice<D>;
+    using B = Brick<D>;
+    constexpr int Q = L::Q;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t ai = node_aux[i];
+    const int64_t slot = node_slot[i];
+    const int64_t b = slot >> 6;
+    const int lane = (int)(slot & 63);
+    const int lx = B::lane_coord(lane, 0), ly = B::lane_coord(lane, 1), lz = D == 3 ? B::lane_coord(lane, 2) : 0;
+    for (int q = 0; q < Q; ++q) {
+        double v;
+        if (ai >= 0) {
+            v = double(aux_f[q * lv.naux + ai]);
+        } else if (fresh || q == 0) {
+            v = double(last[q * lv.nslot + slot]);
+        } else {
+            const int sx = lx - L::cx[q], sy = ly - L::cy[q], sz = lz - L::cz[q];
+            const int ox = sx < 0 ? -1 : (sx >= B::edge ? 1 : 0);
+            const int oy = sy < 0 ? -1 : (sy >= B::edge ? 1 : 0);
+            const int oz = D == 3 ? (sz < 0 ? -1 : (sz >= B::edge ? 1 : 0)) : 0;
+            const int k = D == 3 ? (oz + 1) * 9 + (oy + 1) * 3 + (ox + 1) : (oy + 1) * 3 + (ox + 1);
+            const int64_t sb = lv.nbr[b * B::nnbr + k];
+            const int sl = B::lane_of(sx & (B::edge - 1), sy & (B::edge - 1), sz & (B::edge - 1));
+            v = double(last[q * lv.nslot + sb * 64 + sl]);
+        }
+        out[q * n + i] = v + (SHIFT ? L::w[q] : 0.);
+    }
+}
+
+__global__ void k_sum4(const double *fq, int64_t n, int Q, const int *cx, const int *cy, const int *cz,
+                       double *acc) {
+    __shared__ double sh[4][256];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0, jx = 0, jy = 0, jz = 0;
+    if (i < n) {
+        for (int q = 0; q < Q; ++q) {
+            const double f = fq[q * n + i];
+            m += f; jx += cx[q] * f; jy += cy[q] * f; jz += cz[q] * f;
+        }
+    }
+    sh[0][threadIdx.x] = m; sh[1][threadIdx.x] = jx; sh[2][threadIdx.x] = jy; sh[3][threadIdx.x] = jz;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int c = 0; c < 4; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 4; ++c) atomicAdd(&acc[c], sh[c][0]);
+}
+
+// ----------------------------------------------------------------------------------------------
+// Domain boundary conditions on aux nodes, one launch per direction in the reference order
+// (ComputeDomainBoundaryCondition, lbm_grid_interface.cpp:225-314)
+// ----------------------------------------------------------------------------------------------
+struct BcIdx {
+    int n;
+    int ind[8];
+    int inv[8];
+};
+
+// BoundaryBounceBack{2D,3D} (lbm_bounce_back.cpp:29-91): f[in] = f_collide[out] - 6 rho0 w (c.u_w)
+template <int D, typename T>
+__global__ void k_bc_bounce_back(int64_t nb, const int32_t *node_ai, const int32_t *node_slot, const T *fa,
+                                 int64_t nslot, T *aux_f, int64_t naux, BcIdx idx, double rho0, double ux,
+                                 double uy, double uz) {
+    using L = Lattice<D>;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb) return;
+    const int64_t ai = node_ai[t], slot = node_slot[t];
+    for (int k = 0; k < idx.n; ++k) {
+        const int o = idx.inv[k];
+        const double cu = L::cx[o] * ux + L::cy[o] * uy + (D == 3 ? L::cz[o] * uz : 0.);
+        aux_f[idx.ind[k] * naux + ai] = T(double(fa[o * nslot + slot]) - 6. * rho0 * L::w[o] * cu);
+    }
+}
+
+// BoundaryPeriodic{2D,3D} (lbm_periodic.cpp:22-133): incoming directions copied from the twin
+template <typename T>
+__global__ void k_bc_periodic(int64_t nb, const int32_t *node_ai, const int32_t *twin_ai, T *aux_f,
+                              int64_t naux, BcIdx idx) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb) return;
+    const int64_t ai = node_ai[t], tj = twin_ai[t];
+    if (tj < 0) return;
+    for (int k = 0; k < idx.n; ++k) aux_f[idx.ind[k] * naux + ai] = aux_f[idx.ind[k] * naux + tj];
+}
+
+// BoundaryNonEqExtrapolation{2D,3D} (lbm_noneq_extrapolation.cpp:29-144):
+// f = feq(rho0, u_b) + f(inner) - feq(rho, u of inner)
+template <int D, typename T, typename C, bool SHIFT>
+__global__ void k_bc_nee(int64_t nb, const int32_t *node_ai, const int32_t *inner_ai, T *aux_f, int64_t naux,
+                         const double *aux_force, StepParams p, double rho0, double ux, double uy, double uz) {
+    constexpr int Q = Lattice<D>::Q;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb) return;
+    const int64_t ai = node_ai[t], j = inner_ai[t];
+    if (j < 0) return;
+    C f[Q];
+    for (int q = 0; q < Q; ++q) f[q] = C(aux_f[q * naux + j]);
+    C F[3] = {C(p.const_force[0]), C(p.const_force[1]), C(p.const_force[2])};
+    if (p.has_force && aux_force) {
+        F[0] += C(aux_force[j * 3]); F[1] += C(aux_force[j * 3 + 1]); F[2] += C(aux_force[j * 3 + 2]);
+    }
+    C rho, drho, u[3];
+    macro<D, SHIFT, C>(f, F, C(p.dt), p.has_force, p.compressible, rho, drho, u);
+    const C usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const C rb = C(rho0), drb = C(rho0 - 1.);
+    const C ubx = C(ux), uby = C(uy), ubz = C(D == 3 ? uz : 0.);
+    const C ubsq = ubx * ubx + uby * uby + ubz * ubz;
+    for (int q = 0; q < Q; ++q) {
+        const C fb = feq_stored<D, SHIFT, C>(q, rb, drb, ubx, uby, ubz, ubsq, p.compressible);
+        const C fi = feq_stored<D, SHIFT, C>(q, rho, drho, u[0], u[1], u[2], usq, p.compressible);
+        aux_f[q * naux + ai] = T(fb + f[q] - fi);
+    }
+}
+
+}  // namespace zlh
+
+// ==============================================================================================
+// host context
+// ==============================================================================================
+using namespace zlh;
+
+namespace {
+
+#define ZLH_TRY(expr)                                                                         \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return ctx->fail(ZLH_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    template <typename X> X *as() const { return static_cast<X *>(p); }
+};
+
+struct IbMarkers {
+    std::vector<double> xyz, area, vel;
+    int stencil_dis = 2;
+    double stiffness = 2.;
+    int64_t n = 0;
+};
+
+struct LevelState {
+    // host copies
+    std::vector<uint64_t> keys;
+    std::vector<int32_t> flags;
+    std::map<std::pair<int, int>, std::vector<uint64_t>> layers;
+    IbMarkers ib;
+    bool have = false;
+    // sizes
+    int64_t n = 0, nbrick = 0, naux = 0;
+    std::vector<uint64_t> bkeys;
+    std::vector<int32_t> node_slot_h, node_aux_h;
+    // device
+    DevBuf keys_d, bkeys_d, node_slot_d, slot_node_d, occ_d, nbr_d, auxmask_d, auxbase_d, aux_missing_d,
+        aux_slot_d, node_aux_d, f_d[2], aux_f_d, aux_macro_d, aux_force_d, scratch_d;
+    // boundary tables per direction: aux index of node, partner aux index, slot
+    DevBuf bc_ai_d[7], bc_partner_d[7], bc_slot_d[7];
+    int64_t bc_n[7] = {0, 0, 0, 0, 0, 0, 0};
+    // transfer / IB tables
+    TransferTables xfer;
+    IbTables ibt;
+    int cur = 0;       // buffer read by the next fused kernel
+    bool fresh = true; // no step since upload
+    StepParams sp{};
+};
+
+}  // namespace
+
+struct zlh_ctx {
+    zlh_solver_desc d{};
+    int q = 0;
+    hipStream_t stream = nullptr;
+    std::vector<LevelState> lv;
+    bool built = false, initialised = false;
+    std::string err;
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    double timing_ms = 0.;
+    int64_t timing_launches = 0;
+    DevBuf lattice_d;
+
+    zlh_status fail(zlh_status code, const char *fmt, ...) {
+        char buf[1024];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+    bool fp64() const { return d.precision == ZLH_FP64; }
+    size_t tsize() const { return fp64() ? 8 : 4; }
+};
+
+#include "amr_ib.hpp"
+
+namespace {
+
+zlh_status dalloc(zlh_ctx *ctx, DevBuf &b, size_t bytes, bool zero = true) {
+    if (b.p) { (void)hipFree(b.p); b.p = nullptr; }
+    b.bytes = bytes;
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) { b.p = nullptr; return ctx->fail(ZLH_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e)); }
+    if (zero) ZLH_TRY(hipMemsetAsync(b.p, 0, bytes, ctx->stream));
+    return ZLH_OK;
+}
+
+template <typename X>
+zlh_status upload(zlh_ctx *ctx, DevBuf &b, const std::vector<X> &v) {
+    zlh_status s = dalloc(ctx, b, v.size() * sizeof(X), false);
+    if (s) return s;
+    if (!v.empty()) ZLH_TRY(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice, ctx->stream));
+    ZLH_TRY(hipStreamSynchronize(ctx->stream));
+    return ZLH_OK;
+}
+
+inline unsigned blocks(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+StepParams level_params(const zlh_ctx *ctx, int level) {
+    StepParams p{};
+    const double nu = ctx->d.nu0;
+    p.dt = 1. / (double)(1u << level);
+    p.tau0 = nu * 3. / p.dt + 0.5;   // CalRelaxationTime (lbm_collision_models.cpp:114-118)
+    p.cs18 = 18. * ctx->d.cs * ctx->d.cs;
+    for (int a = 0; a < 3; ++a) p.const_force[a] = ctx->d.has_force ? ctx->d.const_force[a] : 0.;
+    p.les = ctx->d.les == ZLH_LES_SMAGORINSKY ? 1 : 0;
+    p.les_tau_mode = ctx->d.les_tau_mode;
+    p.has_force = ctx->d.has_force;
+    p.compressible = ctx->d.compressible;
+    return p;
+}
+
+template <typename T>
+BrickLevel<T> view(const LevelState &L) {
+    BrickLevel<T> v;
+    v.nbrick = L.nbrick;
+    v.nslot = L.nbrick * 64;
+    v.naux = L.naux;
+    v.occ = L.occ_d.as<uint64_t>();
+    v.nbr = L.nbr_d.as<int32_t>();
+    v.auxmask = L.auxmask_d.as<uint64_t>();
+    v.auxbase = L.auxbase_d.as<int32_t>();
+    v.aux_missing = L.aux_missing_d.as<uint32_t>();
+    v.aux_slot = L.aux_slot_d.as<int32_t>();
+    return v;
+}
+
+// incoming / inverse direction lists of a face (GetBoundaryNInverseIndices,
+// lbm_boundary_conditions.cpp:102-156)
+BcIdx bc_indices(int dims, int dir) {
+    BcIdx r{};
+    const int axis = (dir - 1) / 2;
+    const int sign_in = (dir % 2 == 1) ? 1 : -1;
+    const int Q = dims == 2 ? 9 : 19;
+    for (int q = 1; q < Q; ++q) {
+        const int c = dims == 2 ? (axis == 0 ? Lattice<2>::cx[q] : Lattice<2>::cy[q])
+                                : (axis == 0 ? Lattice<3>::cx[q] : axis == 1 ? Lattice<3>::cy[q] : Lattice<3>::cz[q]);
+        if (c == sign_in) {
+            r.ind[r.n] = q;
+            r.inv[r.n] = dims == 2 ? opposite<2>(q) : opposite<3>(q);
+            ++r.n;
+        }
+    }
+    return r;
+}
+
+int64_t find_key(const std::vector<uint64_t> &keys, uint64_t k) {
+    auto it = std::lower_bound(keys.begin(), keys.end(), k);
+    return (it != keys.end() && *it == k) ? (int64_t)(it - keys.begin()) : -1;
+}
+
+}  // namespace
+
+// ==============================================================================================
+// templated dispatch helpers
+// ==============================================================================================
+namespace {
+
+template <int D, typename T, typename C, bool SHIFT>
+zlh_status launch_step_kernels(zlh_ctx *ctx, int level, int coll) {
+    LevelState &L = ctx->lv[level];
+    BrickLevel<T> v = view<T>(L);
+    T *fa = L.f_d[L.cur].as<T>(), *fb = L.f_d[L.cur ^ 1].as<T>();
+    T *aux_f = L.aux_f_d.as<T>();
+    const double *aux_force = L.aux_force_d.as<double>();
+    if (L.naux > 0) {
+        if (coll == ZLH_SRT)
+            hipLaunchKernelGGL((k_aux_collide<D, T, C, SHIFT, 0>), dim3(blocks(L.naux)), dim3(256), 0, ctx->stream,
+                               v, aux_f, aux_force, fa, L.aux_macro_d.as<double>(), L.sp);
+        else
+            hipLaunchKernelGGL((k_aux_collide<D, T, C, SHIFT, 1>), dim3(blocks(L.naux)), dim3(256), 0, ctx->stream,
+                               v, aux_f, aux_force, fa, L.aux_macro_d.as<double>(), L.sp);
+    }
+    const int64_t groups = (L.nbrick + 3) / 4;
+    const unsigned grid = (unsigned)(((groups + 7) / 8) * 8);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ctx->timing) {
+        ZLH_TRY(hipEventCreate(&e0));
+        ZLH_TRY(hipEventCreate(&e1));
+        ZLH_TRY(hipEventRecord(e0, ctx->stream));
+    }
+    if (coll == ZLH_SRT)
+        hipLaunchKernelGGL((k_stream_collide<D, T, C, SHIFT, 0>), dim3(grid), dim3(256), 0, ctx->stream, v, fa, fb,
+                           aux_f, L.sp);
+    else
+        hipLaunchKernelGGL((k_stream_collide<D, T, C, SHIFT, 1>), dim3(grid), dim3(256), 0, ctx->stream, v, fa, fb,
+                           aux_f, L.sp);
+    if (ctx->timing) {
+        ZLH_TRY(hipEventRecord(e1, ctx->stream));
+        ctx->events.emplace_back(e0, e1);
+    }
+    ZLH_TRY(hipGetLastError());
+    // domain boundaries in the reference order: every minimum face, then every maximum face
+    const int dims = D;
+    int order[6];
+    for (int a = 0; a < dims; ++a) { order[a] = 1 + 2 * a; order[dims + a] = 2 + 2 * a; }
+    for (int k = 0; k < 2 * dims; ++k) {
+        const int dir = order[k];
+        if (L.bc_n[dir] == 0) continue;
+        const BcIdx idx = bc_indices(dims, dir);
+        const double *bv = ctx->d.bc_value[dir];
+        switch (ctx->d.bc_scheme[dir]) {
+            case ZLH_BC_BOUNCE_BACK:
+                hipLaunchKernelGGL((k_bc_bounce_back<D, T>), dim3(blocks(L.bc_n[dir])), dim3(256), 0, ctx->stream,
+                                   L.bc_n[dir], L.bc_ai_d[dir].as<int32_t>(), L.bc_slot_d[dir].as<int32_t>(), fa,
+                                   L.nbrick * 64, aux_f, L.naux, idx, ctx->d.rho0, bv[0], bv[1], bv[2]);
+                break;
+            case ZLH_BC_PERIODIC:
+                hipLaunchKernelGGL((k_bc_periodic<T>), dim3(blocks(L.bc_n[dir])), dim3(256), 0, ctx->stream,
+                                   L.bc_n[dir], L.bc_ai_d[dir].as<int32_t>(), L.bc_partner_d[dir].as<int32_t>(),
+                                   aux_f, L.naux, idx);
+                break;
+            case ZLH_BC_NEE:
+                hipLaunchKernelGGL((k_bc_nee<D, T, C, SHIFT>), dim3(blocks(L.bc_n[dir])), dim3(256), 0, ctx->stream,
+                                   L.bc_n[dir], L.bc_ai_d[dir].as<int32_t>(), L.bc_partner_d[dir].as<int32_t>(),
+                                   aux_f, L.naux, aux_force, L.sp, ctx->d.rho0, bv[0], bv[1], bv[2]);
+                break;
+            default:
+                break;
+        }
+    }
+    ZLH_TRY(hipGetLastError());
+    L.cur ^= 1;
+    L.fresh = false;
+    return ZLH_OK;
+}
+
+template <int D, typename T, typename C, bool SHIFT>
+zlh_status dispatch_upload(zlh_ctx *ctx, int level, const double *f_dev) {
+    LevelState &L = ctx->lv[level];
+    BrickLevel<T> v = view<T>(L);
+    T *fa = L.f_d[L.cur].as<T>(), *fb = L.f_d[L.cur ^ 1].as<T>();
+    const int coll = ctx->d.collision[level];
+    if (coll == ZLH_SRT)
+        hipLaunchKernelGGL((k_upload<D, T, C, SHIFT, 0>), dim3(blocks(L.n)), dim3(256), 0, ctx->stream, v, f_dev, L.n,
+                           L.node_slot_d.as<int32_t>(), L.node_aux_d.as<int32_t>(), fa, fb, L.aux_f_d.as<T>(), L.sp);
+    else
+        hipLaunchKernelGGL((k_upload<D, T, C, SHIFT, 1>), dim3(blocks(L.n)), dim3(256), 0, ctx->stream, v, f_dev, L.n,
+                           L.node_slot_d.as<int32_t>(), L.node_aux_d.as<int32_t>(), fa, fb, L.aux_f_d.as<T>(), L.sp);
+    ZLH_TRY(hipGetLastError());
+    L.fresh = true;
+    return ZLH_OK;
+}
+
+template <int D, typename T, bool SHIFT>
+zlh_status dispatch_materialize(zlh_ctx *ctx, int level, double *out_dev) {
+    LevelState &L = ctx->lv[level];
+    BrickLevel<T> v = view<T>(L);
+    // the buffer read by the last fused kernel (or, before any step, the uploaded populations)
+    const T *last = L.f_d[L.cur ^ 1].as<T>();
+    hipLaunchKernelGGL((k_materialize<D, T, SHIFT>), dim3(blocks(L.n)), dim3(256), 0, ctx->stream, v,
+                       L.node_slot_d.as<int32_t>(), L.node_aux_d.as<int32_t>(), L.n, last, L.aux_f_d.as<T>(),
+                       L.fresh ? 1 : 0, out_dev);
+    ZLH_TRY(hipGetLastError());
+    return ZLH_OK;
+}
+
+zlh_status do_step_kernels(zlh_ctx *ctx, int level) {
+    const int coll = ctx->d.collision[level];
+    if (ctx->d.dims == 2) {
+        return ctx->fp64() ? launch_step_kernels<2, double, double, false>(ctx, level, coll)
+                           : launch_step_kernels<2, float, float, true>(ctx, level, coll);
+    }
+    return ctx->fp64() ? launch_step_kernels<3, double, double, false>(ctx, level, coll)
+                       : launch_step_kernels<3, float, float, true>(ctx, level, coll);
+}
+
+zlh_status do_upload(zlh_ctx *ctx, int level, const double *f_dev) {
+    if (ctx->d.dims == 2)
+        return ctx->fp64() ? dispatch_upload<2, double, double, false>(ctx, level, f_dev)
+                           : dispatch_upload<2, float, float, true>(ctx, level, f_dev);
+    return ctx->fp64() ? dispatch_upload<3, double, double, false>(ctx, level, f_dev)
+                       : dispatch_upload<3, float, float, true>(ctx, level, f_dev);
+}
+
+zlh_status do_materialize(zlh_ctx *ctx, int level, double *out_dev) {
+    if (ctx->d.dims == 2)
+        return ctx->fp64() ? dispatch_materialize<2, double, false>(ctx, level, out_dev)
+                           : dispatch_materialize<2, float, true>(ctx, level, out_dev);
+    return ctx->fp64() ? dispatch_materialize<3, double, false>(ctx, level, out_dev)
+                       : dispatch_materialize<3, float, true>(ctx, level, out_dev);
+}
+
+// feq of given macros (double, unshifted) [n][Q]
+void host_feq(int dims, double rho, const double *u, int compressible, double *feq) {
+    const int Q = dims == 2 ? 9 : 19;
+    const double ux = u[0], uy = u[1], uz = dims == 3 ? u[2] : 0.;
+    const double usq = ux * ux + uy * uy + uz * uz;
+    for (int q = 0; q < Q; ++q) {
+        const double cx = dims == 2 ? Lattice<2>::cx[q] : Lattice<3>::cx[q];
+        const double cy = dims == 2 ? Lattice<2>::cy[q] : Lattice<3>::cy[q];
+        const double cz = dims == 2 ? 0. : Lattice<3>::cz[q];
+        const double w = dims == 2 ? Lattice<2>::w[q] : Lattice<3>::w[q];
+        const double cu = ux * cx + uy * cy + uz * cz;
+        feq[q] = compressible ? w * rho * (1. + 3. * cu + 4.5 * cu * cu - 1.5 * usq)
+                              : w * (rho + 3. * cu + 4.5 * cu * cu - 1.5 * usq);
+    }
+}
+
+}  // namespace
+
+// ==============================================================================================
+// build: device tables + host classification
+// ==============================================================================================
+namespace {
+
+zlh_status build_level_tables(zlh_ctx *ctx, int level) {
+    LevelState &L = ctx->lv[level];
+    const int dims = ctx->d.dims;
+    const int Q = ctx->q;
+    const int NN = dims == 3 ? 27 : 9;
+    L.n = (int64_t)L.keys.size();
+    // brick keys: keys are sorted and the brick key is a prefix of the Morton code, so the
+    // brick keys of sorted nodes are non-decreasing
+    L.bkeys.clear();
+    for (uint64_t k : L.keys)
+        if (L.bkeys.empty() || L.bkeys.back() != (k >> 6)) L.bkeys.push_back(k >> 6);
+    L.nbrick = (int64_t)L.bkeys.size();
+    const int64_t nslot = L.nbrick * 64;
+    if (nslot >= ((int64_t)1 << 31) / 19) return ctx->fail(ZLH_EINVAL, "level %d too large (%lld slots)", level, (long long)nslot);
+    zlh_status s;
+    if ((s = upload(ctx, L.keys_d, L.keys))) return s;
+    if ((s = upload(ctx, L.bkeys_d, L.bkeys))) return s;
+    if ((s = dalloc(ctx, L.node_slot_d, sizeof(int32_t) * L.n))) return s;
+    if ((s = dalloc(ctx, L.slot_node_d, sizeof(int32_t) * nslot))) return s;
+    ZLH_TRY(hipMemsetAsync(L.slot_node_d.p, 0xff, sizeof(int32_t) * nslot, ctx->stream));
+    if ((s = dalloc(ctx, L.occ_d, sizeof(uint64_t) * L.nbrick))) return s;
+    if ((s = dalloc(ctx, L.nbr_d, sizeof(int32_t) * L.nbrick * NN))) return s;
+    hipLaunchKernelGGL(k_node_slots, dim3(blocks(L.n)), dim3(256), 0, ctx->stream, L.keys_d.as<uint64_t>(), L.n,
+                       L.bkeys_d.as<uint64_t>(), L.nbrick, L.node_slot_d.as<int32_t>(),
+                       L.occ_d.as<unsigned long long>(), L.slot_node_d.as<int32_t>());
+    hipLaunchKernelGGL(k_brick_nbrs, dim3(blocks(L.nbrick * NN)), dim3(256), 0, ctx->stream, dims,
+                       L.bkeys_d.as<uint64_t>(), L.nbrick, L.nbr_d.as<int32_t>());
+    ZLH_TRY(hipGetLastError());
+    DevBuf missing_d;
+    if ((s = dalloc(ctx, missing_d, sizeof(uint32_t) * L.n))) return s;
+    if (dims == 2)
+        hipLaunchKernelGGL(k_missing<2>, dim3(blocks(L.n)), dim3(256), 0, ctx->stream, L.node_slot_d.as<int32_t>(),
+                           L.n, L.occ_d.as<unsigned long long>(), L.nbr_d.as<int32_t>(), missing_d.as<uint32_t>());
+    else
+        hipLaunchKernelGGL(k_missing<3>, dim3(blocks(L.n)), dim3(256), 0, ctx->stream, L.node_slot_d.as<int32_t>(),
+                           L.n, L.occ_d.as<unsigned long long>(), L.nbr_d.as<int32_t>(), missing_d.as<uint32_t>());
+    ZLH_TRY(hipGetLastError());
+    std::vector<uint32_t> missing(L.n);
+    L.node_slot_h.resize(L.n);
+    ZLH_TRY(hipMemcpyAsync(missing.data(), missing_d.p, sizeof(uint32_t) * L.n, hipMemcpyDeviceToHost, ctx->stream));
+    ZLH_TRY(hipMemcpyAsync(L.node_slot_h.data(), L.node_slot_d.p, sizeof(int32_t) * L.n, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    ZLH_TRY(hipStreamSynchronize(ctx->stream));
+    for (int64_t i = 0; i < L.n; ++i)
+        if (L.node_slot_h[i] < 0) return ctx->fail(ZLH_EDEVICE, "slot lookup failed for node %lld", (long long)i);
+
+    // ---- classify the nodes that take the deferred path
+    std::vector<uint8_t> aux(L.n, 0);
+    for (int64_t i = 0; i < L.n; ++i)
+        if (missing[i]) aux[i] = 1;
+    // domain boundaries (CheckIfNodeOutsideCubicDomain, grid/grid_info.cpp:84-142) and partners
+    std::vector<int64_t> bnode[7], bpart[7];
+    for (int64_t i = 0; i < L.n; ++i) {
+        for (int a = 0; a < dims; ++a) {
+            const uint32_t c = axis_coord(dims, L.keys[i], a);
+            const uint32_t lo = ctx->d.dom_min[a] << level, hi = ctx->d.dom_max[a] << level;
+            for (int side = 0; side < 2; ++side) {
+                if (c != (side ? hi : lo)) continue;
+                const int dir = 1 + 2 * a + side;
+                int64_t partner = -1;
+                const int scheme = ctx->d.bc_scheme[dir];
+                if (scheme == ZLH_BC_PERIODIC) {
+                    // (key & others) | bound of the opposite face (lbm_periodic.cpp:77-133)
+                    const uint64_t tw = spread(dims, side ? lo : hi, a);
+                    partner = find_key(L.keys, (L.keys[i] & ~axis_mask(dims, a)) | tw);
+                } else if (scheme == ZLH_BC_NEE) {
+                    uint64_t ik;
+                    if (step_axis(dims, L.keys[i], a, side ? -1 : 1, &ik)) partner = find_key(L.keys, ik);
+                }
+                bnode[dir].push_back(i);
+                bpart[dir].push_back(partner);
+                aux[i] = 1;
+                if (partner >= 0) aux[partner] = 1;
+            }
+        }
+    }
+    // refinement interfaces and immersed boundary supports
+    {
+        zlh_status st = mark_transfer_nodes(ctx->d, ctx->lv, level, aux);
+        if (st) return ctx->fail(st, "interface layer tables: %s", transfer_error());
+        st = mark_ib_nodes(ctx->d, L.keys, L.ib.xyz, L.ib.n, L.ib.stencil_dis, level, aux);
+        if (st) return ctx->fail(st, "immersed boundary stencil: %s", transfer_error());
+    }
+    // aux ordering by slot
+    std::vector<int64_t> aux_nodes;
+    for (int64_t i = 0; i < L.n; ++i)
+        if (aux[i]) aux_nodes.push_back(i);
+    std::sort(aux_nodes.begin(), aux_nodes.end(),
+              [&](int64_t a, int64_t b) { return L.node_slot_h[a] < L.node_slot_h[b]; });
+    L.naux = (int64_t)aux_nodes.size();
+    std::vector<uint64_t> auxmask(L.nbrick, 0);
+    std::vector<int32_t> auxbase(L.nbrick, 0), aux_slot(L.naux);
+    std::vector<uint32_t> aux_missing(L.naux);
+    L.node_aux_h.assign(L.n, -1);
+    for (int64_t k = 0; k < L.naux; ++k) {
+        const int64_t i = aux_nodes[k];
+        const int32_t slot = L.node_slot_h[i];
+        auxmask[slot >> 6] |= 1ull << (slot & 63);
+        aux_slot[k] = slot;
+        aux_missing[k] = missing[i];
+        L.node_aux_h[i] = (int32_t)k;
+    }
+    int32_t acc = 0;
+    for (int64_t b = 0; b < L.nbrick; ++b) {
+        auxbase[b] = acc;
+        acc += __builtin_popcountll(auxmask[b]);
+    }
+    if ((s = upload(ctx, L.auxmask_d, auxmask))) return s;
+    if ((s = upload(ctx, L.auxbase_d, auxbase))) return s;
+    if ((s = upload(ctx, L.aux_slot_d, aux_slot))) return s;
+    if ((s = upload(ctx, L.aux_missing_d, aux_missing))) return s;
+    if ((s = upload(ctx, L.node_aux_d, L.node_aux_h))) return s;
+    // boundary tables
+    for (int dir = 1; dir <= 2 * dims; ++dir) {
+        std::vector<int32_t> ai, part, slot;
+        for (size_t k = 0; k < bnode[dir].size(); ++k) {
+            ai.push_back(L.node_aux_h[bnode[dir][k]]);
+            part.push_back(bpart[dir][k] >= 0 ? L.node_aux_h[bpart[dir][k]] : -1);
+            slot.push_back(L.node_slot_h[bnode[dir][k]]);
+        }
+        L.bc_n[dir] = (int64_t)ai.size();
+        if ((s = upload(ctx, L.bc_ai_d[dir], ai))) return s;
+        if ((s = upload(ctx, L.bc_partner_d[dir], part))) return s;
+        if ((s = upload(ctx, L.bc_slot_d[dir], slot))) return s;
+    }
+    // population storage
+    const size_t ts = ctx->tsize();
+    if ((s = dalloc(ctx, L.f_d[0], ts * Q * nslot))) return s;
+    if ((s = dalloc(ctx, L.f_d[1], ts * Q * nslot))) return s;
+    if ((s = dalloc(ctx, L.aux_f_d, ts * Q * std::max<int64_t>(L.naux, 1)))) return s;
+    if ((s = dalloc(ctx, L.aux_macro_d, sizeof(double) * 4 * std::max<int64_t>(L.naux, 1)))) return s;
+    if ((s = dalloc(ctx, L.aux_force_d, sizeof(double) * 3 * std::max<int64_t>(L.naux, 1)))) return s;
+    L.sp = level_params(ctx, level);
+    ZLH_TRY(hipStreamSynchronize(ctx->stream));
+    return ZLH_OK;
+}
+
+}  // namespace
+
+// ==============================================================================================
+// C ABI
+// ==============================================================================================
+extern "C" {
+
+zlh_status zlh_create(const zlh_solver_desc *desc, zlh_ctx **out) {
+    if (!desc || !out) return ZLH_EINVAL;
+    *out = nullptr;
+    if ((desc->dims != 2 && desc->dims != 3) || desc->max_level < 0 || desc->max_level > 15) return ZLH_EINVAL;
+    if (desc->precision != ZLH_FP64 && desc->precision != ZLH_FP32_SHIFTED) return ZLH_EINVAL;
+    if (!(desc->nu0 > 0.)) return ZLH_EINVAL;
+    for (int l = 0; l <= desc->max_level; ++l)
+        if (desc->collision[l] != ZLH_SRT && desc->collision[l] != ZLH_MRT) return ZLH_EINVAL;
+    if (desc->les != ZLH_LES_NONE && desc->les != ZLH_LES_SMAGORINSKY && desc->les != ZLH_LES_MLP) return ZLH_EINVAL;
+    for (int dir = 1; dir <= 2 * desc->dims; ++dir)
+        if (desc->bc_scheme[dir] < ZLH_BC_BOUNCE_BACK || desc->bc_scheme[dir] > ZLH_BC_NEE) return ZLH_EINVAL;
+    for (int a = 0; a < desc->dims; ++a)
+        if (desc->dom_min[a] < 1 || desc->dom_max[a] <= desc->dom_min[a]) return ZLH_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return ZLH_EDEVICE;
+    std::unique_ptr<zlh_ctx> ctx(new zlh_ctx());
+    ctx->d = *desc;
+    ctx->q = desc->dims == 2 ? 9 : 19;
+    ctx->lv.resize(desc->max_level + 1);
+    if (hipStreamCreate(&ctx->stream) != hipSuccess) return ZLH_EDEVICE;
+    *out = ctx.release();
+    return ZLH_OK;
+}
+
+void zlh_destroy(zlh_ctx *ctx) {
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &e : ctx->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    ctx->lv.clear();
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *zlh_last_error(const zlh_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+zlh_status zlh_set_level_nodes(zlh_ctx *ctx, int level, const uint64_t *keys, const int32_t *flags, int64_t n) {
+    if (!ctx) return ZLH_EINVAL;
+    if (level < 0 || level > ctx->d.max_level) return ctx->fail(ZLH_EINVAL, "level %d out of range", level);
+    if (n < 1 || !keys) return ctx->fail(ZLH_EINVAL, "empty level %d", level);
+    const int dims = ctx->d.dims;
+    const uint32_t top = dims == 3 ? (1u << 21) - 1 : 0xffffffffu;
+    for (int64_t i = 0; i < n; ++i) {
+        if (i > 0 && keys[i] <= keys[i - 1])
+            return ctx->fail(ZLH_EINVAL, "keys of level %d not sorted/unique at %lld", level, (long long)i);
+        for (int a = 0; a < dims; ++a) {
+            const uint32_t c = axis_coord(dims, keys[i], a);
+            if (c == 0 || c == top) return ctx->fail(ZLH_EINVAL, "node %lld has index %u on axis %d", (long long)i, c, a);
+        }
+    }
+    LevelState &L = ctx->lv[level];
+    L.keys.assign(keys, keys + n);
+    L.flags.assign(n, 0);
+    if (flags) L.flags.assign(flags, flags + n);
+    L.have = true;
+    ctx->built = false;
+    ctx->initialised = false;
+    return ZLH_OK;
+}
+
+zlh_status zlh_set_interface_layer(zlh_ctx *ctx, int level, int kind, int layer, const uint64_t *keys, int64_t n) {
+    if (!ctx) return ZLH_EINVAL;
+    if (level < 0 || level > ctx->d.max_level || kind < 0 || kind > 3 || layer < 0 || layer > 2 || n < 0 || (n && !keys))
+        return ctx->fail(ZLH_EINVAL, "bad interface layer arguments");
+    LevelState &L = ctx->lv[level];
+    if (!L.have) return ctx->fail(ZLH_ESTATE, "level %d nodes not set", level);
+    std::vector<uint64_t> v(keys, keys + n);
+    std::sort(v.begin(), v.end());
+    for (uint64_t k : v)
+        if (find_key(L.keys, k) < 0) return ctx->fail(ZLH_EINVAL, "layer node not in level %d", level);
+    L.layers[{kind, layer}] = std::move(v);
+    ctx->built = false;
+    return ZLH_OK;
+}
+
+zlh_status zlh_ib_set_markers(zlh_ctx *ctx, int level, const double *xyz, const double *area, const double *vel,
+                              int64_t n, int stencil_dis, double stiffness) {
+    if (!ctx) return ZLH_EINVAL;
+    if (level < 0 || level > ctx->d.max_level || n < 0 || stencil_dis < 1 || stencil_dis > 2 || (n && (!xyz || !area || !vel)))
+        return ctx->fail(ZLH_EINVAL, "bad immersed boundary arguments");
+    if (n && !ctx->d.has_force) return ctx->fail(ZLH_EINVAL, "immersed boundary needs has_force");
+    IbMarkers &m = ctx->lv[level].ib;
+    m.n = n;
+    m.xyz.assign(xyz, xyz + 3 * n);
+    m.vel.assign(vel, vel + 3 * n);
+    m.area.assign(area, area + n);
+    m.stencil_dis = stencil_dis;
+    m.stiffness = stiffness;
+    ctx->built = false;
+    return ZLH_OK;
+}
+
+zlh_status zlh_build(zlh_ctx *ctx) {
+    if (!ctx) return ZLH_EINVAL;
+    for (int l = 0; l <= ctx->d.max_level; ++l)
+        if (!ctx->lv[l].have) return ctx->fail(ZLH_ESTATE, "level %d nodes not set", l);
+    if (ctx->d.les == ZLH_LES_MLP) return ctx->fail(ZLH_EINVAL, "learned closure is configured through zlh_mlp_*");
+    for (int l = 0; l <= ctx->d.max_level; ++l) {
+        zlh_status s = build_level_tables(ctx, l);
+        if (s) return s;
+    }
+    for (int l = 1; l <= ctx->d.max_level; ++l) {
+        zlh_status s = build_transfer_tables(ctx->d, ctx->lv, l, ctx->stream);
+        if (s) return ctx->fail(s, "transfer tables level %d: %s", l, transfer_error());
+    }
+    for (int l = 0; l <= ctx->d.max_level; ++l) {
+        zlh_status s = build_ib_tables(ctx->d, ctx->lv[l], l, ctx->stream);
+        if (s) return ctx->fail(s, "immersed boundary tables level %d: %s", l, transfer_error());
+    }
+    ctx->built = true;
+    return ZLH_OK;
+}
+
+zlh_status zlh_upload_populations(zlh_ctx *ctx, int level, const double *f) {
+    if (!ctx) return ZLH_EINVAL;
+    if (!ctx->built) return ctx->fail(ZLH_ESTATE, "build before upload");
+    if (level < 0 || level > ctx->d.max_level || !f) return ctx->fail(ZLH_EINVAL, "bad upload arguments");
+    LevelState &L = ctx->lv[level];
+    DevBuf tmp;
+    zlh_status s = dalloc(ctx, tmp, sizeof(double) * L.n * ctx->q, false);
+    if (s) return s;
+    ZLH_TRY(hipMemcpyAsync(tmp.p, f, sizeof(double) * L.n * ctx->q, hipMemcpyHostToDevice, ctx->stream));
+    L.cur = 0;
+    if ((s = do_upload(ctx, level, tmp.as<double>()))) return s;
+    ZLH_TRY(hipStreamSynchronize(ctx->stream));
+    return ZLH_OK;
+}
+
+zlh_status zlh_init_macros(zlh_ctx *ctx, int level, const double *rho, const double *u) {
+    if (!ctx) return ZLH_EINVAL;
+    if (!ctx->built) return ctx->fail(ZLH_ESTATE, "build before init");
+    if (level < 0 || level > ctx->d.max_level || !rho || !u) return ctx->fail(ZLH_EINVAL, "bad init arguments");
+    LevelState &L = ctx->lv[level];
+    const int dims = ctx->d.dims, Q = ctx->q;
+    std::vector<double> f((size_t)L.n * Q);
+    for (int64_t i = 0; i < L.n; ++i) host_feq(dims, rho[i], u + i * dims, ctx->d.compressible, &f[i * Q]);
+    return zlh_upload_populations(ctx, level, f.data());
+}
+
+zlh_status zlh_init_feq(zlh_ctx *ctx) {
+    if (!ctx) return ZLH_EINVAL;
+    if (!ctx->built) return ctx->fail(ZLH_ESTATE, "build before init");
+    for (int l = 0; l <= ctx->d.max_level; ++l) {
+        LevelState &L = ctx->lv[l];
+        std::vector<double> rho(L.n, ctx->d.rho0), u((size_t)L.n * ctx->d.dims);
+        for (int64_t i = 0; i < L.n; ++i)
+            for (int a = 0; a < ctx->d.dims; ++a) u[i * ctx->d.dims + a] = ctx->d.u0[a];
+        zlh_status s = zlh_init_macros(ctx, l, rho.data(), u.data());
+        if (s) return s;
+    }
+    ctx->initialised = true;
+    return ZLH_OK;
+}
+
+zlh_status zlh_step_level(zlh_ctx *ctx, int level, int time_step_level) {
+    if (!ctx) return ZLH_EINVAL;
+    if (!ctx->built) return ctx->fail(ZLH_ESTATE, "build before stepping");
+    if (level < 0 || level > ctx->d.max_level) return ctx->fail(ZLH_EINVAL, "bad level");
+    zlh_status s;
+    // immersed-boundary forcing (UpdateCriterion, lbm_grid_interface.cpp:61-95)
+    if ((s = launch_ib(ctx->d, ctx->lv[level], level, ctx->stream))) return ctx->fail(s, "ib: %s", transfer_error());
+    if ((s = do_step_kernels(ctx, level))) return s;
+    // InformationFromGridOfDifferentLevel (lbm_solver_interface.cpp:224-238)
+    if (level > 0 && time_step_level % 2 == 0) {
+        if ((s = launch_transfers(ctx->d, ctx->lv, level, ctx->stream))) return ctx->fail(s, "transfer: %s", transfer_error());
+    }
+    return ZLH_OK;
+}
+
+zlh_status zlh_time_marching(zlh_ctx *ctx, int nsteps) {
+    if (!ctx) return ZLH_EINVAL;
+    // level order: a level step is followed by two substeps of the next finer level
+    std::vector<int> order;
+    std::function<void(int)> rec = [&](int l) {
+        order.push_back(l);
+        if (l < ctx->d.max_level) { rec(l + 1); rec(l + 1); }
+    };
+    rec(0);
+    for (int t = 0; t < nsteps; ++t) {
+        std::vector<int> tsl(ctx->d.max_level + 1, 0);
+        for (int l : order) {
+            ++tsl[l];
+            zlh_status s = zlh_step_level(ctx, l, tsl[l]);
+            if (s) return s;
+        }
+    }
+    return ZLH_OK;
+}
+
+zlh_status zlh_download_populations(zlh_ctx *ctx, int level, double *f) {
+    if (!ctx) return ZLH_EINVAL;
+    if (!ctx->built) return ctx->fail(ZLH_ESTATE, "not built");
+    if (level < 0 || level > ctx->d.max_level || !f) return ctx->fail(ZLH_EINVAL, "bad download arguments");
+    LevelState &L = ctx->lv[level];
+    const int Q = ctx->q;
+    DevBuf tmp;
+    zlh_status s = dalloc(ctx, tmp, sizeof(double) * L.n * Q, false);
+    if (s) return s;
+    if ((s = do_materialize(ctx, level, tmp.as<double>()))) return s;
+    std::vector<double> h((size_t)L.n * Q);
+    ZLH_TRY(hipMemcpyAsync(h.data(), tmp.p, sizeof(double) * L.n * Q, hipMemcpyDeviceToHost, ctx->stream));
+    ZLH_TRY(hipStreamSynchronize(ctx->stream));
+    for (int64_t i = 0; i < L.n; ++i)
+        for (int q = 0; q < Q; ++q) f[i * Q + q] = h[(size_t)q * L.n + i];
+    return ZLH_OK;
+}
+
+zlh_status zlh_download_macros(zlh_ctx *ctx, int level, double *rho, double *u) {
+    if (!ctx) return ZLH_EINVAL;
+    if (level < 0 || level > ctx->d.max_level || !rho || !u) return ctx->fail(ZLH_EINVAL, "bad download arguments");
+    LevelState &L = ctx->lv[level];
+    const int Q = ctx->q, dims = ctx->d.dims;
+    std::vector<double> f((size_t)L.n * Q);
+    zlh_status s = zlh_download_populations(ctx, level, f.data());
+    if (s) return s;
+    for (int64_t i = 0; i < L.n; ++i) {
+        double r = f[i * Q], j[3] = {0, 0, 0};
+        for (int q = 1; q < Q; ++q) {
+            const double v = f[i * Q + q];
+            r += v;
+            j[0] += (dims == 2 ? Lattice<2>::cx[q] : Lattice<3>::cx[q]) * v;
+            j[1] += (dims == 2 ? Lattice<2>::cy[q] : Lattice<3>::cy[q]) * v;
+            if (dims == 3) j[2] += Lattice<3>::cz[q] * v;
+        }
+        rho[i] = r;
+        for (int a = 0; a < dims; ++a) u[i * dims + a] = ctx->d.compressible ? j[a] / r : j[a];
+    }
+    return ZLH_OK;
+}
+
+zlh_status zlh_download_source_table(zlh_ctx *ctx, int level, int32_t *src) {
+    if (!ctx) return ZLH_EINVAL;
+    if (!ctx->built) return ctx->fail(ZLH_ESTATE, "not built");
+    if (level < 0 || level > ctx->d.max_level || !src) return ctx->fail(ZLH_EINVAL, "bad arguments");
+    LevelState &L = ctx->lv[level];
+    DevBuf tmp;
+    zlh_status s = dalloc(ctx, tmp, sizeof(int32_t) * L.n * ctx->q, false);
+    if (s) return s;
+    if (ctx->d.dims == 2)
+        hipLaunchKernelGGL(k_source_table<2>, dim3(blocks(L.n)), dim3(256), 0, ctx->stream, L.node_slot_d.as<int32_t>(),
+                           L.n, L.occ_d.as<unsigned long long>(), L.nbr_d.as<int32_t>(), L.slot_node_d.as<int32_t>(),
+                           tmp.as<int32_t>());
+    else
+        hipLaunchKernelGGL(k_source_table<3>, dim3(blocks(L.n)), dim3(256), 0, ctx->stream, L.node_slot_d.as<int32_t>(),
+                           L.n, L.occ_d.as<unsigned long long>(), L.nbr_d.as<int32_t>(), L.slot_node_d.as<int32_t>(),
+                           tmp.as<int32_t>());
+    ZLH_TRY(hipGetLastError());
+    ZLH_TRY(hipMemcpyAsync(src, tmp.p, sizeof(int32_t) * L.n * ctx->q, hipMemcpyDeviceToHost, ctx->stream));
+    ZLH_TRY(hipStreamSynchronize(ctx->stream));
+    return ZLH_OK;
+}
+
+zlh_status zlh_ib_download_forces(zlh_ctx *ctx, int level, double *force) {
+    if (!ctx) return ZLH_EINVAL;
+    if (level < 0 || level > ctx->d.max_level || !force) return ctx->fail(ZLH_EINVAL, "bad arguments");
+    LevelState &L = ctx->lv[level];
+    if (L.ib.n == 0) return ZLH_OK;
+    ZLH_TRY(hipMemcpyAsync(force, L.ibt.marker_force_d, sizeof(double) * 3 * L.ib.n, hipMemcpyDeviceToHost, ctx->stream));
+    ZLH_TRY(hipStreamSynchronize(ctx->stream));
+    return ZLH_OK;
+}
+
+zlh_status zlh_conservation(zlh_ctx *ctx, int level, double *out4) {
+    if (!ctx) return ZLH_EINVAL;
+    if (level < 0 || level > ctx->d.max_level || !out4) return ctx->fail(ZLH_EINVAL, "bad arguments");
+    LevelState &L = ctx->lv[level];
+    const int Q = ctx->q;
+    zlh_status s;
+    if (!ctx->lattice_d.p) {
+        std::vector<int> c(3 * 19, 0);
+        for (int q = 0; q < Q; ++q) {
+            c[q] = ctx->d.dims == 2 ? Lattice<2>::cx[q] : Lattice<3>::cx[q];
+            c[19 + q] = ctx->d.dims == 2 ? Lattice<2>::cy[q] : Lattice<3>::cy[q];
+            c[38 + q] = ctx->d.dims == 2 ? 0 : Lattice<3>::cz[q];
+        }
+        if ((s = upload(ctx, ctx->lattice_d, c))) return s;
+    }
+    DevBuf tmp, acc;
+    if ((s = dalloc(ctx, tmp, sizeof(double) * L.n * Q, false))) return s;
+    if ((s = dalloc(ctx, acc, sizeof(double) * 4))) return s;
+    if ((s = do_materialize(ctx, level, tmp.as<double>()))) return s;
+    const int *c = ctx->lattice_d.as<int>();
+    hipLaunchKernelGGL(k_sum4, dim3(blocks(L.n)), dim3(256), 0, ctx->stream, tmp.as<double>(), L.n, Q, c, c + 19,
+                       c + 38, acc.as<double>());
+    ZLH_TRY(hipGetLastError());
+    ZLH_TRY(hipMemcpyAsync(out4, acc.p, sizeof(double) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    ZLH_TRY(hipStreamSynchronize(ctx->stream));
+    return ZLH_OK;
+}
+
+zlh_status zlh_level_info(zlh_ctx *ctx, int level, int64_t *n_nodes, int64_t *n_aux, int64_t *n_bricks) {
+    if (!ctx) return ZLH_EINVAL;
+    if (level < 0 || level > ctx->d.max_level) return ctx->fail(ZLH_EINVAL, "bad level");
+    const LevelState &L = ctx->lv[level];
+    if (n_nodes) *n_nodes = (int64_t)L.keys.size();
+    if (n_aux) *n_aux = L.naux;
+    if (n_bricks) *n_bricks = L.nbrick;
+    return ZLH_OK;
+}
+
+zlh_status zlh_synchronize(zlh_ctx *ctx) {
+    if (!ctx) return ZLH_EINVAL;
+    ZLH_TRY(hipStreamSynchronize(ctx->stream));
+    return ZLH_OK;
+}
+
+zlh_status zlh_timing_enable(zlh_ctx *ctx, int enable) {
+    if (!ctx) return ZLH_EINVAL;
+    ZLH_TRY(hipStreamSynchronize(ctx->stream));
+    for (auto &e : ctx->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    ctx->events.clear();
+    ctx->timing_ms = 0.;
+    ctx->timing_launches = 0;
+    ctx->timing = enable != 0;
+    return ZLH_OK;
+}
+
+zlh_status zlh_timing_read(zlh_ctx *ctx, double *total_ms, int64_t *launches) {
+    if (!ctx) return ZLH_EINVAL;
+    ZLH_TRY(hipStreamSynchronize(ctx->stream));
+    for (auto &e : ctx->events) {
+        float ms = 0.f;
+        ZLH_TRY(hipEventElapsedTime(&ms, e.first, e.second));
+        ctx->timing_ms += ms;
+        ++ctx->timing_launches;
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    ctx->events.clear();
+    if (total_ms) *total_ms = ctx->timing_ms;
+    if (launches) *launches = ctx->timing_launches;
+    return ZLH_OK;
+}
+
+}  // extern "C"

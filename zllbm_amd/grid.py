@@ -1,0 +1,229 @@
+"""Morton-keyed grid description: node key sets, domain bounds and refinement-interface layers.
+
+This is the host-side grid ingest for the LBM time step.  The reference keeps one
+``unordered_map`` per level keyed by a 64-bit Morton code (defs_libs.h:35-45,
+grid/grid_info_interface.h:258); here a level is the sorted array of those codes plus a flag per
+node, which is what the device library builds its slot and neighbour tables from.
+
+Conventions restated from the reference:
+
+* Morton layout: x at bit 3i, y at 3i+1, z at 3i+2 (3D); x at 2i, y at 2i+1 (2D)
+  (grid/morton_aux.cpp:238-286).
+* Background node indices run over [1, N+1] per axis for a domain of N cells, i.e. both end
+  planes exist and periodic domains carry a duplicated plane (grid/grid_manager3d.cpp:105-112).
+* A level-l node has level-l indices; a background index i is i << l at level l
+  (grid/grid_manager3d.cpp:165-205).
+* Refinement interfaces: the fine level keeps three layers (0 innermost .. 2 outermost) along
+  the coarse-fine boundary; the coarse level keeps two layers (0 at the fine level's outermost
+  row, 1 one coarse spacing inside), layer counts from grid/grid_info_interface.h:207-210 and the
+  layer picture at grid/grid_manager.h:97-104.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+# node flags (NodeBitStatus, grid/grid_manager.h:64-73)
+FLAG_C2F0 = 1 << 1
+FLAG_C2F_GHOST = 1 << 2
+FLAG_F2C0 = 1 << 3
+FLAG_F2C_GHOST = 1 << 4
+
+# boundary directions (EDomainBoundaryDirection, grid/grid_enumerates.h:37-45)
+XMIN, XMAX, YMIN, YMAX, ZMIN, ZMAX = 1, 2, 3, 4, 5, 6
+# boundary schemes (ELbmBoundaryConditionScheme, boundary_conditions/lbm_boundary_conditions.h)
+BC_BOUNCE_BACK, BC_PERIODIC, BC_NEE = 1, 2, 3
+# interface layer kinds (InterfaceLayerInfo, grid/grid_info_interface.h:117-127)
+OUTER_F2C, OUTER_C2F, INNER_F2C, INNER_C2F = 0, 1, 2, 3
+
+
+_U = np.uint64
+
+
+def _spread3(v):
+    v = np.asarray(v, dtype=_U) & _U(0x1FFFFF)
+    v = (v | (v << _U(32))) & _U(0x1F00000000FFFF)
+    v = (v | (v << _U(16))) & _U(0x1F0000FF0000FF)
+    v = (v | (v << _U(8))) & _U(0x100F00F00F00F00F)
+    v = (v | (v << _U(4))) & _U(0x10C30C30C30C30C3)
+    v = (v | (v << _U(2))) & _U(0x1249249249249249)
+    return v
+
+
+def _compact3(v):
+    v = np.asarray(v, dtype=_U) & _U(0x1249249249249249)
+    v = (v | (v >> _U(2))) & _U(0x10C30C30C30C30C3)
+    v = (v | (v >> _U(4))) & _U(0x100F00F00F00F00F)
+    v = (v | (v >> _U(8))) & _U(0x1F0000FF0000FF)
+    v = (v | (v >> _U(16))) & _U(0x1F00000000FFFF)
+    v = (v | (v >> _U(32))) & _U(0x1FFFFF)
+    return v
+
+
+def _spread2(v):
+    v = np.asarray(v, dtype=_U) & _U(0xFFFFFFFF)
+    v = (v | (v << _U(16))) & _U(0x0000FFFF0000FFFF)
+    v = (v | (v << _U(8))) & _U(0x00FF00FF00FF00FF)
+    v = (v | (v << _U(4))) & _U(0x0F0F0F0F0F0F0F0F)
+    v = (v | (v << _U(2))) & _U(0x3333333333333333)
+    v = (v | (v << _U(1))) & _U(0x5555555555555555)
+    return v
+
+
+def _compact2(v):
+    v = np.asarray(v, dtype=_U) & _U(0x5555555555555555)
+    v = (v | (v >> _U(1))) & _U(0x3333333333333333)
+    v = (v | (v >> _U(2))) & _U(0x0F0F0F0F0F0F0F0F)
+    v = (v | (v >> _U(4))) & _U(0x00FF00FF00FF00FF)
+    v = (v | (v >> _U(8))) & _U(0x0000FFFF0000FFFF)
+    v = (v | (v >> _U(16))) & _U(0xFFFFFFFF)
+    return v
+
+
+def morton_encode(dims: int, x, y, z=None) -> np.ndarray:
+    """Interleave integer index arrays into 64-bit Morton codes (SFBitsetEncoding,
+    grid/morton_aux.cpp:274-286): x at bit dims*i, y at dims*i+1, z at 3i+2."""
+    if dims == 2:
+        return _spread2(x) | (_spread2(y) << _U(1))
+    return _spread3(x) | (_spread3(y) << _U(1)) | (_spread3(z) << _U(2))
+
+
+def morton_decode(dims: int, keys) -> np.ndarray:
+    """Inverse of :func:`morton_encode`; returns an (n, dims) uint32 index array
+    (SFBitsetComputeIndices, grid/morton_aux.cpp:336-351)."""
+    keys = np.asarray(keys, dtype=_U)
+    if dims == 2:
+        cols = [_compact2(keys), _compact2(keys >> _U(1))]
+    else:
+        cols = [_compact3(keys), _compact3(keys >> _U(1)), _compact3(keys >> _U(2))]
+    return np.stack(cols, axis=-1).astype(np.uint32)
+
+
+@dataclass
+class Level:
+    """One refinement level: sorted Morton keys, flags and interface layer key sets."""
+    keys: np.ndarray
+    flags: np.ndarray
+    layers: Dict[Tuple[int, int], np.ndarray] = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(self.keys.size)
+
+
+@dataclass
+class Grid:
+    """All levels of a Morton-keyed AMR grid over a box of background cells."""
+    dims: int
+    cells: Tuple[int, ...]                 # background cells per axis (domain_size / dx)
+    levels: List[Level]
+    dom_min: Tuple[int, ...] = ()
+    dom_max: Tuple[int, ...] = ()
+
+    def __post_init__(self):
+        if not self.dom_min:
+            self.dom_min = tuple([1] * self.dims)
+        if not self.dom_max:
+            self.dom_max = tuple(1 + c for c in self.cells)
+
+    @property
+    def max_level(self) -> int:
+        return len(self.levels) - 1
+
+    def coords(self, level: int) -> np.ndarray:
+        return morton_decode(self.dims, self.levels[level].keys)
+
+
+def _box_keys(dims: int, lo: Sequence[int], hi: Sequence[int]) -> np.ndarray:
+    rng = [np.arange(lo[a], hi[a] + 1, dtype=np.uint64) for a in range(dims)]
+    mesh = np.meshgrid(*rng, indexing="ij")
+    flat = [m.ravel() for m in mesh]
+    return morton_encode(dims, *flat)
+
+
+def uniform_grid(dims: int, cells: Sequence[int]) -> Grid:
+    """Single-level grid of ``cells`` background cells per axis, indices [1, cells+1]."""
+    cells = tuple(int(c) for c in cells)
+    keys = np.sort(_box_keys(dims, [1] * dims, [c + 1 for c in cells]))
+    return Grid(dims, cells, [Level(keys, np.zeros(keys.size, dtype=np.int32))])
+
+
+def refined_box_grid(dims: int, cells: Sequence[int],
+                     boxes: Sequence[Tuple[Sequence[int], Sequence[int]]]) -> Grid:
+    """Nested box refinement.
+
+    ``boxes[k] = (lo, hi)`` is the region refined to level k+1, given in level-k node indices
+    (inclusive); each box must lie strictly inside the previous level's interior and keep at
+    least three coarse spacings to the previous box's edge (the reference's minimum extension,
+    grid/grid_manager.h:97).  The fine level covers the box's nodes at level k+1 resolution.  The
+    coarse level keeps its nodes outside the box plus its two outermost rows inside the box; the
+    fine level's three outermost rows form the fine-to-coarse layers.
+    """
+    cells = tuple(int(c) for c in cells)
+    sets: List[np.ndarray] = [_box_keys(dims, [1] * dims, [c + 1 for c in cells])]
+    prev_lo = np.full(dims, 1, dtype=np.int64)
+    prev_hi = np.asarray([c + 1 for c in cells], dtype=np.int64)
+    for k, (lo, hi) in enumerate(boxes):
+        lo = np.asarray(lo, dtype=np.int64)
+        hi = np.asarray(hi, dtype=np.int64)
+        if np.any(hi - lo < 6):
+            raise ValueError("refinement box too small")
+        if np.any(lo - prev_lo < 4) or np.any(prev_hi - hi < 4):
+            raise ValueError("refinement box %d must keep 4 spacings to the enclosing region" % k)
+        sets.append(_box_keys(dims, list(2 * lo), list(2 * hi)))
+        prev_lo, prev_hi = 2 * lo, 2 * hi
+    nlev = len(sets)
+    levels: List[Level] = []
+    # remove coarse nodes strictly inside each box beyond the two interface rows
+    for k in range(nlev):
+        keys = sets[k]
+        lay: Dict[Tuple[int, int], np.ndarray] = {}
+        fl = np.zeros(keys.size, dtype=np.int32)
+        xyz = morton_decode(dims, keys).astype(np.int64)
+        if k < nlev - 1:
+            lo, hi = (np.asarray(boxes[k][0], dtype=np.int64), np.asarray(boxes[k][1], dtype=np.int64))
+            # ring index: 0 on the box edge, 1 one row inside, ... (Chebyshev distance to edge)
+            inside = np.all((xyz >= lo) & (xyz <= hi), axis=1)
+            dist = np.min(np.concatenate([xyz - lo, hi - xyz], axis=1), axis=1)
+            keep = ~inside | (dist <= 1)
+            c2f0 = inside & (dist == 0)
+            c2f1 = inside & (dist == 1)
+            lay[(OUTER_C2F, 0)] = keys[c2f0]
+            lay[(OUTER_C2F, 1)] = keys[c2f1]
+            # layer 1 is the outmost coarse-to-fine layer and the only ghost one
+            # (InstantiateOverlapLayerOfRefinementInterface, grid/grid_manager.cpp:413-568)
+            fl[c2f1] |= FLAG_C2F_GHOST | FLAG_C2F0
+            keys, fl, xyz = keys[keep], fl[keep], xyz[keep]
+        if k > 0:
+            lo, hi = (2 * np.asarray(boxes[k - 1][0], dtype=np.int64),
+                      2 * np.asarray(boxes[k - 1][1], dtype=np.int64))
+            dist = np.min(np.concatenate([xyz - lo, hi - xyz], axis=1), axis=1)
+            for layer, d in ((2, 0), (1, 1), (0, 2)):
+                sel = dist == d
+                lay[(OUTER_F2C, layer)] = keys[sel]
+                if layer == 2:
+                    fl[sel] |= FLAG_F2C0 | FLAG_F2C_GHOST
+                elif layer == 1:
+                    fl[sel] |= FLAG_F2C_GHOST
+        order = np.argsort(keys)
+        keys, fl = keys[order], fl[order]
+        lev = Level(keys, fl)
+        for kk, v in lay.items():
+            lev.layers[kk] = np.sort(v)
+        levels.append(lev)
+    return Grid(dims, cells, levels)
+
+
+def boundary_masks(grid: Grid, level: int) -> Dict[int, np.ndarray]:
+    """Nodes on each domain face at ``level`` (CheckIfNodeOutsideCubicDomain,
+    grid/grid_info.cpp:84-142)."""
+    xyz = grid.coords(level).astype(np.int64)
+    out = {}
+    for a in range(grid.dims):
+        lo = grid.dom_min[a] << level
+        hi = grid.dom_max[a] << level
+        out[1 + 2 * a] = xyz[:, a] == lo
+        out[2 + 2 * a] = xyz[:, a] == hi
+    return out

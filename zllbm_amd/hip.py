@@ -1,0 +1,225 @@
+"""ctypes binding of the HIP library (C ABI in include/zllbm_hip.h).
+
+The library is the only compute path: :func:`lib` raises when ``libzllbm_hip.so`` is missing
+and nothing here computes populations on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+from . import grid as _grid
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libzllbm_hip.so")
+_lib = None
+
+FP64, FP32_SHIFTED = 0, 1
+SRT, MRT = 0, 1
+LES_NONE, LES_SMAGORINSKY, LES_MLP = 0, 1, 2
+LES_TAU_REFERENCE, LES_TAU_PHYSICAL = 0, 1
+
+
+class SolverDesc(C.Structure):
+    _fields_ = [
+        ("dims", C.c_int32), ("max_level", C.c_int32), ("precision", C.c_int32),
+        ("compressible", C.c_int32), ("nu0", C.c_double), ("rho0", C.c_double),
+        ("u0", C.c_double * 3), ("les", C.c_int32), ("les_tau_mode", C.c_int32),
+        ("cs", C.c_double), ("has_force", C.c_int32), ("pad0", C.c_int32),
+        ("const_force", C.c_double * 3), ("collision", C.c_int32 * 16),
+        ("bc_scheme", C.c_int32 * 7), ("pad1", C.c_int32), ("bc_value", (C.c_double * 3) * 7),
+        ("dom_min", C.c_uint32 * 3), ("dom_max", C.c_uint32 * 3),
+    ]
+
+
+EXPORTED = (
+    "zlh_create", "zlh_destroy", "zlh_last_error", "zlh_set_level_nodes", "zlh_set_interface_layer",
+    "zlh_ib_set_markers", "zlh_build", "zlh_init_feq", "zlh_init_macros", "zlh_upload_populations",
+    "zlh_step_level", "zlh_time_marching", "zlh_download_populations", "zlh_download_macros",
+    "zlh_download_source_table", "zlh_ib_download_forces", "zlh_conservation", "zlh_level_info",
+    "zlh_synchronize", "zlh_timing_enable", "zlh_timing_read",
+)
+
+
+def lib():
+    """Load the HIP library; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libzllbm_hip.so not built: run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        D = np.ctypeslib.ndpointer
+        L.zlh_create.argtypes = [C.POINTER(SolverDesc), C.POINTER(P)]
+        L.zlh_destroy.argtypes = [P]
+        L.zlh_destroy.restype = None
+        L.zlh_last_error.argtypes = [P]
+        L.zlh_last_error.restype = C.c_char_p
+        L.zlh_set_level_nodes.argtypes = [P, C.c_int, D(np.uint64), D(np.int32), C.c_int64]
+        L.zlh_set_interface_layer.argtypes = [P, C.c_int, C.c_int, C.c_int, D(np.uint64), C.c_int64]
+        L.zlh_ib_set_markers.argtypes = [P, C.c_int, D(np.float64), D(np.float64), D(np.float64), C.c_int64,
+                                         C.c_int, C.c_double]
+        L.zlh_build.argtypes = [P]
+        L.zlh_init_feq.argtypes = [P]
+        L.zlh_init_macros.argtypes = [P, C.c_int, D(np.float64), D(np.float64)]
+        L.zlh_upload_populations.argtypes = [P, C.c_int, D(np.float64)]
+        L.zlh_step_level.argtypes = [P, C.c_int, C.c_int]
+        L.zlh_time_marching.argtypes = [P, C.c_int]
+        L.zlh_download_populations.argtypes = [P, C.c_int, D(np.float64)]
+        L.zlh_download_macros.argtypes = [P, C.c_int, D(np.float64), D(np.float64)]
+        L.zlh_download_source_table.argtypes = [P, C.c_int, D(np.int32)]
+        L.zlh_ib_download_forces.argtypes = [P, C.c_int, D(np.float64)]
+        L.zlh_conservation.argtypes = [P, C.c_int, D(np.float64)]
+        L.zlh_level_info.argtypes = [P, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.zlh_synchronize.argtypes = [P]
+        L.zlh_timing_enable.argtypes = [P, C.c_int]
+        L.zlh_timing_read.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        _lib = L
+    return _lib
+
+
+def make_desc(grid: "_grid.Grid", *, nu0: float, precision: int = FP64, rho0: float = 1.0,
+              u0=(0., 0., 0.), collision: Sequence[int] = (SRT,), les: int = LES_NONE,
+              les_tau_mode: int = LES_TAU_PHYSICAL, cs: float = 0.1, has_force: bool = False,
+              const_force=(0., 0., 0.), bc: Optional[Dict[int, tuple]] = None,
+              compressible: bool = True) -> SolverDesc:
+    d = SolverDesc()
+    d.dims = grid.dims
+    d.max_level = grid.max_level
+    d.precision = precision
+    d.compressible = 1 if compressible else 0
+    d.nu0, d.rho0 = nu0, rho0
+    for a in range(3):
+        d.u0[a] = u0[a] if a < len(u0) else 0.
+        d.const_force[a] = const_force[a] if a < len(const_force) else 0.
+    d.les, d.les_tau_mode, d.cs = les, les_tau_mode, cs
+    d.has_force = 1 if has_force else 0
+    for l in range(16):
+        d.collision[l] = collision[min(l, len(collision) - 1)]
+    for dr, (scheme, val) in (bc or {}).items():
+        d.bc_scheme[dr] = scheme
+        for a in range(3):
+            d.bc_value[dr][a] = val[a] if a < len(val) else 0.
+    for a in range(grid.dims):
+        d.dom_min[a] = grid.dom_min[a]
+        d.dom_max[a] = grid.dom_max[a]
+    return d
+
+
+class Solver:
+    """One device solver over a :class:`zllbm_amd.grid.Grid` (the SolverLbmInterface +
+    GridInfoLbmInteface pair of the reference, lbm_interface.h:140-474)."""
+
+    def __init__(self, grid: "_grid.Grid", **kw):
+        self.grid = grid
+        self.dims = grid.dims
+        self.q = 9 if grid.dims == 2 else 19
+        L = lib()
+        self.desc = make_desc(grid, **kw)
+        h = C.c_void_p()
+        rc = L.zlh_create(C.byref(self.desc), C.byref(h))
+        if rc != 0:
+            raise RuntimeError("zlh_create failed with status %d" % rc)
+        self.h = h
+        for l, lev in enumerate(grid.levels):
+            self._chk(L.zlh_set_level_nodes(self.h, l, np.ascontiguousarray(lev.keys, np.uint64),
+                                            np.ascontiguousarray(lev.flags, np.int32), lev.n))
+        for l, lev in enumerate(grid.levels):
+            for (kind, layer), keys in lev.layers.items():
+                self._chk(L.zlh_set_interface_layer(self.h, l, kind, layer,
+                                                    np.ascontiguousarray(keys, np.uint64), keys.size))
+        self._ib_n = {}
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise RuntimeError("zllbm_hip: status %d: %s" % (rc, lib().zlh_last_error(self.h).decode()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().zlh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_ib(self, level, xyz, area, vel, stencil_dis=2, stiffness=2.0):
+        n = len(area)
+
+        def pad3(a):
+            a = np.asarray(a, np.float64).reshape(n, -1)
+            return np.ascontiguousarray(np.pad(a, ((0, 0), (0, 3 - a.shape[1]))).ravel())
+        self._ib_n[level] = n
+        self._chk(lib().zlh_ib_set_markers(self.h, level, pad3(xyz), np.ascontiguousarray(area, np.float64),
+                                           pad3(vel), n, stencil_dis, stiffness))
+
+    def build(self):
+        self._chk(lib().zlh_build(self.h))
+        return self
+
+    def init_feq(self):
+        self._chk(lib().zlh_init_feq(self.h))
+        return self
+
+    def init_macro(self, level, rho, u):
+        self._chk(lib().zlh_init_macros(self.h, level, np.ascontiguousarray(rho, np.float64),
+                                        np.ascontiguousarray(u, np.float64).ravel()))
+
+    def set_f(self, level, f):
+        self._chk(lib().zlh_upload_populations(self.h, level, np.ascontiguousarray(f, np.float64).ravel()))
+
+    def step_level(self, level, time_step_level):
+        self._chk(lib().zlh_step_level(self.h, level, time_step_level))
+
+    def time_marching(self, nsteps=1):
+        self._chk(lib().zlh_time_marching(self.h, nsteps))
+
+    def n(self, level=0) -> int:
+        return self.grid.levels[level].n
+
+    def info(self, level=0):
+        n, a, b = C.c_int64(), C.c_int64(), C.c_int64()
+        self._chk(lib().zlh_level_info(self.h, level, C.byref(n), C.byref(a), C.byref(b)))
+        return n.value, a.value, b.value
+
+    def f(self, level=0) -> np.ndarray:
+        out = np.zeros(self.n(level) * self.q)
+        self._chk(lib().zlh_download_populations(self.h, level, out))
+        return out.reshape(-1, self.q)
+
+    def macros(self, level=0):
+        rho = np.zeros(self.n(level))
+        u = np.zeros(self.n(level) * self.dims)
+        self._chk(lib().zlh_download_macros(self.h, level, rho, u))
+        return rho, u.reshape(-1, self.dims)
+
+    def source_table(self, level=0) -> np.ndarray:
+        out = np.zeros(self.n(level) * self.q, dtype=np.int32)
+        self._chk(lib().zlh_download_source_table(self.h, level, out))
+        return out.reshape(-1, self.q)
+
+    def ib_force(self, level) -> np.ndarray:
+        out = np.zeros(self._ib_n[level] * 3)
+        self._chk(lib().zlh_ib_download_forces(self.h, level, out))
+        return out.reshape(-1, 3)[:, :self.dims]
+
+    def conservation(self, level=0) -> np.ndarray:
+        out = np.zeros(4)
+        self._chk(lib().zlh_conservation(self.h, level, out))
+        return out
+
+    def synchronize(self):
+        self._chk(lib().zlh_synchronize(self.h))
+
+    def timing_enable(self, on=True):
+        self._chk(lib().zlh_timing_enable(self.h, 1 if on else 0))
+
+    def timing_read(self):
+        ms, n = C.c_double(), C.c_int64()
+        self._chk(lib().zlh_timing_read(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
