@@ -952,6 +952,14 @@ int zo_get_ib_force(const zo_sim *s, int level, double *force) {
     return 0;
 }
 
+/* test hook: the coarse/fine transfer of one refined level on the current populations */
+int zo_debug_transfer(zo_sim *s, int fine_level) {
+    if (fine_level < 1 || fine_level > s->p.max_level) { zo_seterr(s, "bad level"); return -1; }
+    int rc = zo_transfer_from_coarse(s, fine_level);
+    zo_transfer_to_coarse(s, fine_level);
+    return rc;
+}
+
 /* test hook: replace the non-viscous MRT relaxation rates (all levels) */
 int zo_debug_set_s_base(zo_sim *s, const double *s_base) {
     memcpy(s->s_base, s_base, sizeof(double) * s->q);

@@ -97,6 +97,8 @@ int zo_get_ib_force(const zo_sim *s, int level, double *force);   /* Lagrangian 
 /* mass and momentum sums over a level (FP64) */
 int zo_conservation(const zo_sim *s, int level, double *out4);
 
+/* test hook: run the coarse/fine transfer of one refined level on the current populations */
+int zo_debug_transfer(zo_sim *s, int fine_level);
 /* test hook: replace the non-viscous MRT relaxation rates */
 int zo_debug_set_s_base(zo_sim *s, const double *s_base);
 

@@ -77,6 +77,7 @@ def lib():
         L.zo_find_neighbor.argtypes = [C.c_int, C.c_uint64, C.c_int, C.c_int]
         L.zo_set_threads.argtypes = [C.c_int]
         L.zo_debug_set_s_base.argtypes = [P, D(np.float64)]
+        L.zo_debug_transfer.argtypes = [P, C.c_int]
         _lib = L
     return _lib
 
@@ -226,6 +227,9 @@ class Oracle:
 
     def debug_set_s_base(self, s_base):
         self._chk(lib().zo_debug_set_s_base(self.h, np.ascontiguousarray(s_base, np.float64)))
+
+    def debug_transfer(self, fine_level):
+        self._chk(lib().zo_debug_transfer(self.h, fine_level))
 
     def conservation(self, level=0) -> np.ndarray:
         out = np.zeros(4)

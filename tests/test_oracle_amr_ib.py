@@ -1,0 +1,98 @@
+"""Pins the oracle's coarse/fine transfer and immersed boundary on the CPU (closed forms)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import zlbm_oracle as zo
+from oracle.zlbm_oracle import Oracle
+from zllbm_amd import geometry, grid as g
+from tests import scenarios as sc
+
+
+def _cubic(xyz):
+    x, y = xyz[:, 0], xyz[:, 1]
+    return 1.0 + 0.01 * x - 0.002 * y + 3e-4 * x * x * y - 2e-5 * y ** 3 + 1e-5 * x ** 3
+
+
+def test_c2f_reproduces_cubic_density():
+    """Equilibrium populations at rest with a cubic density: feq = w rho is unchanged by the
+    tau rescaling, and the 4-point Lagrange weights (-1, 9, 9, -1)/16 reproduce cubics, so every
+    interpolated ghost node must equal w rho at its own position."""
+    grid, kw = sc.amr2d(levels=2)
+    o = Oracle(grid, **kw)
+    for l in range(2):
+        xyz = grid.coords(l).astype(float) / (1 << l)
+        rho = _cubic(xyz)
+        o.init_macro(l, rho, np.zeros((len(rho), 2)))
+    o.debug_transfer(1)
+    _, _, _, w, _ = zo.lattice(2)
+    f1 = o.f(1)
+    xyz1 = grid.coords(1).astype(float) / 2
+    lev = grid.levels[1]
+    for layer in (1, 2):
+        idx = np.searchsorted(lev.keys, lev.layers[(g.OUTER_F2C, layer)])
+        np.testing.assert_allclose(f1[idx], np.outer(_cubic(xyz1[idx]), w), rtol=1e-13)
+
+
+def test_f2c_rescale_closed_form():
+    """Coarse layer-1 nodes receive feq(rho_f, u_f) + tau_f2c (f_f - feq) of the coincident fine
+    node with tau_f2c = 2 tau_c / tau_f (CalRelaxationTimeRatio)."""
+    grid, kw = sc.amr2d(levels=2)
+    o = Oracle(grid, **kw)
+    rng = np.random.default_rng(0)
+    for l in range(2):
+        rho, u = sc.level_fields(grid, l, 0.05)
+        o.init_macro(l, rho, u)
+    f1 = o.f(1) + rng.normal(0, 1e-4, o.f(1).shape)
+    o.set_f(1, f1)
+    rho1, u1 = o.node_macro(1)
+    o.debug_transfer(1)
+    nu = kw["nu0"]
+    tau_c, tau_f = 3 * nu + 0.5, 3 * nu * 2 + 0.5
+    ratio = 2 * tau_c / tau_f
+    lev0, lev1 = grid.levels
+    c_keys = lev0.layers[(g.OUTER_C2F, 1)]
+    ci = np.searchsorted(lev0.keys, c_keys)
+    fi = np.searchsorted(lev1.keys, c_keys << np.uint64(2))
+    cx, cy, _, w, _ = zo.lattice(2)
+    cu = u1[fi] @ np.stack([cx, cy])
+    feq = w * rho1[fi, None] * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * (u1[fi] ** 2).sum(1)[:, None])
+    np.testing.assert_allclose(o.f(0)[ci], feq + ratio * (f1[fi] - feq), rtol=1e-14)
+
+
+def test_amr_uniform_state_is_fixed_point():
+    grid, kw = sc.amr2d(levels=3)
+    o = Oracle(grid, u0=(0.02, -0.01), **kw)
+    f0 = [o.f(l).copy() for l in range(3)]
+    o.time_marching(4)
+    for l in range(3):
+        np.testing.assert_allclose(o.f(l), f0[l], atol=1e-15)
+
+
+def test_marker_counts_and_areas():
+    xy, area = geometry.circle_markers((0, 0), 16.0, 1.0)
+    assert len(xy) == 101                                   # cylinder of Cfg 2: r = 16
+    np.testing.assert_allclose(np.hypot(xy[:, 0], xy[:, 1]), 16.0)
+    assert abs(area.sum() - 2 * math.pi * 16) < 1e-12
+    xyz, area = geometry.fibonacci_sphere_markers((0, 0, 0), 24.0, 1.0)
+    assert len(xyz) == 7239                                 # sphere of Cfg 5: r = 24
+    np.testing.assert_allclose(np.linalg.norm(xyz, axis=1), 24.0)
+    assert abs(area.sum() - 4 * math.pi * 24 ** 2) < 1e-9
+
+
+@pytest.mark.parametrize("stencil", [1, 2])
+def test_ib_force_in_uniform_flow(stencil):
+    """The discrete delta kernels sum to one, so in a uniform flow U every marker at rest gets
+    F = k (0 - U), and the spread Eulerian force totals sum(F_m area_m / dt)."""
+    grid = g.uniform_grid(2, (40, 24))
+    U = np.array([0.03, -0.01])
+    bc = sc.all_bc(2, sc.BB)
+    o = Oracle(grid, nu0=0.05, u0=tuple(U), bc=bc, has_force=True)
+    xy, area = geometry.circle_markers((17.3, 12.6), 4.0, 1.0)
+    o.set_ib(0, xy, area, np.zeros_like(xy), stencil_dis=stencil, stiffness=2.0)
+    o.advance_level(0, 1)
+    F = o.ib_force(0)
+    np.testing.assert_allclose(F, np.tile(-2.0 * U, (len(F), 1)), rtol=1e-13)
+    node_F = o.node_force(0)
+    np.testing.assert_allclose(node_F.sum(0), (F * area[:, None]).sum(0), rtol=1e-12)

@@ -1,0 +1,117 @@
+// state.hpp -- per-level device state of the time step (brick layout, deferred-node arrays,
+// boundary, transfer and immersed-boundary tables).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <map>
+#include <vector>
+#include "kernels.hpp"
+
+namespace zlh {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            if (p) (void)hipFree(p);
+            p = o.p; bytes = o.bytes;
+            o.p = nullptr; o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    template <typename X> X *as() const { return static_cast<X *>(p); }
+};
+
+// Coarse-to-fine interpolation and fine-to-coarse copy of one refined level; the tables live on
+// the fine level's state (TransferInfoFromCoarseGrid / TransferInfoToCoarseGrid,
+// lbm_grid_interface.cpp:408-564).
+struct TransferTables {
+    int64_t n_c2f = 0;          // fine ghost nodes interpolated from the coarse level
+    int64_t n_src = 0;          // coarse nodes feeding the interpolation
+    int64_t n_f2c = 0;          // coarse nodes copied from the fine level
+    DevBuf c2f_target;          // [n_c2f] fine aux index
+    DevBuf c2f_begin;           // [n_c2f + 1] offsets into the contributor list
+    DevBuf c2f_src;             // contributor: >= 0 coarse source row, < 0: -(fine aux index) - 1
+    DevBuf c2f_coeff;           // contributor weight (double)
+    DevBuf src_coarse_ai;       // [n_src] coarse aux index of each source row
+    DevBuf src_f;               // [Q][n_src] rescaled coarse populations (double, unshifted)
+    DevBuf f2c_coarse_ai;       // [n_f2c]
+    DevBuf f2c_fine_ai;         // [n_f2c] fine aux index of the coincident node
+};
+
+// Direct-forcing immersed boundary of one level (FsiImmersedBoundary::CalculateBodyForce,
+// immersed_boundary.cpp:35-384); the marker stencils are fixed, so weights are precomputed.
+struct IbTables {
+    int64_t n_markers = 0;
+    int support = 0;            // (2 * stencil)^dims entries per marker
+    int64_t n_reset = 0;
+    DevBuf node_ai;             // [n_markers][support] aux index, -1 for unused entries
+    DevBuf weight;              // [n_markers][support] delta weight (double)
+    DevBuf reset_ai;            // [n_reset] aux indices of every supported node
+    DevBuf node_u;              // [naux][3] velocity recomputed at the reset (double)
+    DevBuf marker_vel;          // [n_markers][3]
+    DevBuf marker_scale;        // [n_markers] area / (dt dx^(dims-1))
+    DevBuf marker_force;        // [n_markers][3] Lagrangian force
+    double stiffness = 2.;
+};
+
+struct IbMarkers {
+    std::vector<double> xyz, area, vel;
+    int stencil_dis = 2;
+    double stiffness = 2.;
+    int64_t n = 0;
+};
+
+// host-side plans (node indices in key order) from which the device tables are laid out
+struct TransferPlan {
+    std::vector<int64_t> tgt;          // fine node of each interpolated ghost node
+    std::vector<int64_t> begin;        // contributor offsets per target
+    std::vector<int64_t> contrib;      // >= 0 coarse node index, < 0: -(fine node index) - 1
+    std::vector<double> coeff;
+    std::vector<int64_t> f2c_coarse, f2c_fine;
+    int64_t incomplete = 0;            // stencils with a contributor absent on both levels
+};
+
+struct IbPlan {
+    std::vector<int64_t> node;         // [n_markers][support] node index or -1
+    std::vector<double> weight;
+    std::vector<double> scale;
+    int support = 0;
+};
+
+struct LevelState {
+    // host copies
+    std::vector<uint64_t> keys;
+    std::vector<int32_t> flags;
+    std::map<std::pair<int, int>, std::vector<uint64_t>> layers;
+    IbMarkers ib;
+    bool have = false;
+    // sizes and host maps
+    int64_t n = 0, nbrick = 0, naux = 0;
+    std::vector<uint64_t> bkeys;
+    std::vector<int32_t> node_slot_h, node_aux_h;
+    std::vector<uint32_t> missing_h;
+    std::vector<uint8_t> aux_mark;   // nodes that must take the deferred path
+    // device
+    DevBuf keys_d, bkeys_d, node_slot_d, slot_node_d, occ_d, nbr_d, auxmask_d, auxbase_d, aux_missing_d,
+        aux_slot_d, node_aux_d, f_d[2], aux_f_d, aux_macro_d, aux_force_d;
+    // boundary tables per direction: aux index of node, partner aux index, slot
+    DevBuf bc_ai_d[7], bc_partner_d[7], bc_slot_d[7];
+    int64_t bc_n[7] = {0, 0, 0, 0, 0, 0, 0};
+    std::vector<int64_t> bnode[7], bpart[7];
+    TransferTables xfer;
+    IbTables ibt;
+    TransferPlan xplan;
+    IbPlan ibplan;
+    int cur = 0;        // buffer read by the next fused kernel
+    bool fresh = true;  // no step since upload
+    StepParams sp{};
+};
+
+}  // namespace zlh

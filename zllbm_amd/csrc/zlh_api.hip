@@ -18,7 +18,7 @@
 #include "zllbm_hip.h"
 #include <functional>
 #include "kernels.hpp"
-#include "amr_ib_types.hpp"
+#include "state.hpp"
 
 namespace zlh {
 
@@ -320,45 +320,6 @@ namespace {
         if (e_ != hipSuccess) return ctx->fail(ZLH_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-    template <typename X> X *as() const { return static_cast<X *>(p); }
-};
-
-struct IbMarkers {
-    std::vector<double> xyz, area, vel;
-    int stencil_dis = 2;
-    double stiffness = 2.;
-    int64_t n = 0;
-};
-
-struct LevelState {
-    // host copies
-    std::vector<uint64_t> keys;
-    std::vector<int32_t> flags;
-    std::map<std::pair<int, int>, std::vector<uint64_t>> layers;
-    IbMarkers ib;
-    bool have = false;
-    // sizes
-    int64_t n = 0, nbrick = 0, naux = 0;
-    std::vector<uint64_t> bkeys;
-    std::vector<int32_t> node_slot_h, node_aux_h;
-    // device
-    DevBuf keys_d, bkeys_d, node_slot_d, slot_node_d, occ_d, nbr_d, auxmask_d, auxbase_d, aux_missing_d,
-        aux_slot_d, node_aux_d, f_d[2], aux_f_d, aux_macro_d, aux_force_d, scratch_d;
-    // boundary tables per direction: aux index of node, partner aux index, slot
-    DevBuf bc_ai_d[7], bc_partner_d[7], bc_slot_d[7];
-    int64_t bc_n[7] = {0, 0, 0, 0, 0, 0, 0};
-    // transfer / IB tables
-    TransferTables xfer;
-    IbTables ibt;
-    int cur = 0;       // buffer read by the next fused kernel
-    bool fresh = true; // no step since upload
-    StepParams sp{};
-};
-
 }  // namespace
 
 struct zlh_ctx {
@@ -386,8 +347,6 @@ struct zlh_ctx {
     bool fp64() const { return d.precision == ZLH_FP64; }
     size_t tsize() const { return fp64() ? 8 : 4; }
 };
-
-#include "amr_ib.hpp"
 
 namespace {
 
@@ -466,6 +425,8 @@ int64_t find_key(const std::vector<uint64_t> &keys, uint64_t k) {
 }
 
 }  // namespace
+
+#include "amr_ib.hpp"
 
 // ==============================================================================================
 // templated dispatch helpers
@@ -620,10 +581,10 @@ void host_feq(int dims, double rho, const double *u, int compressible, double *f
 // ==============================================================================================
 namespace {
 
-zlh_status build_level_tables(zlh_ctx *ctx, int level) {
+// phase 1: device slot / neighbour tables and the absent-source masks
+zlh_status build_level_slots(zlh_ctx *ctx, int level) {
     LevelState &L = ctx->lv[level];
     const int dims = ctx->d.dims;
-    const int Q = ctx->q;
     const int NN = dims == 3 ? 27 : 9;
     L.n = (int64_t)L.keys.size();
     // brick keys: keys are sorted and the brick key is a prefix of the Morton code, so the
@@ -633,7 +594,8 @@ zlh_status build_level_tables(zlh_ctx *ctx, int level) {
         if (L.bkeys.empty() || L.bkeys.back() != (k >> 6)) L.bkeys.push_back(k >> 6);
     L.nbrick = (int64_t)L.bkeys.size();
     const int64_t nslot = L.nbrick * 64;
-    if (nslot >= ((int64_t)1 << 31) / 19) return ctx->fail(ZLH_EINVAL, "level %d too large (%lld slots)", level, (long long)nslot);
+    if (nslot >= ((int64_t)1 << 31) / 19)
+        return ctx->fail(ZLH_EINVAL, "level %d too large (%lld slots)", level, (long long)nslot);
     zlh_status s;
     if ((s = upload(ctx, L.keys_d, L.keys))) return s;
     if ((s = upload(ctx, L.bkeys_d, L.bkeys))) return s;
@@ -657,21 +619,21 @@ zlh_status build_level_tables(zlh_ctx *ctx, int level) {
         hipLaunchKernelGGL(k_missing<3>, dim3(blocks(L.n)), dim3(256), 0, ctx->stream, L.node_slot_d.as<int32_t>(),
                            L.n, L.occ_d.as<unsigned long long>(), L.nbr_d.as<int32_t>(), missing_d.as<uint32_t>());
     ZLH_TRY(hipGetLastError());
-    std::vector<uint32_t> missing(L.n);
+    L.missing_h.resize(L.n);
     L.node_slot_h.resize(L.n);
-    ZLH_TRY(hipMemcpyAsync(missing.data(), missing_d.p, sizeof(uint32_t) * L.n, hipMemcpyDeviceToHost, ctx->stream));
+    ZLH_TRY(hipMemcpyAsync(L.missing_h.data(), missing_d.p, sizeof(uint32_t) * L.n, hipMemcpyDeviceToHost, ctx->stream));
     ZLH_TRY(hipMemcpyAsync(L.node_slot_h.data(), L.node_slot_d.p, sizeof(int32_t) * L.n, hipMemcpyDeviceToHost,
                            ctx->stream));
     ZLH_TRY(hipStreamSynchronize(ctx->stream));
     for (int64_t i = 0; i < L.n; ++i)
         if (L.node_slot_h[i] < 0) return ctx->fail(ZLH_EDEVICE, "slot lookup failed for node %lld", (long long)i);
-
-    // ---- classify the nodes that take the deferred path
-    std::vector<uint8_t> aux(L.n, 0);
+    // nodes with an absent pull source keep that population from the previous substep
+    L.aux_mark.assign(L.n, 0);
     for (int64_t i = 0; i < L.n; ++i)
-        if (missing[i]) aux[i] = 1;
-    // domain boundaries (CheckIfNodeOutsideCubicDomain, grid/grid_info.cpp:84-142) and partners
-    std::vector<int64_t> bnode[7], bpart[7];
+        if (L.missing_h[i]) L.aux_mark[i] = 1;
+    // domain boundaries (CheckIfNodeOutsideCubicDomain, grid/grid_info.cpp:84-142) and the
+    // nodes their conditions read
+    for (int d = 0; d < 7; ++d) { L.bnode[d].clear(); L.bpart[d].clear(); }
     for (int64_t i = 0; i < L.n; ++i) {
         for (int a = 0; a < dims; ++a) {
             const uint32_t c = axis_coord(dims, L.keys[i], a);
@@ -689,24 +651,26 @@ zlh_status build_level_tables(zlh_ctx *ctx, int level) {
                     uint64_t ik;
                     if (step_axis(dims, L.keys[i], a, side ? -1 : 1, &ik)) partner = find_key(L.keys, ik);
                 }
-                bnode[dir].push_back(i);
-                bpart[dir].push_back(partner);
-                aux[i] = 1;
-                if (partner >= 0) aux[partner] = 1;
+                L.bnode[dir].push_back(i);
+                L.bpart[dir].push_back(partner);
+                L.aux_mark[i] = 1;
+                if (partner >= 0) L.aux_mark[partner] = 1;
             }
         }
     }
-    // refinement interfaces and immersed boundary supports
-    {
-        zlh_status st = mark_transfer_nodes(ctx->d, ctx->lv, level, aux);
-        if (st) return ctx->fail(st, "interface layer tables: %s", transfer_error());
-        st = mark_ib_nodes(ctx->d, L.keys, L.ib.xyz, L.ib.n, L.ib.stencil_dis, level, aux);
-        if (st) return ctx->fail(st, "immersed boundary stencil: %s", transfer_error());
-    }
-    // aux ordering by slot
+    return ZLH_OK;
+}
+
+// phase 2: deferred-node arrays, boundary tables and population storage
+zlh_status build_level_aux(zlh_ctx *ctx, int level) {
+    LevelState &L = ctx->lv[level];
+    const int dims = ctx->d.dims;
+    const int Q = ctx->q;
+    const int64_t nslot = L.nbrick * 64;
+    zlh_status s;
     std::vector<int64_t> aux_nodes;
     for (int64_t i = 0; i < L.n; ++i)
-        if (aux[i]) aux_nodes.push_back(i);
+        if (L.aux_mark[i]) aux_nodes.push_back(i);
     std::sort(aux_nodes.begin(), aux_nodes.end(),
               [&](int64_t a, int64_t b) { return L.node_slot_h[a] < L.node_slot_h[b]; });
     L.naux = (int64_t)aux_nodes.size();
@@ -719,7 +683,7 @@ zlh_status build_level_tables(zlh_ctx *ctx, int level) {
         const int32_t slot = L.node_slot_h[i];
         auxmask[slot >> 6] |= 1ull << (slot & 63);
         aux_slot[k] = slot;
-        aux_missing[k] = missing[i];
+        aux_missing[k] = L.missing_h[i];
         L.node_aux_h[i] = (int32_t)k;
     }
     int32_t acc = 0;
@@ -732,20 +696,18 @@ zlh_status build_level_tables(zlh_ctx *ctx, int level) {
     if ((s = upload(ctx, L.aux_slot_d, aux_slot))) return s;
     if ((s = upload(ctx, L.aux_missing_d, aux_missing))) return s;
     if ((s = upload(ctx, L.node_aux_d, L.node_aux_h))) return s;
-    // boundary tables
     for (int dir = 1; dir <= 2 * dims; ++dir) {
         std::vector<int32_t> ai, part, slot;
-        for (size_t k = 0; k < bnode[dir].size(); ++k) {
-            ai.push_back(L.node_aux_h[bnode[dir][k]]);
-            part.push_back(bpart[dir][k] >= 0 ? L.node_aux_h[bpart[dir][k]] : -1);
-            slot.push_back(L.node_slot_h[bnode[dir][k]]);
+        for (size_t k = 0; k < L.bnode[dir].size(); ++k) {
+            ai.push_back(L.node_aux_h[L.bnode[dir][k]]);
+            part.push_back(L.bpart[dir][k] >= 0 ? L.node_aux_h[L.bpart[dir][k]] : -1);
+            slot.push_back(L.node_slot_h[L.bnode[dir][k]]);
         }
         L.bc_n[dir] = (int64_t)ai.size();
         if ((s = upload(ctx, L.bc_ai_d[dir], ai))) return s;
         if ((s = upload(ctx, L.bc_partner_d[dir], part))) return s;
         if ((s = upload(ctx, L.bc_slot_d[dir], slot))) return s;
     }
-    // population storage
     const size_t ts = ctx->tsize();
     if ((s = dalloc(ctx, L.f_d[0], ts * Q * nslot))) return s;
     if ((s = dalloc(ctx, L.f_d[1], ts * Q * nslot))) return s;
@@ -860,18 +822,19 @@ zlh_status zlh_build(zlh_ctx *ctx) {
     for (int l = 0; l <= ctx->d.max_level; ++l)
         if (!ctx->lv[l].have) return ctx->fail(ZLH_ESTATE, "level %d nodes not set", l);
     if (ctx->d.les == ZLH_LES_MLP) return ctx->fail(ZLH_EINVAL, "learned closure is configured through zlh_mlp_*");
-    for (int l = 0; l <= ctx->d.max_level; ++l) {
-        zlh_status s = build_level_tables(ctx, l);
-        if (s) return s;
-    }
-    for (int l = 1; l <= ctx->d.max_level; ++l) {
-        zlh_status s = build_transfer_tables(ctx->d, ctx->lv, l, ctx->stream);
-        if (s) return ctx->fail(s, "transfer tables level %d: %s", l, transfer_error());
-    }
-    for (int l = 0; l <= ctx->d.max_level; ++l) {
-        zlh_status s = build_ib_tables(ctx->d, ctx->lv[l], l, ctx->stream);
-        if (s) return ctx->fail(s, "immersed boundary tables level %d: %s", l, transfer_error());
-    }
+    zlh_status s;
+    for (int l = 0; l <= ctx->d.max_level; ++l)
+        if ((s = build_level_slots(ctx, l))) return s;
+    for (int l = 1; l <= ctx->d.max_level; ++l)
+        if ((s = plan_transfer(ctx, l))) return s;
+    for (int l = 0; l <= ctx->d.max_level; ++l)
+        if ((s = plan_ib(ctx, l))) return s;
+    for (int l = 0; l <= ctx->d.max_level; ++l)
+        if ((s = build_level_aux(ctx, l))) return s;
+    for (int l = 1; l <= ctx->d.max_level; ++l)
+        if ((s = upload_transfer(ctx, l))) return s;
+    for (int l = 0; l <= ctx->d.max_level; ++l)
+        if ((s = upload_ib(ctx, l))) return s;
     ctx->built = true;
     return ZLH_OK;
 }
@@ -923,11 +886,11 @@ zlh_status zlh_step_level(zlh_ctx *ctx, int level, int time_step_level) {
     if (level < 0 || level > ctx->d.max_level) return ctx->fail(ZLH_EINVAL, "bad level");
     zlh_status s;
     // immersed-boundary forcing (UpdateCriterion, lbm_grid_interface.cpp:61-95)
-    if ((s = launch_ib(ctx->d, ctx->lv[level], level, ctx->stream))) return ctx->fail(s, "ib: %s", transfer_error());
+    if ((s = launch_ib(ctx, level))) return s;
     if ((s = do_step_kernels(ctx, level))) return s;
     // InformationFromGridOfDifferentLevel (lbm_solver_interface.cpp:224-238)
     if (level > 0 && time_step_level % 2 == 0) {
-        if ((s = launch_transfers(ctx->d, ctx->lv, level, ctx->stream))) return ctx->fail(s, "transfer: %s", transfer_error());
+        if ((s = launch_transfers(ctx, level))) return s;
     }
     return ZLH_OK;
 }
@@ -1020,7 +983,7 @@ zlh_status zlh_ib_download_forces(zlh_ctx *ctx, int level, double *force) {
     if (level < 0 || level > ctx->d.max_level || !force) return ctx->fail(ZLH_EINVAL, "bad arguments");
     LevelState &L = ctx->lv[level];
     if (L.ib.n == 0) return ZLH_OK;
-    ZLH_TRY(hipMemcpyAsync(force, L.ibt.marker_force_d, sizeof(double) * 3 * L.ib.n, hipMemcpyDeviceToHost, ctx->stream));
+    ZLH_TRY(hipMemcpyAsync(force, L.ibt.marker_force.p, sizeof(double) * 3 * L.ib.n, hipMemcpyDeviceToHost, ctx->stream));
     ZLH_TRY(hipStreamSynchronize(ctx->stream));
     return ZLH_OK;
 }
