@@ -98,10 +98,15 @@ zlh_status zlh_init_feq(zlh_ctx *ctx);
 zlh_status zlh_init_macros(zlh_ctx *ctx, int level, const double *rho, const double *u);
 /* post-stream populations in key order [n][q] (GridNodeLbm::f_); also sets f_collide = f */
 zlh_status zlh_upload_populations(zlh_ctx *ctx, int level, const double *f);
-/* one level substep: IB forcing, collision, streaming, domain boundaries, and on even substeps
- * of a refined level the coarse/fine transfer (GridInfoLbmInteface::AdvancingAtCurrentTime,
- * lbm_grid_interface.cpp:104-153) */
+/* one level substep: immersed-boundary forcing, collision, streaming and domain boundaries
+ * (SolverLbmInterface::RunSolverOnGivenGrid, lbm_solver_interface.cpp:246-262, with the IB hook
+ * UpdateCriterion and ComputeDomainBoundaryCondition of GridInfoLbmInteface::AdvancingAtCurrentTime,
+ * lbm_grid_interface.cpp:61-153); time_step_level counts this level's substeps from 1 */
 zlh_status zlh_step_level(zlh_ctx *ctx, int level, int time_step_level);
+/* coarse-to-fine interpolation into the fine ghost layers, then fine-to-coarse copy into the
+ * coarse interface layer (SolverLbmInterface::InformationFromGridOfDifferentLevel,
+ * lbm_solver_interface.cpp:224-238; the caller applies it after even substeps of fine_level) */
+zlh_status zlh_transfer_levels(zlh_ctx *ctx, int fine_level);
 /* background steps in the coarse-to-fine multi time stepping order (AmrManager::TimeMarching,
  * amr_manager.cpp:279-300; MultiTimeSteppingC2F, grid/grid_manager.cpp:997-1024) */
 zlh_status zlh_time_marching(zlh_ctx *ctx, int nsteps);

@@ -1,0 +1,46 @@
+"""The C-ABI library loads, exports every declared entry point and validates arguments
+before touching a device (CPU only)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from zllbm_amd import hip
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(zlh_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = hip.lib()
+    declared = _declared()
+    assert len(declared) >= 20
+    missing = [n for n in sorted(declared) if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(hip.EXPORTED) <= declared
+
+
+def test_argument_validation_without_device():
+    L = hip.lib()
+    h = C.c_void_p()
+    assert L.zlh_create(None, C.byref(h)) == -1
+    d = hip.SolverDesc()
+    d.dims = 4
+    assert L.zlh_create(C.byref(d), C.byref(h)) == -1
+    d.dims, d.nu0 = 3, 0.0
+    assert L.zlh_create(C.byref(d), C.byref(h)) == -1          # viscosity must be positive
+    d.nu0, d.precision = 0.01, 7
+    assert L.zlh_create(C.byref(d), C.byref(h)) == -1          # unknown precision
+    assert L.zlh_last_error(None) == b"null context"
+    assert L.zlh_step_level(None, 0, 1) == -1
+    assert L.zlh_build(None) == -1

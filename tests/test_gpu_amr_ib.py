@@ -50,6 +50,8 @@ def test_amr_substep_by_substep_fp64():
     for lvl, tsl in [(0, 1), (1, 1), (1, 2), (0, 1), (1, 1)]:
         o.advance_level(lvl, tsl)
         s.step_level(lvl, tsl)
+        if lvl > 0 and tsl % 2 == 0:
+            s.transfer_levels(lvl)
         for l in range(2):
             assert_close(s.f(l), o.f(l))
 

@@ -883,16 +883,21 @@ zlh_status zlh_init_feq(zlh_ctx *ctx) {
 zlh_status zlh_step_level(zlh_ctx *ctx, int level, int time_step_level) {
     if (!ctx) return ZLH_EINVAL;
     if (!ctx->built) return ctx->fail(ZLH_ESTATE, "build before stepping");
-    if (level < 0 || level > ctx->d.max_level) return ctx->fail(ZLH_EINVAL, "bad level");
+    if (level < 0 || level > ctx->d.max_level || time_step_level < 1)
+        return ctx->fail(ZLH_EINVAL, "bad level / substep");
     zlh_status s;
     // immersed-boundary forcing (UpdateCriterion, lbm_grid_interface.cpp:61-95)
     if ((s = launch_ib(ctx, level))) return s;
+    // collision, streaming and domain boundaries (RunSolverOnGivenGrid + ComputeDomainBoundaryCondition)
     if ((s = do_step_kernels(ctx, level))) return s;
-    // InformationFromGridOfDifferentLevel (lbm_solver_interface.cpp:224-238)
-    if (level > 0 && time_step_level % 2 == 0) {
-        if ((s = launch_transfers(ctx, level))) return s;
-    }
     return ZLH_OK;
+}
+
+zlh_status zlh_transfer_levels(zlh_ctx *ctx, int fine_level) {
+    if (!ctx) return ZLH_EINVAL;
+    if (!ctx->built) return ctx->fail(ZLH_ESTATE, "build before transfer");
+    if (fine_level < 1 || fine_level > ctx->d.max_level) return ctx->fail(ZLH_EINVAL, "bad fine level %d", fine_level);
+    return launch_transfers(ctx, fine_level);
 }
 
 zlh_status zlh_time_marching(zlh_ctx *ctx, int nsteps) {
@@ -910,6 +915,9 @@ zlh_status zlh_time_marching(zlh_ctx *ctx, int nsteps) {
             ++tsl[l];
             zlh_status s = zlh_step_level(ctx, l, tsl[l]);
             if (s) return s;
+            // InformationFromGridOfDifferentLevel (lbm_solver_interface.cpp:224-238): refined
+            // levels exchange with their parent when their substep count is even
+            if (l > 0 && tsl[l] % 2 == 0 && (s = zlh_transfer_levels(ctx, l))) return s;
         }
     }
     return ZLH_OK;

@@ -227,3 +227,23 @@ def boundary_masks(grid: Grid, level: int) -> Dict[int, np.ndarray]:
         out[1 + 2 * a] = xyz[:, a] == lo
         out[2 + 2 * a] = xyz[:, a] == hi
     return out
+
+
+def write_grid_file(grid: Grid, path: str) -> None:
+    """Serialise a grid for the C++ host driver (zllbm_lbm_driver): magic "ZLGRID1", dims,
+    level count, background index bounds, then per level the keys, flags and interface layers."""
+    with open(path, "wb") as fh:
+        fh.write(b"ZLGRID1\0")
+        lo = list(grid.dom_min) + [0] * (3 - grid.dims)
+        hi = list(grid.dom_max) + [0] * (3 - grid.dims)
+        np.asarray([grid.dims, len(grid.levels)], dtype=np.int32).tofile(fh)
+        np.asarray(lo + hi, dtype=np.uint32).tofile(fh)
+        for lev in grid.levels:
+            np.asarray([lev.n], dtype=np.int64).tofile(fh)
+            np.ascontiguousarray(lev.keys, dtype=np.uint64).tofile(fh)
+            np.ascontiguousarray(lev.flags, dtype=np.int32).tofile(fh)
+            np.asarray([len(lev.layers)], dtype=np.int32).tofile(fh)
+            for (kind, layer), keys in sorted(lev.layers.items()):
+                np.asarray([kind, layer], dtype=np.int32).tofile(fh)
+                np.asarray([keys.size], dtype=np.int64).tofile(fh)
+                np.ascontiguousarray(keys, dtype=np.uint64).tofile(fh)

@@ -38,7 +38,7 @@ class SolverDesc(C.Structure):
 EXPORTED = (
     "zlh_create", "zlh_destroy", "zlh_last_error", "zlh_set_level_nodes", "zlh_set_interface_layer",
     "zlh_ib_set_markers", "zlh_build", "zlh_init_feq", "zlh_init_macros", "zlh_upload_populations",
-    "zlh_step_level", "zlh_time_marching", "zlh_download_populations", "zlh_download_macros",
+    "zlh_step_level", "zlh_transfer_levels", "zlh_time_marching", "zlh_download_populations", "zlh_download_macros",
     "zlh_download_source_table", "zlh_ib_download_forces", "zlh_conservation", "zlh_level_info",
     "zlh_synchronize", "zlh_timing_enable", "zlh_timing_read",
 )
@@ -67,6 +67,7 @@ def lib():
         L.zlh_init_macros.argtypes = [P, C.c_int, D(np.float64), D(np.float64)]
         L.zlh_upload_populations.argtypes = [P, C.c_int, D(np.float64)]
         L.zlh_step_level.argtypes = [P, C.c_int, C.c_int]
+        L.zlh_transfer_levels.argtypes = [P, C.c_int]
         L.zlh_time_marching.argtypes = [P, C.c_int]
         L.zlh_download_populations.argtypes = [P, C.c_int, D(np.float64)]
         L.zlh_download_macros.argtypes = [P, C.c_int, D(np.float64), D(np.float64)]
@@ -175,6 +176,9 @@ class Solver:
 
     def step_level(self, level, time_step_level):
         self._chk(lib().zlh_step_level(self.h, level, time_step_level))
+
+    def transfer_levels(self, fine_level):
+        self._chk(lib().zlh_transfer_levels(self.h, fine_level))
 
     def time_marching(self, nsteps=1):
         self._chk(lib().zlh_time_marching(self.h, nsteps))

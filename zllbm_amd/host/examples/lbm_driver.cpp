@@ -1,0 +1,154 @@
+// lbm_driver.cpp -- runs the LBM time step through the reference-style plugin interfaces:
+// a grid manager with one GridInfoLbmHip per level holding the node maps and interface layers,
+// a SolverLbmHip created by its creator and configured with the reference's setters, and the
+// AmrManager::TimeMarching loop.  Mirrors the setup sequence of the reference's example drivers
+// (examples/lbm/immersed_boundary/cylinder2d/cylinder2d.cpp:14-111).
+//
+// usage: zllbm_lbm_driver <grid file> <parameter file> <background steps> <output file>
+//   grid file:  "ZLGRID1" + dims, levels, index bounds, then per level keys, flags and layers
+//   parameters: "key = value" lines (viscosity, rho, velocity, collision, les, cs, les_tau_mode,
+//               precision, bc.xmin ... bc.zmax = <bounce_back|periodic|nee> [u v w])
+//   output:     per level the node count and the post-stream populations in key order
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include "lbm_hip_solver.h"
+
+using namespace rootproject;
+using namespace rootproject::lbmproject;
+
+namespace {
+template <typename T>
+bool rd(std::ifstream &in, T *v, size_t n = 1) {
+    in.read(reinterpret_cast<char *>(v), sizeof(T) * n);
+    return bool(in);
+}
+
+amrproject::EDomainBoundaryDirection dir_of(const std::string &s) {
+    if (s == "xmin") return amrproject::EDomainBoundaryDirection::kBoundaryXMin;
+    if (s == "xmax") return amrproject::EDomainBoundaryDirection::kBoundaryXMax;
+    if (s == "ymin") return amrproject::EDomainBoundaryDirection::kBoundaryYMin;
+    if (s == "ymax") return amrproject::EDomainBoundaryDirection::kBoundaryYMax;
+    if (s == "zmin") return amrproject::EDomainBoundaryDirection::kBoundaryZMin;
+    if (s == "zmax") return amrproject::EDomainBoundaryDirection::kBoundaryZMax;
+    throw ZlhError("unknown boundary " + s);
+}
+}  // namespace
+
+int main(int argc, char **argv) {
+    if (argc != 5) {
+        std::fprintf(stderr, "usage: %s <grid> <params> <steps> <out>\n", argv[0]);
+        return 2;
+    }
+    try {
+        std::ifstream in(argv[1], std::ios::binary);
+        char magic[8];
+        if (!rd(in, magic, 8) || std::strncmp(magic, "ZLGRID1", 7) != 0) throw ZlhError("bad grid file");
+        int32_t dims, nlev;
+        uint32_t dmin[3], dmax[3];
+        rd(in, &dims); rd(in, &nlev); rd(in, dmin, 3); rd(in, dmax, 3);
+        // solver (SolverCreatorLbmD2Q9 / D3Q19) and grid manager
+        std::shared_ptr<amrproject::SolverInterface> solver =
+            dims == 2 ? SolverCreatorLbmD2Q9Hip().CreateSolver() : SolverCreatorLbmD3Q19Hip().CreateSolver();
+        auto *lbm = dynamic_cast<SolverLbmHip *>(solver.get());
+        amrproject::GridManagerInterface gm(dims);
+        gm.k0MaxLevel_ = nlev - 1;
+        gm.k0MinIndexOfBackgroundNode_.assign(dmin, dmin + dims);
+        gm.k0MaxIndexOfBackgroundNode_.assign(dmax, dmax + dims);
+        gm.vec_ptr_solver_.push_back(solver);
+        solver->SetPtrToGridManager(&gm);
+        std::vector<std::vector<uint64_t>> level_keys(nlev);
+        for (int l = 0; l < nlev; ++l) {
+            auto gi = solver->ptr_grid_info_creator_->CreateGridInfo();
+            gi->SetGridLevel(l);
+            gi->SetPtrSolver(solver);
+            gi->SetPtrToParentGridManager(&gm);
+            gi->InitialGridInfo(dims);
+            int64_t n;
+            rd(in, &n);
+            std::vector<uint64_t> keys(n);
+            std::vector<int32_t> flags(n);
+            rd(in, keys.data(), n);
+            rd(in, flags.data(), n);
+            for (int64_t i = 0; i < n; ++i) {
+                auto node = gi->GridNodeCreator();
+                node->flag_status_ = flags[i];
+                gi->map_grid_node_.emplace(DefSFBitset(keys[i]), std::move(node));
+            }
+            level_keys[l] = keys;
+            int32_t nlayers;
+            rd(in, &nlayers);
+            auto info = std::make_shared<amrproject::InterfaceLayerInfo>();
+            for (int k = 0; k < nlayers; ++k) {
+                int32_t kind, layer;
+                int64_t m;
+                rd(in, &kind); rd(in, &layer); rd(in, &m);
+                std::vector<uint64_t> lk(m);
+                rd(in, lk.data(), m);
+                std::vector<DefMap<DefInt>> *v = kind == ZLH_OUTER_F2C ? &info->vec_outer_fine2coarse_
+                                               : kind == ZLH_OUTER_C2F ? &info->vec_outer_coarse2fine_
+                                               : kind == ZLH_INNER_F2C ? &info->vec_inner_fine2coarse_
+                                                                       : &info->vec_inner_coarse2fine_;
+                if ((int)v->size() <= layer) v->resize(layer + 1);
+                for (uint64_t key : lk) (*v)[layer].emplace(DefSFBitset(key), 0);
+            }
+            if (nlayers) gi->map_ptr_interface_layer_info_[{amrproject::ECriterionType::kGeometry, 0}] = info;
+            gm.vec_ptr_grid_info_.push_back(gi);
+        }
+        if (!in) throw ZlhError("truncated grid file");
+        // parameters (ReadAndSetupSolverParameters keys, lbm_solver_interface.cpp:49-117)
+        std::ifstream pin(argv[2]);
+        std::string line;
+        while (std::getline(pin, line)) {
+            if (line.empty() || line[0] == '#') continue;
+            const size_t eq = line.find('=');
+            if (eq == std::string::npos) continue;
+            std::string key = line.substr(0, eq);
+            key.erase(key.find_last_not_of(" \t") + 1);
+            std::istringstream val(line.substr(eq + 1));
+            if (key == "viscosity") { double v; val >> v; lbm->SetDefaultViscosity(v); }
+            else if (key == "rho") { double v; val >> v; lbm->SetDefaultDensity(v); }
+            else if (key == "velocity") { std::vector<double> u(dims); for (auto &x : u) val >> x; lbm->SetDefaultVelocity(u); }
+            else if (key == "collision") {
+                std::string t; int l = 0;
+                while (val >> t) lbm->SetCollisionOperator(l++, t == "mrt" ? ELbmCollisionOperatorType::kLbmMrt
+                                                                           : ELbmCollisionOperatorType::kLbmSrt);
+            }
+            else if (key == "les") { std::string t; val >> t; if (t != "none") lbm->SetLesModel(t); }
+            else if (key == "cs") { double v; val >> v; lbm->SetSmagorinskyConstant(v); }
+            else if (key == "les_tau_mode") { int v; val >> v; lbm->SetLesTauMode(v); }
+            else if (key == "precision") { std::string t; val >> t; lbm->SetPrecision(t == "fp32" ? ZLH_FP32_SHIFTED : ZLH_FP64); }
+            else if (key.rfind("bc.", 0) == 0) {
+                std::string scheme; val >> scheme;
+                std::vector<double> v; double x; while (val >> x) v.push_back(x);
+                ELbmBoundaryConditionScheme s = scheme == "bounce_back" ? ELbmBoundaryConditionScheme::kBounceBack
+                                              : scheme == "periodic" ? ELbmBoundaryConditionScheme::kPeriodic
+                                              : scheme == "nee" ? ELbmBoundaryConditionScheme::kNonEqExtrapolation
+                                                                : ELbmBoundaryConditionScheme::kUndefined;
+                lbm->SetDomainBoundaryCondition(dir_of(key.substr(3)), s, v);
+            }
+        }
+        solver->SolverInitial();
+        std::cout << solver->GetSolverMethod() << std::endl;
+        const int steps = std::atoi(argv[3]);
+        for (int t = 0; t < steps; ++t) TimeMarching(&gm, (DefAmrLUint)t);
+        std::ofstream out(argv[4], std::ios::binary);
+        for (int l = 0; l < nlev; ++l) {
+            lbm->CopyPopulationsToHost(l);
+            const int64_t n = (int64_t)level_keys[l].size();
+            out.write(reinterpret_cast<const char *>(&n), sizeof(n));
+            for (uint64_t key : level_keys[l]) {
+                auto *node = dynamic_cast<GridNodeLbmHip *>(gm.vec_ptr_grid_info_[l]->map_grid_node_.at(DefSFBitset(key)).get());
+                out.write(reinterpret_cast<const char *>(node->f_.data()), sizeof(double) * node->f_.size());
+            }
+        }
+        std::cout << "done " << steps << " steps" << std::endl;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

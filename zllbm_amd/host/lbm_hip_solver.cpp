@@ -1,0 +1,274 @@
+// lbm_hip_solver.cpp -- SolverLbmHip / GridInfoLbmHip: the reference's solver-manager and
+// grid-information plugin calls translated to the C ABI of the device library.
+#include "lbm_hip_solver.h"
+#include <algorithm>
+#include <cstdio>
+
+namespace rootproject {
+namespace lbmproject {
+
+namespace {
+SolverLbmHip *solver_of(const amrproject::GridInfoInterface &g) {
+    auto sp = g.GetPtrToSolver().lock();
+    auto *s = dynamic_cast<SolverLbmHip *>(sp.get());
+    if (!s) throw ZlhError("grid information has no LBM HIP solver");
+    return s;
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// GridInfoLbmHip
+// ---------------------------------------------------------------------------------------------
+// GridInfoLbmInteface::AdvancingAtCurrentTime (lbm_grid_interface.cpp:104-153): the IB hook,
+// collide + stream and the domain boundaries run inside RunSolverOnGivenGrid on the device; the
+// coarse/fine exchange follows through InformationFromGridOfDifferentLevel.
+void GridInfoLbmHip::AdvancingAtCurrentTime(const amrproject::ETimeSteppingScheme time_scheme,
+                                            const DefInt time_step_level, const DefReal time_step_current,
+                                            amrproject::MpiManager *const, amrproject::CriterionManager *const) {
+    SolverLbmHip *s = solver_of(*this);
+    amrproject::SFBitsetAuxInterface aux;
+    s->RunSolverOnGivenGrid(time_scheme, time_step_level, time_step_current, aux, this);
+    s->InformationFromGridOfDifferentLevel(time_step_level, aux, this);
+}
+
+std::unique_ptr<amrproject::GridNode> GridInfoLbmHip::GridNodeCreator() const {
+    auto n = std::make_unique<GridNodeLbmHip>();
+    if (auto sp = ptr_solver_.lock()) {
+        auto *s = dynamic_cast<SolverLbmHip *>(sp.get());
+        if (s) n->rho_ = s->GetDefaultDensity();
+    }
+    return n;
+}
+
+void GridInfoLbmHip::SetNodeVariablesAsZeros(amrproject::GridNode *const ptr_node) {
+    auto *n = dynamic_cast<GridNodeLbmHip *>(ptr_node);
+    if (!n) return;
+    n->rho_ = 0.;
+    std::fill(n->velocity_.begin(), n->velocity_.end(), 0.);
+    std::fill(n->f_.begin(), n->f_.end(), 0.);
+}
+
+// ---------------------------------------------------------------------------------------------
+// SolverLbmHip
+// ---------------------------------------------------------------------------------------------
+SolverLbmHip::SolverLbmHip(DefInt dims, DefInt num_q) : k0NumQ_(num_q) {
+    k0SolverDims_ = dims;
+    name_ = dims == 2 ? "LbmD2Q9" : "LbmD3Q19";
+    solver_type_ = name_;
+    k0Velocity_.assign(dims, 0.);
+    ptr_grid_info_creator_ = std::make_unique<GridInfoLbmHipCreator>();
+}
+
+SolverLbmHip::~SolverLbmHip() {
+    if (ctx_) zlh_destroy(ctx_);
+}
+
+std::string SolverLbmHip::GetSolverMethod() {
+    return "Solver LBM D" + std::to_string(k0SolverDims_) + "Q" + std::to_string(k0NumQ_) +
+           " Model is active (MI355X).";
+}
+
+void SolverLbmHip::Check(zlh_status s, const char *what) const {
+    if (s != ZLH_OK) {
+        std::string msg = std::string(what) + " failed (" + std::to_string(s) + "): " + zlh_last_error(ctx_);
+        std::fprintf(stderr, "%s\n", msg.c_str());
+        throw ZlhError(msg);
+    }
+}
+
+void SolverLbmHip::SetLesModel(const std::string &type) {
+    if (type == "smagorinsky") les_ = true;
+    else throw ZlhError("LES model " + type + " is not defined, please choose from: smagorinsky");
+}
+
+void SolverLbmHip::SetCollisionOperator(const DefInt i_level, const ELbmCollisionOperatorType type) {
+    if (static_cast<DefInt>(collision_type_.size()) <= i_level) collision_type_.resize(i_level + 1, type);
+    collision_type_[i_level] = type;
+}
+
+void SolverLbmHip::SetDomainBoundaryCondition(const amrproject::EDomainBoundaryDirection which,
+                                              const ELbmBoundaryConditionScheme scheme,
+                                              const std::vector<DefReal> &values) {
+    BoundarySetting &b = bc_[static_cast<int>(which)];
+    b.boundary_scheme_ = scheme;
+    b.SetValues(values);
+}
+
+void SolverLbmHip::SetImmersedBoundary(DefInt level, const std::vector<std::array<DefReal, 3>> &xyz,
+                                       const std::vector<DefReal> &area,
+                                       const std::vector<std::array<DefReal, 3>> &velocity, DefInt stencil_dis,
+                                       DefReal stiffness) {
+    Ib &m = ib_[level];
+    m.xyz.clear();
+    m.vel.clear();
+    for (auto &p : xyz) m.xyz.insert(m.xyz.end(), p.begin(), p.end());
+    for (auto &p : velocity) m.vel.insert(m.vel.end(), p.begin(), p.end());
+    m.area = area;
+    m.stencil = stencil_dis;
+    m.stiffness = stiffness;
+    bool_forces_ = true;
+}
+
+// SolverLbmInterface::SolverInitial (lbm_solver_interface.cpp:19-44): parameter checks; the
+// device state needs the mesh and is built on the first time step (or by BuildDevice)
+void SolverLbmHip::SolverInitial() {
+    if (k0LbmViscosity_ < 1e-6) std::fprintf(stderr, "LBM viscosity is less than predefined kEps\n");
+    if (static_cast<DefInt>(k0Velocity_.size()) != k0SolverDims_)
+        throw ZlhError("size of initial velocity is not equal to solver dimension");
+}
+
+void SolverLbmHip::BuildDevice() {
+    if (ctx_) return;
+    if (!ptr_grid_manager_) throw ZlhError("Grid manager is not assigned for solver.");
+    amrproject::GridManagerInterface &gm = *ptr_grid_manager_;
+    zlh_solver_desc d{};
+    d.dims = k0SolverDims_;
+    d.max_level = gm.k0MaxLevel_;
+    d.precision = precision_;
+    d.compressible = 1;
+    d.nu0 = k0LbmViscosity_;
+    d.rho0 = k0Rho_;
+    for (int a = 0; a < k0SolverDims_ && a < (int)k0Velocity_.size(); ++a) d.u0[a] = k0Velocity_[a];
+    d.les = les_ ? ZLH_LES_SMAGORINSKY : ZLH_LES_NONE;
+    d.les_tau_mode = les_tau_mode_;
+    d.cs = cs_;
+    d.has_force = bool_forces_ ? 1 : 0;
+    for (int a = 0; a < (int)k0ConstForce_.size() && a < 3; ++a) d.const_force[a] = k0ConstForce_[a];
+    // InstantiateCollisionOperator (collision_model/lbm_collision_models.cpp:60-73): SRT by default,
+    // the last given type for levels beyond the list
+    for (int l = 0; l < 16; ++l) {
+        ELbmCollisionOperatorType t = collision_type_.empty() ? ELbmCollisionOperatorType::kLbmSrt
+                                      : collision_type_[std::min<size_t>(l, collision_type_.size() - 1)];
+        d.collision[l] = t == ELbmCollisionOperatorType::kLbmMrt ? ZLH_MRT : ZLH_SRT;
+    }
+    for (auto &kv : bc_) {
+        d.bc_scheme[kv.first] = static_cast<int>(kv.second.boundary_scheme_);
+        for (int a = 0; a < 3; ++a) d.bc_value[kv.first][a] = kv.second.values_[a];
+    }
+    for (int a = 0; a < k0SolverDims_; ++a) {
+        if (a >= (int)gm.k0MinIndexOfBackgroundNode_.size() || a >= (int)gm.k0MaxIndexOfBackgroundNode_.size())
+            throw ZlhError("background index range not set on the grid manager");
+        d.dom_min[a] = gm.k0MinIndexOfBackgroundNode_[a];
+        d.dom_max[a] = gm.k0MaxIndexOfBackgroundNode_[a];
+    }
+    zlh_status s = zlh_create(&d, &ctx_);
+    if (s != ZLH_OK) {
+        ctx_ = nullptr;
+        throw ZlhError("zlh_create failed (" + std::to_string(s) + ")");
+    }
+    for (int l = 0; l <= gm.k0MaxLevel_; ++l) {
+        amrproject::GridInfoInterface &gi = *gm.vec_ptr_grid_info_.at(l);
+        std::vector<std::pair<uint64_t, int32_t>> nodes;
+        nodes.reserve(gi.map_grid_node_.size());
+        for (auto &kv : gi.map_grid_node_) nodes.emplace_back(kv.first.to_ullong(), kv.second->flag_status_);
+        std::sort(nodes.begin(), nodes.end());
+        std::vector<uint64_t> keys(nodes.size());
+        std::vector<int32_t> flags(nodes.size());
+        for (size_t i = 0; i < nodes.size(); ++i) { keys[i] = nodes[i].first; flags[i] = nodes[i].second; }
+        Check(zlh_set_level_nodes(ctx_, l, keys.data(), flags.data(), (int64_t)keys.size()), "zlh_set_level_nodes");
+        // interface layers: every interface's maps of one kind and layer are merged
+        std::map<std::pair<int, int>, std::vector<uint64_t>> layers;
+        for (auto &it : gi.map_ptr_interface_layer_info_) {
+            const amrproject::InterfaceLayerInfo &li = *it.second;
+            auto add = [&](int kind, const std::vector<DefMap<DefInt>> &v) {
+                for (size_t layer = 0; layer < v.size() && layer < 3; ++layer)
+                    for (auto &kv : v[layer]) layers[{kind, (int)layer}].push_back(kv.first.to_ullong());
+            };
+            add(ZLH_OUTER_F2C, li.vec_outer_fine2coarse_);
+            add(ZLH_OUTER_C2F, li.vec_outer_coarse2fine_);
+            add(ZLH_INNER_F2C, li.vec_inner_fine2coarse_);
+            add(ZLH_INNER_C2F, li.vec_inner_coarse2fine_);
+        }
+        for (auto &kv : layers) {
+            std::sort(kv.second.begin(), kv.second.end());
+            kv.second.erase(std::unique(kv.second.begin(), kv.second.end()), kv.second.end());
+            Check(zlh_set_interface_layer(ctx_, l, kv.first.first, kv.first.second, kv.second.data(),
+                                          (int64_t)kv.second.size()),
+                  "zlh_set_interface_layer");
+        }
+    }
+    for (auto &kv : ib_) {
+        const Ib &m = kv.second;
+        Check(zlh_ib_set_markers(ctx_, kv.first, m.xyz.data(), m.area.data(), m.vel.data(), (int64_t)m.area.size(),
+                                 m.stencil, m.stiffness),
+              "zlh_ib_set_markers");
+    }
+    Check(zlh_build(ctx_), "zlh_build");
+    // initial populations: node mirrors with f_ set, else feq(rho0, u0)
+    Check(zlh_init_feq(ctx_), "zlh_init_feq");
+    for (int l = 0; l <= gm.k0MaxLevel_; ++l) {
+        amrproject::GridInfoInterface &gi = *gm.vec_ptr_grid_info_.at(l);
+        std::vector<std::pair<uint64_t, const GridNodeLbmHip *>> nodes;
+        bool any = false;
+        for (auto &kv : gi.map_grid_node_) {
+            auto *n = dynamic_cast<const GridNodeLbmHip *>(kv.second.get());
+            nodes.emplace_back(kv.first.to_ullong(), n);
+            if (n && (int)n->f_.size() == k0NumQ_) any = true;
+        }
+        if (!any) continue;
+        std::sort(nodes.begin(), nodes.end(),
+                  [](const auto &a, const auto &b) { return a.first < b.first; });
+        std::vector<double> f(nodes.size() * k0NumQ_);
+        std::vector<double> feq(k0NumQ_);
+        for (size_t i = 0; i < nodes.size(); ++i) {
+            const GridNodeLbmHip *n = nodes[i].second;
+            if (!n || (int)n->f_.size() != k0NumQ_)
+                throw ZlhError("level " + std::to_string(l) + ": populations set on some nodes only");
+            std::copy(n->f_.begin(), n->f_.end(), f.begin() + i * k0NumQ_);
+        }
+        Check(zlh_upload_populations(ctx_, l, f.data()), "zlh_upload_populations");
+    }
+}
+
+// SolverLbmInterface::RunSolverOnGivenGrid (lbm_solver_interface.cpp:246-262)
+void SolverLbmHip::RunSolverOnGivenGrid(const amrproject::ETimeSteppingScheme, const DefInt time_step_level,
+                                        const DefReal, const amrproject::SFBitsetAuxInterface &,
+                                        amrproject::GridInfoInterface *const ptr_grid_info) {
+    if (!ctx_) BuildDevice();
+    Check(zlh_step_level(ctx_, ptr_grid_info->GetGridLevel(), time_step_level), "zlh_step_level");
+}
+
+// SolverLbmInterface::InformationFromGridOfDifferentLevel (lbm_solver_interface.cpp:224-238)
+void SolverLbmHip::InformationFromGridOfDifferentLevel(const DefInt time_step_level,
+                                                       const amrproject::SFBitsetAuxInterface &,
+                                                       amrproject::GridInfoInterface *const ptr_grid_info) {
+    const DefInt i_level = ptr_grid_info->GetGridLevel();
+    if (i_level > 0 && time_step_level % 2 == 0) Check(zlh_transfer_levels(ctx_, i_level), "zlh_transfer_levels");
+}
+
+void SolverLbmHip::CopyPopulationsToHost(DefInt level) {
+    if (!ctx_) throw ZlhError("device state not built");
+    amrproject::GridInfoInterface &gi = *ptr_grid_manager_->vec_ptr_grid_info_.at(level);
+    std::vector<uint64_t> keys;
+    keys.reserve(gi.map_grid_node_.size());
+    for (auto &kv : gi.map_grid_node_) keys.push_back(kv.first.to_ullong());
+    std::sort(keys.begin(), keys.end());
+    const size_t n = keys.size();
+    std::vector<double> f(n * k0NumQ_), rho(n), u(n * k0SolverDims_);
+    Check(zlh_download_populations(ctx_, level, f.data()), "zlh_download_populations");
+    Check(zlh_download_macros(ctx_, level, rho.data(), u.data()), "zlh_download_macros");
+    for (size_t i = 0; i < n; ++i) {
+        auto *node = dynamic_cast<GridNodeLbmHip *>(gi.map_grid_node_.at(DefSFBitset(keys[i])).get());
+        if (!node) continue;
+        node->f_.assign(f.begin() + i * k0NumQ_, f.begin() + (i + 1) * k0NumQ_);
+        node->rho_ = rho[i];
+        node->velocity_.assign(u.begin() + i * k0SolverDims_, u.begin() + (i + 1) * k0SolverDims_);
+    }
+}
+
+// AmrManager::TimeMarching (amr_manager.cpp:279-300)
+void TimeMarching(amrproject::GridManagerInterface *gm, DefAmrLUint time_step_background) {
+    amrproject::MultiTimeSteppingC2F scheme(gm->k0MaxLevel_);
+    std::vector<DefInt> time_step_level(gm->k0MaxLevel_ + 1, 0);
+    for (DefInt i_level : scheme.k0TimeSteppingOrder_) {
+        ++time_step_level[i_level];
+        const DefReal t = scheme.GetCurrentTimeStep(i_level, time_step_background, time_step_level[i_level]);
+        amrproject::GridInfoInterface &g = *gm->vec_ptr_grid_info_.at(i_level);
+        g.SetUpGridAtBeginningOfTimeStep(time_step_level[i_level]);
+        g.AdvancingAtCurrentTime(amrproject::ETimeSteppingScheme::kMultiSteppingC2F, time_step_level[i_level], t,
+                                 nullptr, nullptr);
+    }
+}
+
+}  // namespace lbmproject
+}  // namespace rootproject

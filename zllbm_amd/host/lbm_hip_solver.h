@@ -1,0 +1,156 @@
+// lbm_hip_solver.h -- LBM solver plugin whose time step runs on the MI355X through the C ABI
+// of include/zllbm_hip.h.  Mirrors the reference's solver manager SolverLbmInterface and grid
+// information GridInfoLbmInteface (lbm_project/lbm_interface.h:140-474): the same setters, the
+// same plugin virtuals, and the node maps kept as host mirrors filled on request.
+#pragma once
+#include <array>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include "amr_interface.h"
+#include "zllbm_hip.h"
+
+namespace rootproject {
+namespace lbmproject {
+
+enum class ELbmCollisionOperatorType { kLbmSrt, kLbmMrt, kUndefined };
+enum class ELbmBoundaryConditionScheme { kUndefined = 0, kBounceBack = 1, kPeriodic = 2, kNonEqExtrapolation = 3 };
+enum class ELbmLesModelType { kSmagorinsky, kUndefined };
+
+// Fatal errors of the device library surface as this exception (the reference's
+// LogManager::LogError terminates; callers that want that behaviour let it propagate).
+class ZlhError : public std::runtime_error {
+ public:
+    using std::runtime_error::runtime_error;
+};
+
+// host mirror of one node (GridNodeLbm, lbm_interface.h:34-194)
+struct GridNodeLbmHip : public amrproject::GridNode {
+    DefReal rho_ = 1.;
+    std::vector<DefReal> velocity_, f_, force_;
+};
+
+// per-direction boundary setting (BoundaryConditionLbmInterface::SetValues)
+struct BoundarySetting {
+    ELbmBoundaryConditionScheme boundary_scheme_ = ELbmBoundaryConditionScheme::kUndefined;
+    std::array<DefReal, 3> values_{0., 0., 0.};
+    void SetValues(const std::vector<DefReal> &values) {
+        for (size_t i = 0; i < values.size() && i < 3; ++i) values_[i] = values[i];
+    }
+};
+
+class SolverLbmHip;
+
+class GridInfoLbmHip : public amrproject::GridInfoInterface {
+ public:
+    std::map<amrproject::EDomainBoundaryDirection, std::unique_ptr<BoundarySetting>> domain_boundary_condition_;
+    void AdvancingAtCurrentTime(const amrproject::ETimeSteppingScheme time_scheme, const DefInt time_step_level,
+                                const DefReal time_step_current, amrproject::MpiManager *const ptr_mpi_manager,
+                                amrproject::CriterionManager *const ptr_criterion_manager) override;
+    std::unique_ptr<amrproject::GridNode> GridNodeCreator() const override;
+    void SetNodeVariablesAsZeros(amrproject::GridNode *const ptr_node) override;
+    void InitialGridInfo(const DefInt dims) override { dims_ = dims; }
+
+ private:
+    DefInt dims_ = 0;
+};
+
+class GridInfoLbmHipCreator : public amrproject::GridInfoCreatorInterface {
+ public:
+    std::shared_ptr<amrproject::GridInfoInterface> CreateGridInfo() const override {
+        return std::make_shared<GridInfoLbmHip>();
+    }
+};
+
+class SolverLbmHip : public amrproject::SolverInterface {
+ public:
+    SolverLbmHip(DefInt dims, DefInt num_q);
+    ~SolverLbmHip() override;
+
+    // SolverLbmInterface setters (lbm_interface.h:296-319)
+    void SetDefaultViscosity(const DefReal v) { k0LbmViscosity_ = v; }
+    void SetDefaultDensity(const DefReal rho) { k0Rho_ = rho; }
+    void SetDefaultVelocity(const std::vector<DefReal> &u) { k0Velocity_ = u; }
+    void SetConstantForce(const std::vector<DefReal> &force) { bool_forces_ = true; k0ConstForce_ = force; }
+    void SetNumForces(const DefInt n) { bool_forces_ = true; k0NumForces_ = n; }
+    DefReal GetDefaultViscosity() const { return k0LbmViscosity_; }
+    DefReal GetDefaultDensity() const { return k0Rho_; }
+    // LES (SetLesModel, les_models/lbm_les_models.cpp:17-41) with the Smagorinsky constant and the
+    // relaxation-time mode exposed
+    void SetLesModel(const ELbmLesModelType type) { les_ = type == ELbmLesModelType::kSmagorinsky; }
+    void SetLesModel(const std::string &type);
+    void SetSmagorinskyConstant(const DefReal cs) { cs_ = cs; }
+    void SetLesTauMode(const int mode) { les_tau_mode_ = mode; }
+    // collision operator per level (SetCollisionOperator, collision_model/lbm_collision_models.cpp:78-90)
+    void SetCollisionOperator(const DefInt i_level, const ELbmCollisionOperatorType type);
+    // SetDomainBoundaryCondition (boundary_conditions/lbm_boundary_conditions.cpp:20-52) for every level
+    void SetDomainBoundaryCondition(const amrproject::EDomainBoundaryDirection which,
+                                    const ELbmBoundaryConditionScheme scheme, const std::vector<DefReal> &values = {});
+    // storage form of the populations on the device
+    void SetPrecision(const int precision) { precision_ = precision; }
+    // immersed-boundary markers of one level (coordinates in background units)
+    void SetImmersedBoundary(DefInt level, const std::vector<std::array<DefReal, 3>> &xyz,
+                             const std::vector<DefReal> &area, const std::vector<std::array<DefReal, 3>> &velocity,
+                             DefInt stencil_dis = 2, DefReal stiffness = 2.);
+
+    const DefInt k0NumQ_;
+    bool bool_forces_ = false;
+
+    // SolverInterface
+    std::string GetSolverMethod() override;
+    void SolverInitial() override;
+    void RunSolverOnGivenGrid(const amrproject::ETimeSteppingScheme time_scheme, const DefInt time_step_level,
+                              const DefReal time_step_current, const amrproject::SFBitsetAuxInterface &sfbitset_aux,
+                              amrproject::GridInfoInterface *const ptr_grid_info) override;
+    void InformationFromGridOfDifferentLevel(const DefInt time_step_level,
+                                             const amrproject::SFBitsetAuxInterface &sfbitset_aux,
+                                             amrproject::GridInfoInterface *const ptr_grid_info) override;
+
+    // device state: built from the grid manager's node maps and interface layers on first use,
+    // populations taken from the node mirrors' f_ when set, else feq(rho0, u0)
+    void BuildDevice();
+    // copy the device populations of a level into the node mirrors (GridNodeLbmHip::f_, rho_, velocity_)
+    void CopyPopulationsToHost(DefInt level);
+    zlh_ctx *Context() const { return ctx_; }
+
+ private:
+    void Check(zlh_status s, const char *what) const;
+    DefReal k0LbmViscosity_ = 1., k0Rho_ = 1.;
+    std::vector<DefReal> k0Velocity_, k0ConstForce_;
+    DefInt k0NumForces_ = 0;
+    bool les_ = false;
+    DefReal cs_ = 0.16;   // the reference's fixed constant (lbm_les_models.h:49) unless set
+    int les_tau_mode_ = ZLH_LES_TAU_PHYSICAL;
+    int precision_ = ZLH_FP64;
+    std::vector<ELbmCollisionOperatorType> collision_type_;
+    std::map<int, BoundarySetting> bc_;
+    struct Ib {
+        std::vector<double> xyz, area, vel;
+        int stencil = 2;
+        double stiffness = 2.;
+    };
+    std::map<DefInt, Ib> ib_;
+    zlh_ctx *ctx_ = nullptr;
+};
+
+class SolverCreatorLbmD2Q9Hip final : public amrproject::SolverCreatorInterface {
+ public:
+    std::shared_ptr<amrproject::SolverInterface> CreateSolver() const override {
+        return std::make_shared<SolverLbmHip>(2, 9);
+    }
+};
+class SolverCreatorLbmD3Q19Hip final : public amrproject::SolverCreatorInterface {
+ public:
+    std::shared_ptr<amrproject::SolverInterface> CreateSolver() const override {
+        return std::make_shared<SolverLbmHip>(3, 19);
+    }
+};
+
+// AmrManager::TimeMarching (amr_project/amr_manager.cpp:279-300) over a grid manager whose
+// levels hold GridInfoLbmHip instances
+void TimeMarching(amrproject::GridManagerInterface *grid_manager, DefAmrLUint time_step_background);
+
+}  // namespace lbmproject
+}  // namespace rootproject
