@@ -129,6 +129,30 @@ zlh_status zlh_synchronize(zlh_ctx *ctx);
  * milliseconds and the number of launches since enabling (synchronising) */
 zlh_status zlh_timing_enable(zlh_ctx *ctx, int enable);
 zlh_status zlh_timing_read(zlh_ctx *ctx, double *total_ms, int64_t *launches);
+
+/* ---- learned sub-grid closure (no counterpart in the reference; replaces the Smagorinsky
+ * relaxation time of les_models/lbm_les_smagorinsky.cpp:22-94 when desc.les = ZLH_LES_MLP) ----
+ * 9 -> 32 -> 32 -> 1 MLP, SiLU hidden layers, softplus output times nu_scale = eddy viscosity in
+ * lattice units; tau_eff = tau0 + 3 nu_t.  params: W1[32][9], b1[32], W2[32][32], b2[32],
+ * w3[32], b3 (1409 floats).  Inputs are normalised as (x - mean) / std; where any raw input
+ * lies outside [fmin, fmax] the node uses Smagorinsky (physical tau mode, desc.cs). */
+zlh_status zlh_mlp_set(zlh_ctx *ctx, const float *params, const float *mean, const float *std_dev,
+                       const float *fmin, const float *fmax, double nu_scale);
+/* full-batch mean-squared-error gradient of softplus(net(x)) against t over n samples
+ * (normalised inputs [n][9]); grad is [1409], loss may be null */
+zlh_status zlh_mlp_gradient(zlh_ctx *ctx, const float *params, const float *x, const float *t, int64_t n,
+                            float *grad, float *loss);
+/* Adam training on the device: `steps` minibatches of `batch` samples drawn with a seeded hash
+ * (batch >= n: every step uses all n samples in order); params is updated in place;
+ * loss_history (may be null) receives each minibatch loss */
+zlh_status zlh_mlp_train(zlh_ctx *ctx, const float *x, const float *t, int64_t n, int steps, int batch,
+                         double lr, uint64_t seed, float *params, float *loss_history);
+/* network output (nu_t / nu_scale) of normalised inputs with the parameters set by zlh_mlp_set */
+zlh_status zlh_mlp_eval(zlh_ctx *ctx, const float *x, int64_t n, float *y);
+/* per node of a level (key order): raw closure inputs [n][9], the closure's nu_t and whether the
+ * node falls back to Smagorinsky, from the current post-stream populations (synchronising) */
+zlh_status zlh_closure_features(zlh_ctx *ctx, int level, float *features, float *nu_t, int32_t *fallback);
+
 const char *zlh_last_error(const zlh_ctx *ctx);
 void zlh_destroy(zlh_ctx *ctx);
 

@@ -62,6 +62,10 @@ struct zo_sim {
     zo_level lv[ZO_MAXL];
     char err[512];
     int finalized;
+    /* learned closure (les == 2): W1[32][9], b1, W2[32][32], b2, w3, b3; normalisation, range */
+    double mlp_p[1409], mlp_mean[9], mlp_istd[9], mlp_min[9], mlp_max[9], mlp_scale;
+    int mlp_set;
+    int64_t mlp_fallbacks;
 };
 
 static void zo_seterr(zo_sim *s, const char *fmt, ...) {
@@ -487,9 +491,63 @@ static double zo_les_tau_t(const zo_sim *s, double dt, double tau0, const double
     return 0.5 * (sqrt(tau0 * tau0 + k * sqrt(2. * pi_neq2)) - tau0);
 }
 
+/* learned closure: features of P = Pi/rho and the FP64 network (mirrors the device closure) */
+static double zo_mlp_tau_t(zo_sim *s, const double *pi, double rho, const double *u, double tau0, int *fb) {
+    double ir = 1. / rho;
+    double pxx = pi[0] * ir, pyy = pi[1] * ir, pzz = pi[2] * ir, pxy = pi[3] * ir, pxz = pi[4] * ir, pyz = pi[5] * ir;
+    double pp = pxx * pxx + pyy * pyy + pzz * pzz + 2. * (pxy * pxy + pxz * pxz + pyz * pyz);
+    double a = sqrt(2. * pp);
+    double i1 = pxx + pyy + pzz;
+    double i2 = 0.5 * (i1 * i1 - pp);
+    double i3 = pxx * (pyy * pzz - pyz * pyz) - pxy * (pxy * pzz - pyz * pxz) + pxz * (pxy * pyz - pyy * pxz);
+    double x[9] = {a, i1, i2, i3, u[0] * u[0] + u[1] * u[1] + u[2] * u[2], rho - 1., tau0, a * tau0, a * a};
+    for (int k = 0; k < 9; ++k) {
+        /* the device compares the float value of each input */
+        float xf = (float)x[k];
+        if (!(xf >= (float)s->mlp_min[k] && xf <= (float)s->mlp_max[k])) {
+            *fb = 1;
+            double q = sqrt(2. * (pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2] +
+                                  2. * (pi[3] * pi[3] + pi[4] * pi[4] + pi[5] * pi[5])));
+            double k18 = 18. * s->p.cs * s->p.cs;
+            return 0.5 * (sqrt(tau0 * tau0 + k18 * q) - tau0);
+        }
+    }
+    *fb = 0;
+    double xn[9], h1[32];
+    for (int k = 0; k < 9; ++k) xn[k] = (x[k] - s->mlp_mean[k]) * s->mlp_istd[k];
+    const double *W1 = s->mlp_p, *b1 = W1 + 288, *W2 = b1 + 32, *b2 = W2 + 1024, *w3 = b2 + 32, b3 = w3[32];
+    for (int j = 0; j < 32; ++j) {
+        double z = b1[j];
+        for (int i = 0; i < 9; ++i) z += W1[j * 9 + i] * xn[i];
+        h1[j] = z / (1. + exp(-z));
+    }
+    double out = b3;
+    for (int j = 0; j < 32; ++j) {
+        double z = b2[j];
+        for (int i = 0; i < 32; ++i) z += W2[j * 32 + i] * h1[i];
+        out += w3[j] * (z / (1. + exp(-z)));
+    }
+    double sp = out > 30. ? out : log1p(exp(out));
+    return 3. * s->mlp_scale * sp;
+}
+
+int zo_set_mlp(zo_sim *s, const double *params, const double *mean, const double *std_dev, const double *fmin,
+               const double *fmax, double nu_scale) {
+    memcpy(s->mlp_p, params, sizeof(s->mlp_p));
+    for (int k = 0; k < 9; ++k) {
+        s->mlp_mean[k] = mean[k];
+        s->mlp_istd[k] = 1. / std_dev[k];
+        s->mlp_min[k] = fmin[k];
+        s->mlp_max[k] = fmax[k];
+    }
+    s->mlp_scale = nu_scale;
+    s->mlp_set = 1;
+    return 0;
+}
+
 /* func_collision_node_ (lbm_solver_interface.cpp:266-300) with the SRT/MRT operators
  * (lbm_collision_model_srt.cpp:20-39, lbm_collision_model_mrt.cpp:114-143) */
-static void zo_collide_node(const zo_sim *s, const zo_level *L, int collision, int64_t i) {
+static void zo_collide_node(zo_sim *s, const zo_level *L, int collision, int64_t i) {
     int q = s->q;
     const double *f = L->f + i * q;
     double *fc = L->fc + i * q;
@@ -508,9 +566,28 @@ static void zo_collide_node(const zo_sim *s, const zo_level *L, int collision, i
     if (forces) zo_forcing(s, u, F, fi);
     double tau_eff = L->tau0;
     int les = s->p.les;
-    if (les) {
+    if (les == 1) {
         double tau_t = zo_les_tau_t(s, L->dt, L->tau0, f, feq, forces ? F : NULL, u);
         tau_eff = s->p.les_tau_mode == 0 ? tau_t : L->tau0 + tau_t;
+    } else if (les == 2) {
+        double pi[6] = {0, 0, 0, 0, 0, 0};
+        for (int iq = 1; iq < q; ++iq) {
+            double d = f[iq] - feq[iq];
+            pi[0] += s->cx[iq] * s->cx[iq] * d; pi[1] += s->cy[iq] * s->cy[iq] * d; pi[2] += s->cz[iq] * s->cz[iq] * d;
+            pi[3] += s->cx[iq] * s->cy[iq] * d; pi[4] += s->cx[iq] * s->cz[iq] * d; pi[5] += s->cy[iq] * s->cz[iq] * d;
+        }
+        if (forces) {
+            pi[0] += L->dt * F[0] * u[0];
+            pi[1] += L->dt * F[1] * u[1];
+            pi[3] += 0.5 * L->dt * (F[0] * u[0] + F[1] * u[1]);
+            if (s->p.dims == 3) {
+                pi[2] += L->dt * F[2] * u[2];
+                pi[4] += 0.5 * L->dt * (F[0] * u[0] + F[2] * u[2]);
+                pi[5] += 0.5 * L->dt * (F[1] * u[1] + F[2] * u[2]);
+            }
+        }
+        int fb = 0;
+        tau_eff = L->tau0 + zo_mlp_tau_t((zo_sim *)s, pi, rho, u, L->tau0, &fb);
     }
     if (collision == 0) {
         for (int iq = 0; iq < q; ++iq) {

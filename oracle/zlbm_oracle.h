@@ -94,6 +94,10 @@ int zo_set_ib(zo_sim *s, int level, int64_t n, const double *xyz, const double *
               const double *vel, int stencil_dis, double stiffness, double dx_background);
 int zo_get_ib_force(const zo_sim *s, int level, double *force);   /* Lagrangian force [n][dims] */
 
+/* learned closure for les == 2 (params: W1[32][9], b1, W2[32][32], b2, w3, b3) */
+int zo_set_mlp(zo_sim *s, const double *params, const double *mean, const double *std_dev, const double *fmin,
+               const double *fmax, double nu_scale);
+
 /* mass and momentum sums over a level (FP64) */
 int zo_conservation(const zo_sim *s, int level, double *out4);
 

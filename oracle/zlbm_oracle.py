@@ -78,6 +78,7 @@ def lib():
         L.zo_set_threads.argtypes = [C.c_int]
         L.zo_debug_set_s_base.argtypes = [P, D(np.float64)]
         L.zo_debug_transfer.argtypes = [P, C.c_int]
+        L.zo_set_mlp.argtypes = [P, D(np.float64), D(np.float64), D(np.float64), D(np.float64), D(np.float64), C.c_double]
         _lib = L
     return _lib
 
@@ -227,6 +228,10 @@ class Oracle:
 
     def debug_set_s_base(self, s_base):
         self._chk(lib().zo_debug_set_s_base(self.h, np.ascontiguousarray(s_base, np.float64)))
+
+    def set_mlp(self, params, mean, std, fmin, fmax, nu_scale):
+        a = lambda v: np.ascontiguousarray(v, np.float64)
+        self._chk(lib().zo_set_mlp(self.h, a(params), a(mean), a(std), a(fmin), a(fmax), float(nu_scale)))
 
     def debug_transfer(self, fine_level):
         self._chk(lib().zo_debug_transfer(self.h, fine_level))

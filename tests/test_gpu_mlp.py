@@ -1,0 +1,160 @@
+"""Learned closure on the device: gradients and training against the FP64 reference, the
+closure fused into the collision against the oracle, and the gain over Smagorinsky."""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+from oracle.zlbm_oracle import Oracle
+from zllbm_amd import closure, grid as g, hip
+from tests import scenarios as sc
+from tests.test_gpu_parity import assert_close
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx():
+    grid, kw = sc.periodic_tgv(3, 4, nu=0.02)
+    return hip.Solver(grid, **kw)
+
+
+def _data(n=3000, seed=0):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(n, 9)).astype(np.float32)
+    w = rng.normal(size=9)
+    T = np.logaddexp(0.0, X @ w * 0.5).astype(np.float32)
+    return X, T
+
+
+def test_gradient_matches_reference():
+    s = _ctx()
+    X, T = _data()
+    p = closure.init_params(3)
+    loss, gd = s.mlp_gradient(p, X, T)
+    lref, gref = mo.loss_and_grad(p.astype(np.float64), X.astype(np.float64), T.astype(np.float64))
+    assert abs(loss - lref) <= 1e-5 * lref
+    scale = np.max(np.abs(gref))
+    assert np.max(np.abs(gd - gref)) <= 2e-5 * scale, np.max(np.abs(gd - gref)) / scale
+
+
+def test_training_reduces_loss_and_adam_step():
+    s = _ctx()
+    X, T = _data()
+    p0 = closure.init_params(5)
+    # one Adam step on the full batch equals the reference update of the device gradient
+    p1, hist = s.mlp_train(p0, X, T, 1, len(T), 1e-3)
+    _, g0 = s.mlp_gradient(p0, X, T)
+    pr, _, _ = mo.adam(p0.astype(np.float64), g0.astype(np.float64), np.zeros(mo.NPARAM), np.zeros(mo.NPARAM), 1, 1e-3)
+    np.testing.assert_allclose(p1, pr, rtol=1e-5, atol=1e-6)
+    p2, hist = s.mlp_train(p0, X, T, 400, 512, 3e-3)
+    l0 = mo.loss_and_grad(p0.astype(np.float64), X, T)[0]
+    l2 = mo.loss_and_grad(p2.astype(np.float64), X, T)[0]
+    assert l2 < 0.2 * l0, (l0, l2)
+
+
+def test_eval_matches_reference():
+    s = _ctx()
+    X, T = _data(500)
+    p = closure.init_params(7)
+    m = closure.ClosureModel(p, np.zeros(9, np.float32), np.ones(9, np.float32), np.full(9, -1e30, np.float32),
+                             np.full(9, 1e30, np.float32), 1.0)
+    s.mlp_set(m)
+    y = s.mlp_eval(X)
+    yref, _ = mo.forward(p.astype(np.float64), X.astype(np.float64))
+    np.testing.assert_allclose(y, yref, rtol=2e-5, atol=1e-6)
+
+
+# typical magnitudes of the raw inputs in the small TGV runs below (a ~ 1e-2)
+_SCALE = np.array([2e-2, 1e-3, 1e-4, 1e-6, 2.5e-3, 1e-3, 1.0, 1e-2, 4e-4], np.float32)
+
+
+def _random_model(tau0, seed=11, nu_scale=2e-3):
+    p = closure.init_params(seed)
+    mean = np.zeros(9, np.float32)
+    mean[6] = tau0
+    return closure.ClosureModel(p, mean, _SCALE.copy(), np.full(9, -1e30, np.float32),
+                                np.full(9, 1e30, np.float32), nu_scale)
+
+
+@pytest.mark.parametrize("coll", [hip.SRT, hip.MRT])
+def test_fused_closure_matches_oracle(coll):
+    nu = 0.002
+    grid, kw = sc.periodic_tgv(3, 10, nu=nu)
+    m = _random_model(3 * nu + 0.5)
+    o = Oracle(grid, collision=(coll,), les=2, cs=0.1, **kw)
+    o.set_mlp(m.params.astype(np.float64), m.mean, m.std, m.fmin, m.fmax, m.nu_scale)
+    s = hip.Solver(grid, precision=hip.FP64, collision=(coll,), les=hip.LES_MLP, cs=0.1, **kw).build()
+    s.mlp_set(m)
+    rho, u = sc.tgv_fields(grid, 0.05, noise=1e-3)
+    o.init_macro(0, rho, u)
+    s.init_macro(0, rho, u)
+    o.time_marching(20)
+    s.time_marching(20)
+    # the device network runs in FP32, the oracle's in FP64
+    assert_close(s.f(0), o.f(0), rtol=2e-7)
+
+
+def test_closure_features_and_fallback():
+    nu, cs = 0.002, 0.1
+    tau0 = 3 * nu + 0.5
+    grid, kw = sc.periodic_tgv(3, 8, nu=nu)
+    m = _random_model(tau0)
+    s = hip.Solver(grid, precision=hip.FP64, collision=(hip.MRT,), les=hip.LES_MLP, cs=cs, **kw).build()
+    s.mlp_set(m)
+    rho, u = sc.tgv_fields(grid, 0.05, noise=1e-3)
+    s.init_macro(0, rho, u)
+    s.time_marching(5)
+    f = s.f(0)
+    xo = mo.features_from_populations(f, tau0).astype(np.float32)
+    x, _, fb = s.closure_features(0)
+    np.testing.assert_allclose(x, xo, rtol=2e-5, atol=1e-12)
+    assert not fb.any()
+    # nodes whose stress invariant exceeds the median leave the training range
+    m.fmax[0] = np.median(x[:, 0])
+    s.mlp_set(m)
+    _, nu_t, fb = s.closure_features(0)
+    out = x[:, 0] > m.fmax[0]
+    np.testing.assert_array_equal(fb.astype(bool), out)
+    assert 0 < out.sum() < len(out)
+    nu_net = m.nu_scale * mo.forward(m.params, (xo.astype(np.float64) - m.mean) / m.std)[0]
+    nu_smag = mo.smagorinsky_nu_t(f, tau0, cs)
+    np.testing.assert_allclose(nu_t[~out], nu_net[~out], rtol=2e-5)
+    np.testing.assert_allclose(nu_t[out], nu_smag[out], rtol=2e-6)
+
+
+def _fine_snapshots(n_fine, steps, snaps, nu=0.004, seed=20261202):
+    grid = g.uniform_grid(3, (n_fine,) * 3)
+    u = closure.random_fourier_velocity(grid, nmodes=64, kmax=8, urms=0.05 / np.sqrt(3.0), seed=seed)
+    s = hip.Solver(grid, precision=hip.FP32_SHIFTED, collision=(hip.MRT,), nu0=nu, bc=sc.all_bc(3, sc.PER),
+                   les=hip.LES_SMAGORINSKY, cs=0.05).build()
+    s.init_macro(0, np.ones(grid.levels[0].n), u)
+    out = []
+    for _ in range(snaps):
+        s.time_marching(steps)
+        _, uu = s.macros(0)
+        out.append(closure.unique_field(grid, uu))
+    return out
+
+
+def test_learned_closure_beats_smagorinsky_a_priori():
+    """Trained on box-filtered fine runs, the network predicts the filtered sub-grid viscosity of
+    a held-out snapshot with a smaller error than Smagorinsky (Cs 0.1)."""
+    ratio, nu_f = 4, 0.004
+    tau0_c = 3 * nu_f / ratio + 0.5
+    snaps = _fine_snapshots(96, 150, 4, nu=nu_f)
+    data = [closure.filtered_samples(u, ratio, tau0_c) for u in snaps]
+    xtr = np.concatenate([d[0] for d in data[:3]])
+    ttr = np.concatenate([d[1] for d in data[:3]])
+    s = _ctx()
+    model, hist = closure.fit_model(s, xtr, ttr, steps=1500, batch=2048, lr=3e-3)
+    assert hist[-50:].mean() < hist[:50].mean()
+    xte, tte, pte = data[3]
+    s.mlp_set(model)
+    inside = np.all((xte >= model.fmin) & (xte <= model.fmax), axis=1)
+    pred = s.mlp_eval(model.normalise(xte)) * model.nu_scale
+    smag = closure.smagorinsky_nu_t(pte, tau0_c, 0.1)
+    err_mlp = np.sqrt(np.mean((pred[inside] - tte[inside]) ** 2))
+    err_smag = np.sqrt(np.mean((smag[inside] - tte[inside]) ** 2))
+    print("a-priori rms error of nu_t: mlp %.3e smagorinsky %.3e (%.0f%% of samples in range)"
+          % (err_mlp, err_smag, 100 * inside.mean()))
+    assert inside.mean() > 0.5
+    assert err_mlp < 0.8 * err_smag

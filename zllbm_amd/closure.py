@@ -1,0 +1,171 @@
+"""Learned sub-grid closure: training data from explicitly filtered fine runs of this solver,
+training on the device, and the model container.
+
+No counterpart in the reference.  The closure replaces the Smagorinsky relaxation time
+(les_models/lbm_les_smagorinsky.cpp:22-94) with tau_eff = tau0 + 3 nu_t, nu_t from a
+9 -> 32 -> 32 -> 1 MLP of local non-equilibrium invariants, and falls back to Smagorinsky where
+an input leaves the training range.
+
+Training data (a-priori): a fine periodic run is box-filtered by `ratio` onto the coarse grid.
+The sub-grid stress tau_ij = <u_i u_j> - <u_i><u_j> and the resolved strain S of the filtered
+velocity give the target nu_t = max(0, -tau^d:S / (2 S:S)) in coarse lattice units.  The inputs
+are those the collision computes from the coarse non-equilibrium stress, which for a closure with
+eddy viscosity nu_t is Pi = -(2/3) rho (tau0 + 3 nu_t) S (first-order Chapman-Enskog).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+NF, NH = 9, 32
+NPARAM = NH * NF + NH + NH * NH + NH + NH + 1
+
+
+@dataclass
+class ClosureModel:
+    params: np.ndarray
+    mean: np.ndarray
+    std: np.ndarray
+    fmin: np.ndarray
+    fmax: np.ndarray
+    nu_scale: float
+
+    def save(self, path):
+        np.savez(path, params=self.params, mean=self.mean, std=self.std, fmin=self.fmin, fmax=self.fmax,
+                 nu_scale=np.float64(self.nu_scale))
+
+    @staticmethod
+    def load(path):
+        z = np.load(path)
+        return ClosureModel(z["params"], z["mean"], z["std"], z["fmin"], z["fmax"], float(z["nu_scale"]))
+
+    def normalise(self, x):
+        return ((x - self.mean) / self.std).astype(np.float32)
+
+
+def init_params(seed=0):
+    """He-style initial weights, zero biases except a negative output bias (small nu_t)."""
+    rng = np.random.default_rng(seed)
+    W1 = rng.normal(0, np.sqrt(2.0 / NF), (NH, NF))
+    W2 = rng.normal(0, np.sqrt(2.0 / NH), (NH, NH))
+    w3 = rng.normal(0, np.sqrt(1.0 / NH), NH)
+    p = np.concatenate([W1.ravel(), np.zeros(NH), W2.ravel(), np.zeros(NH), w3, [-1.0]])
+    return p.astype(np.float32)
+
+
+def features_from_stress(pi6, rho, u, tau0):
+    """The 9 raw inputs the collision computes (mlp.hpp closure_features) from the
+    non-equilibrium stress pi6 = (xx, yy, zz, xy, xz, yz) [n, 6], density and velocity."""
+    P = pi6 / rho[:, None]
+    pxx, pyy, pzz, pxy, pxz, pyz = (P[:, k] for k in range(6))
+    pp = pxx ** 2 + pyy ** 2 + pzz ** 2 + 2 * (pxy ** 2 + pxz ** 2 + pyz ** 2)
+    a = np.sqrt(2 * pp)
+    i1 = pxx + pyy + pzz
+    i2 = 0.5 * (i1 * i1 - pp)
+    i3 = pxx * (pyy * pzz - pyz * pyz) - pxy * (pxy * pzz - pyz * pxz) + pxz * (pxy * pyz - pyy * pxz)
+    u2 = (u ** 2).sum(1)
+    t0 = np.full_like(a, tau0)
+    return np.stack([a, i1, i2, i3, u2, rho - 1.0, t0, a * tau0, a * a], axis=1)
+
+
+def random_fourier_velocity(grid, level=0, nmodes=64, kmin=1, kmax=8, urms=0.05 / np.sqrt(3.0), seed=20261202):
+    """Solenoidal random-mode field: nmodes integer wave vectors with kmin <= |k| <= kmax (in units of
+    2 pi / L), amplitudes from E(k) ~ k^(-5/3), uniform phases, scaled to the given rms speed."""
+    rng = np.random.default_rng(seed)
+    ks = []
+    while len(ks) < nmodes:
+        k = rng.integers(-kmax, kmax + 1, size=3)
+        r = np.linalg.norm(k)
+        if kmin <= r <= kmax:
+            ks.append(k.astype(np.float64))
+    ks = np.asarray(ks)
+    kn = np.linalg.norm(ks, axis=1)
+    amp = kn ** (-11.0 / 6.0)
+    dirs = rng.normal(size=(nmodes, 3))
+    dirs -= (np.sum(dirs * ks, 1) / kn ** 2)[:, None] * ks        # divergence free
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    phase = rng.uniform(0, 2 * np.pi, nmodes)
+    xyz = grid.coords(level).astype(np.float64) / (1 << level) - 1.0
+    L = np.asarray(grid.cells, dtype=np.float64)
+    u = np.zeros((len(xyz), 3))
+    for m in range(nmodes):
+        arg = 2 * np.pi * (xyz @ (ks[m] / L)) + phase[m]
+        u += np.outer(amp[m] * np.cos(arg), dirs[m])
+    uniq = np.all(xyz < L, axis=1)          # rms over the unique nodes of the periodic box
+    u *= urms / np.sqrt(np.mean(np.sum(u[uniq] ** 2, 1)) / 3.0)
+    return u
+
+
+def unique_field(grid, values, level=0):
+    """Values on the unique nodes of a periodic level as an (n, n, n, ...) array (duplicated
+    max planes dropped)."""
+    cells = np.asarray(grid.cells) << level
+    xyz = grid.coords(level).astype(np.int64) - (1 << level)
+    keep = np.all(xyz < cells, axis=1)
+    arr = np.zeros(tuple(cells) + values.shape[1:], dtype=values.dtype)
+    arr[tuple(xyz[keep].T)] = values[keep]
+    return arr
+
+
+def box_filter(arr, r):
+    """Average over r^3 blocks of a periodic (n, n, n, ...) array."""
+    n = arr.shape[:3]
+    s = arr.reshape(n[0] // r, r, n[1] // r, r, n[2] // r, r, *arr.shape[3:])
+    return s.mean(axis=(1, 3, 5))
+
+
+def strain(u):
+    """Symmetric velocity gradient of a periodic (n, n, n, 3) field, unit spacing, central
+    differences; returns (xx, yy, zz, xy, xz, yz)."""
+    g = np.empty(u.shape[:3] + (3, 3))
+    for a in range(3):
+        for b in range(3):
+            g[..., a, b] = 0.5 * (np.roll(u[..., a], -1, axis=b) - np.roll(u[..., a], 1, axis=b))
+    S = 0.5 * (g + np.swapaxes(g, -1, -2))
+    return np.stack([S[..., 0, 0], S[..., 1, 1], S[..., 2, 2], S[..., 0, 1], S[..., 0, 2], S[..., 1, 2]], -1)
+
+
+def filtered_samples(u_fine, ratio, tau0_coarse):
+    """A-priori samples from one fine velocity snapshot (n, n, n, 3): raw inputs [m, 9] and target
+    nu_t [m] in coarse lattice units, plus the coarse strain for the Smagorinsky comparison."""
+    ub = box_filter(u_fine, ratio)
+    uu = box_filter(np.einsum("...a,...b->...ab", u_fine, u_fine), ratio)
+    tau = uu - np.einsum("...a,...b->...ab", ub, ub)
+    tr = np.trace(tau, axis1=-2, axis2=-1)
+    tau_d = tau - tr[..., None, None] * np.eye(3) / 3.0
+    S6 = strain(ub)
+    S = np.empty(S6.shape[:-1] + (3, 3))
+    for k, (a, b) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]):
+        S[..., a, b] = S6[..., k]
+        S[..., b, a] = S6[..., k]
+    SS = np.einsum("...ab,...ab->...", S, S)
+    nu_t = np.maximum(0.0, -np.einsum("...ab,...ab->...", tau_d, S) / np.maximum(2.0 * SS, 1e-30))
+    rho = np.ones(SS.shape)
+    pi = -(2.0 / 3.0) * rho[..., None] * (tau0_coarse + 3.0 * nu_t)[..., None] * S6
+    x = features_from_stress(pi.reshape(-1, 6), rho.ravel(), ub.reshape(-1, 3), tau0_coarse)
+    return x, nu_t.ravel(), pi.reshape(-1, 6)
+
+
+def smagorinsky_nu_t(pi6, tau0, cs):
+    """nu_t of the Smagorinsky closure for a given non-equilibrium stress (physical tau mode)."""
+    q = np.sqrt(2 * (pi6[:, 0] ** 2 + pi6[:, 1] ** 2 + pi6[:, 2] ** 2 +
+                     2 * (pi6[:, 3] ** 2 + pi6[:, 4] ** 2 + pi6[:, 5] ** 2)))
+    return 0.5 * (np.sqrt(tau0 ** 2 + 18 * cs * cs * q) - tau0) / 3.0
+
+
+def fit_model(solver, x_raw, nu_t, steps=3000, batch=4096, lr=2e-3, seed=20261202, margin=0.0):
+    """Normalise, train on the device (solver: any zllbm_amd.hip.Solver context) and return the
+    model with the training range of the raw inputs."""
+    mean = x_raw.mean(0)
+    std = x_raw.std(0)
+    std[std == 0] = 1.0
+    nu_scale = float(np.mean(nu_t)) if np.mean(nu_t) > 0 else 1.0
+    xn = ((x_raw - mean) / std).astype(np.float32)
+    t = (nu_t / nu_scale).astype(np.float32)
+    params, hist = solver.mlp_train(init_params(seed), xn, t, steps, min(batch, len(t)), lr, seed)
+    span = x_raw.max(0) - x_raw.min(0)
+    model = ClosureModel(params, mean.astype(np.float32), std.astype(np.float32),
+                         (x_raw.min(0) - margin * span).astype(np.float32),
+                         (x_raw.max(0) + margin * span).astype(np.float32), nu_scale)
+    return model, hist

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include "lattice.h"
+#include "mlp.hpp"
 
 namespace zlh {
 
@@ -30,10 +31,11 @@ struct StepParams {
     double dt;             // level time step 1/2^l
     double cs18;           // 18 * Cs^2 (k0Smagorinsky)
     double const_force[3];
-    int les;               // 0 none, 1 Smagorinsky
+    int les;               // 0 none, 1 Smagorinsky, 2 learned closure
     int les_tau_mode;      // 0 reference, 1 physical
     int has_force;
     int compressible;
+    const MlpDev *mlp;     // learned closure (les == 2)
 };
 
 template <typename T>
@@ -147,7 +149,7 @@ __device__ __forceinline__ void collide(C (&f)[Lattice<D>::Q], const C *F, const
         }
     }
     C tau = C(p.tau0);
-    if (p.les == 1) {
+    if (p.les) {
         C pxx = 0, pxy = 0, pyy = 0, pzz = 0, pxz = 0, pyz = 0;
 #pragma unroll
         for (int q = 1; q < Q; ++q) {
@@ -171,10 +173,16 @@ __device__ __forceinline__ void collide(C (&f)[Lattice<D>::Q], const C *F, const
                 pyz += C(0.5) * dt * (F[1] * u[1] + F[2] * u[2]);
             }
         }
-        C pi2 = pxx * pxx + pyy * pyy + pzz * pzz + C(2) * (pxy * pxy + pxz * pxz + pyz * pyz);
-        C t0 = C(p.tau0);
-        C tau_t = C(0.5) * (sqrt(t0 * t0 + C(p.cs18) * sqrt(C(2) * pi2)) - t0);
-        tau = p.les_tau_mode ? t0 + tau_t : tau_t;
+        const C t0 = C(p.tau0);
+        if (p.les == 1) {
+            C pi2 = pxx * pxx + pyy * pyy + pzz * pzz + C(2) * (pxy * pxy + pxz * pxz + pyz * pyz);
+            C tau_t = C(0.5) * (sqrt(t0 * t0 + C(p.cs18) * sqrt(C(2) * pi2)) - t0);
+            tau = p.les_tau_mode ? t0 + tau_t : tau_t;
+        } else {
+            const C pi[6] = {pxx, pyy, pzz, pxy, pxz, pyz};
+            bool fb;
+            tau = t0 + mlp_tau_t<C>(*p.mlp, pi, rho, drho, u, t0, C(p.cs18), &fb);
+        }
     }
     if (COLL == 0) {
         const C inv_tau = C(1) / tau;

@@ -305,6 +305,41 @@ __global__ void k_bc_nee(int64_t nb, const int32_t *node_ai, const int32_t *inne
     }
 }
 
+
+// closure inputs, learned nu_t and fallback flag of each node's post-stream populations
+// ([Q][n] double, unshifted), without forcing terms
+template <int D>
+__global__ void k_closure_features(const double *fq, int64_t n, double tau0, double cs18, const MlpDev *m,
+                                   float *x_out, float *nu_out, int32_t *fb_out) {
+    using L = Lattice<D>;
+    constexpr int Q = L::Q;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double f[Q], feq[Q], rho, u[3];
+    for (int q = 0; q < Q; ++q) f[q] = fq[q * n + i];
+    double r = f[0], jx = 0, jy = 0, jz = 0;
+    for (int q = 1; q < Q; ++q) { r += f[q]; jx += L::cx[q] * f[q]; jy += L::cy[q] * f[q]; jz += L::cz[q] * f[q]; }
+    rho = r; u[0] = jx / r; u[1] = jy / r; u[2] = D == 3 ? jz / r : 0.;
+    const double usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    for (int q = 0; q < Q; ++q) {
+        const double cu = u[0] * L::cx[q] + u[1] * L::cy[q] + u[2] * L::cz[q];
+        feq[q] = L::w[q] * rho * (1. + 3. * cu + 4.5 * cu * cu - 1.5 * usq);
+    }
+    double pi[6] = {0, 0, 0, 0, 0, 0};
+    for (int q = 1; q < Q; ++q) {
+        const double d = f[q] - feq[q];
+        pi[0] += L::cx[q] * L::cx[q] * d; pi[1] += L::cy[q] * L::cy[q] * d; pi[2] += L::cz[q] * L::cz[q] * d;
+        pi[3] += L::cx[q] * L::cy[q] * d; pi[4] += L::cx[q] * L::cz[q] * d; pi[5] += L::cy[q] * L::cz[q] * d;
+    }
+    float x[kMlpF];
+    closure_features<double>(pi, rho, rho - 1., u, tau0, x);
+    for (int k = 0; k < kMlpF; ++k) x_out[i * kMlpF + k] = x[k];
+    bool fb = false;
+    const double tt = mlp_tau_t<double>(*m, pi, rho, rho - 1., u, tau0, cs18, &fb);
+    nu_out[i] = float(tt / 3.);
+    fb_out[i] = fb ? 1 : 0;
+}
+
 }  // namespace zlh
 
 // ==============================================================================================
@@ -334,6 +369,8 @@ struct zlh_ctx {
     double timing_ms = 0.;
     int64_t timing_launches = 0;
     DevBuf lattice_d;
+    DevBuf mlp_d;           // MlpDev of the learned closure
+    bool mlp_set = false;
 
     zlh_status fail(zlh_status code, const char *fmt, ...) {
         char buf[1024];
@@ -378,7 +415,8 @@ StepParams level_params(const zlh_ctx *ctx, int level) {
     p.tau0 = nu * 3. / p.dt + 0.5;   // CalRelaxationTime (lbm_collision_models.cpp:114-118)
     p.cs18 = 18. * ctx->d.cs * ctx->d.cs;
     for (int a = 0; a < 3; ++a) p.const_force[a] = ctx->d.has_force ? ctx->d.const_force[a] : 0.;
-    p.les = ctx->d.les == ZLH_LES_SMAGORINSKY ? 1 : 0;
+    p.les = ctx->d.les == ZLH_LES_SMAGORINSKY ? 1 : (ctx->d.les == ZLH_LES_MLP ? 2 : 0);
+    p.mlp = ctx->mlp_d.as<MlpDev>();
     p.les_tau_mode = ctx->d.les_tau_mode;
     p.has_force = ctx->d.has_force;
     p.compressible = ctx->d.compressible;
@@ -746,6 +784,7 @@ zlh_status zlh_create(const zlh_solver_desc *desc, zlh_ctx **out) {
     ctx->q = desc->dims == 2 ? 9 : 19;
     ctx->lv.resize(desc->max_level + 1);
     if (hipStreamCreate(&ctx->stream) != hipSuccess) return ZLH_EDEVICE;
+    if (dalloc(ctx.get(), ctx->mlp_d, sizeof(MlpDev))) return ZLH_ENOMEM;
     *out = ctx.release();
     return ZLH_OK;
 }
@@ -821,7 +860,6 @@ zlh_status zlh_build(zlh_ctx *ctx) {
     if (!ctx) return ZLH_EINVAL;
     for (int l = 0; l <= ctx->d.max_level; ++l)
         if (!ctx->lv[l].have) return ctx->fail(ZLH_ESTATE, "level %d nodes not set", l);
-    if (ctx->d.les == ZLH_LES_MLP) return ctx->fail(ZLH_EINVAL, "learned closure is configured through zlh_mlp_*");
     zlh_status s;
     for (int l = 0; l <= ctx->d.max_level; ++l)
         if ((s = build_level_slots(ctx, l))) return s;
@@ -885,6 +923,7 @@ zlh_status zlh_step_level(zlh_ctx *ctx, int level, int time_step_level) {
     if (!ctx->built) return ctx->fail(ZLH_ESTATE, "build before stepping");
     if (level < 0 || level > ctx->d.max_level || time_step_level < 1)
         return ctx->fail(ZLH_EINVAL, "bad level / substep");
+    if (ctx->d.les == ZLH_LES_MLP && !ctx->mlp_set) return ctx->fail(ZLH_ESTATE, "learned closure not set (zlh_mlp_set)");
     zlh_status s;
     // immersed-boundary forcing (UpdateCriterion, lbm_grid_interface.cpp:61-95)
     if ((s = launch_ib(ctx, level))) return s;
@@ -1065,6 +1104,129 @@ zlh_status zlh_timing_read(zlh_ctx *ctx, double *total_ms, int64_t *launches) {
     ctx->events.clear();
     if (total_ms) *total_ms = ctx->timing_ms;
     if (launches) *launches = ctx->timing_launches;
+    return ZLH_OK;
+}
+
+zlh_status zlh_mlp_set(zlh_ctx *ctx, const float *params, const float *mean, const float *std_dev, const float *fmin,
+                       const float *fmax, double nu_scale) {
+    if (!ctx) return ZLH_EINVAL;
+    if (!params || !mean || !std_dev || !fmin || !fmax || !(nu_scale > 0.)) return ctx->fail(ZLH_EINVAL, "bad closure arguments");
+    MlpDev m;
+    std::memcpy(m.p, params, sizeof(m.p));
+    for (int i = 0; i < kMlpF; ++i) {
+        if (!(std_dev[i] > 0.f)) return ctx->fail(ZLH_EINVAL, "feature %d has zero spread", i);
+        m.mean[i] = mean[i];
+        m.inv_std[i] = 1.f / std_dev[i];
+        m.fmin[i] = fmin[i];
+        m.fmax[i] = fmax[i];
+    }
+    m.nu_scale = (float)nu_scale;
+    ZLH_TRY(hipMemcpyAsync(ctx->mlp_d.p, &m, sizeof(m), hipMemcpyHostToDevice, ctx->stream));
+    ZLH_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->mlp_set = true;
+    return ZLH_OK;
+}
+
+zlh_status zlh_mlp_gradient(zlh_ctx *ctx, const float *params, const float *x, const float *t, int64_t n, float *grad,
+                            float *loss) {
+    if (!ctx) return ZLH_EINVAL;
+    if (!params || !x || !t || !grad || n < 1 || n > (1 << 26)) return ctx->fail(ZLH_EINVAL, "bad gradient arguments");
+    DevBuf dp, dx, dt, dg, dl;
+    zlh_status s;
+    if ((s = dalloc(ctx, dp, sizeof(float) * kMlpParams, false))) return s;
+    if ((s = dalloc(ctx, dx, sizeof(float) * kMlpF * n, false))) return s;
+    if ((s = dalloc(ctx, dt, sizeof(float) * n, false))) return s;
+    if ((s = dalloc(ctx, dg, sizeof(float) * kMlpParams))) return s;
+    if ((s = dalloc(ctx, dl, sizeof(float)))) return s;
+    ZLH_TRY(hipMemcpyAsync(dp.p, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, ctx->stream));
+    ZLH_TRY(hipMemcpyAsync(dx.p, x, sizeof(float) * kMlpF * n, hipMemcpyHostToDevice, ctx->stream));
+    ZLH_TRY(hipMemcpyAsync(dt.p, t, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_mlp_grad, dim3(blocks(n)), dim3(256), 0, ctx->stream, dp.as<float>(), dx.as<float>(),
+                       dt.as<float>(), n, (int)n, 0u, 0, dg.as<float>(), dl.as<float>());
+    ZLH_TRY(hipGetLastError());
+    ZLH_TRY(hipMemcpyAsync(grad, dg.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, ctx->stream));
+    if (loss) ZLH_TRY(hipMemcpyAsync(loss, dl.p, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    ZLH_TRY(hipStreamSynchronize(ctx->stream));
+    return ZLH_OK;
+}
+
+zlh_status zlh_mlp_train(zlh_ctx *ctx, const float *x, const float *t, int64_t n, int steps, int batch, double lr,
+                         uint64_t seed, float *params, float *loss_history) {
+    if (!ctx) return ZLH_EINVAL;
+    if (!params || !x || !t || n < 1 || n > (1 << 26) || steps < 1 || batch < 1 || batch > (1 << 20) || !(lr > 0.))
+        return ctx->fail(ZLH_EINVAL, "bad training arguments");
+    DevBuf dp, dx, dt, dg, dm, dv, dl;
+    zlh_status s;
+    if ((s = dalloc(ctx, dp, sizeof(float) * kMlpParams, false))) return s;
+    if ((s = dalloc(ctx, dx, sizeof(float) * kMlpF * n, false))) return s;
+    if ((s = dalloc(ctx, dt, sizeof(float) * n, false))) return s;
+    if ((s = dalloc(ctx, dg, sizeof(float) * kMlpParams))) return s;
+    if ((s = dalloc(ctx, dm, sizeof(float) * kMlpParams))) return s;
+    if ((s = dalloc(ctx, dv, sizeof(float) * kMlpParams))) return s;
+    if ((s = dalloc(ctx, dl, sizeof(float) * steps))) return s;
+    ZLH_TRY(hipMemcpyAsync(dp.p, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, ctx->stream));
+    ZLH_TRY(hipMemcpyAsync(dx.p, x, sizeof(float) * kMlpF * n, hipMemcpyHostToDevice, ctx->stream));
+    ZLH_TRY(hipMemcpyAsync(dt.p, t, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    // a batch of at least n samples is the full set in order (plain full-batch Adam); smaller
+    // batches are drawn with replacement from a per-step hash
+    const int full = (int64_t)batch >= n;
+    const int nb = full ? (int)n : batch;
+    for (int it = 1; it <= steps; ++it) {
+        ZLH_TRY(hipMemsetAsync(dg.p, 0, sizeof(float) * kMlpParams, ctx->stream));
+        hipLaunchKernelGGL(k_mlp_grad, dim3(blocks(nb)), dim3(256), 0, ctx->stream, dp.as<float>(), dx.as<float>(),
+                           dt.as<float>(), n, nb, (uint32_t)(seed * 1000003ull + (uint64_t)it), full ? 0 : 1,
+                           dg.as<float>(), dl.as<float>() + (it - 1));
+        hipLaunchKernelGGL(k_adam, dim3(blocks(kMlpParams)), dim3(256), 0, ctx->stream, dp.as<float>(), dg.as<float>(),
+                           dm.as<float>(), dv.as<float>(), kMlpParams, (float)lr, 0.9f, 0.999f, 1e-8f, it);
+    }
+    ZLH_TRY(hipGetLastError());
+    ZLH_TRY(hipMemcpyAsync(params, dp.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, ctx->stream));
+    if (loss_history) ZLH_TRY(hipMemcpyAsync(loss_history, dl.p, sizeof(float) * steps, hipMemcpyDeviceToHost, ctx->stream));
+    ZLH_TRY(hipStreamSynchronize(ctx->stream));
+    return ZLH_OK;
+}
+
+zlh_status zlh_mlp_eval(zlh_ctx *ctx, const float *x, int64_t n, float *y) {
+    if (!ctx) return ZLH_EINVAL;
+    if (!ctx->mlp_set) return ctx->fail(ZLH_ESTATE, "learned closure not set");
+    if (!x || !y || n < 1) return ctx->fail(ZLH_EINVAL, "bad evaluation arguments");
+    DevBuf dx, dy;
+    zlh_status s;
+    if ((s = dalloc(ctx, dx, sizeof(float) * kMlpF * n, false))) return s;
+    if ((s = dalloc(ctx, dy, sizeof(float) * n, false))) return s;
+    ZLH_TRY(hipMemcpyAsync(dx.p, x, sizeof(float) * kMlpF * n, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_mlp_eval, dim3(blocks(n)), dim3(256), 0, ctx->stream, ctx->mlp_d.as<MlpDev>(), dx.as<float>(),
+                       n, dy.as<float>());
+    ZLH_TRY(hipGetLastError());
+    ZLH_TRY(hipMemcpyAsync(y, dy.p, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+    ZLH_TRY(hipStreamSynchronize(ctx->stream));
+    return ZLH_OK;
+}
+
+zlh_status zlh_closure_features(zlh_ctx *ctx, int level, float *features, float *nu_t, int32_t *fallback) {
+    if (!ctx) return ZLH_EINVAL;
+    if (!ctx->built) return ctx->fail(ZLH_ESTATE, "not built");
+    if (!ctx->mlp_set) return ctx->fail(ZLH_ESTATE, "learned closure not set");
+    if (level < 0 || level > ctx->d.max_level || !features || !nu_t || !fallback) return ctx->fail(ZLH_EINVAL, "bad arguments");
+    LevelState &L = ctx->lv[level];
+    DevBuf fq, dx, dn, df;
+    zlh_status s;
+    if ((s = dalloc(ctx, fq, sizeof(double) * L.n * ctx->q, false))) return s;
+    if ((s = dalloc(ctx, dx, sizeof(float) * kMlpF * L.n, false))) return s;
+    if ((s = dalloc(ctx, dn, sizeof(float) * L.n, false))) return s;
+    if ((s = dalloc(ctx, df, sizeof(int32_t) * L.n, false))) return s;
+    if ((s = do_materialize(ctx, level, fq.as<double>()))) return s;
+    if (ctx->d.dims == 2)
+        hipLaunchKernelGGL(k_closure_features<2>, dim3(blocks(L.n)), dim3(256), 0, ctx->stream, fq.as<double>(), L.n,
+                           L.sp.tau0, L.sp.cs18, ctx->mlp_d.as<MlpDev>(), dx.as<float>(), dn.as<float>(), df.as<int32_t>());
+    else
+        hipLaunchKernelGGL(k_closure_features<3>, dim3(blocks(L.n)), dim3(256), 0, ctx->stream, fq.as<double>(), L.n,
+                           L.sp.tau0, L.sp.cs18, ctx->mlp_d.as<MlpDev>(), dx.as<float>(), dn.as<float>(), df.as<int32_t>());
+    ZLH_TRY(hipGetLastError());
+    ZLH_TRY(hipMemcpyAsync(features, dx.p, sizeof(float) * kMlpF * L.n, hipMemcpyDeviceToHost, ctx->stream));
+    ZLH_TRY(hipMemcpyAsync(nu_t, dn.p, sizeof(float) * L.n, hipMemcpyDeviceToHost, ctx->stream));
+    ZLH_TRY(hipMemcpyAsync(fallback, df.p, sizeof(int32_t) * L.n, hipMemcpyDeviceToHost, ctx->stream));
+    ZLH_TRY(hipStreamSynchronize(ctx->stream));
     return ZLH_OK;
 }
 

@@ -40,7 +40,8 @@ EXPORTED = (
     "zlh_ib_set_markers", "zlh_build", "zlh_init_feq", "zlh_init_macros", "zlh_upload_populations",
     "zlh_step_level", "zlh_transfer_levels", "zlh_time_marching", "zlh_download_populations", "zlh_download_macros",
     "zlh_download_source_table", "zlh_ib_download_forces", "zlh_conservation", "zlh_level_info",
-    "zlh_synchronize", "zlh_timing_enable", "zlh_timing_read",
+    "zlh_synchronize", "zlh_timing_enable", "zlh_timing_read", "zlh_mlp_set", "zlh_mlp_gradient",
+    "zlh_mlp_train", "zlh_mlp_eval", "zlh_closure_features",
 )
 
 
@@ -78,6 +79,12 @@ def lib():
         L.zlh_synchronize.argtypes = [P]
         L.zlh_timing_enable.argtypes = [P, C.c_int]
         L.zlh_timing_read.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        F32 = D(np.float32)
+        L.zlh_mlp_set.argtypes = [P, F32, F32, F32, F32, F32, C.c_double]
+        L.zlh_mlp_gradient.argtypes = [P, F32, F32, F32, C.c_int64, F32, F32]
+        L.zlh_mlp_train.argtypes = [P, F32, F32, F32, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_uint64, F32, F32]
+        L.zlh_mlp_eval.argtypes = [P, F32, C.c_int64, F32]
+        L.zlh_closure_features.argtypes = [P, C.c_int, F32, F32, D(np.int32)]
         _lib = L
     return _lib
 
@@ -216,6 +223,40 @@ class Solver:
         out = np.zeros(4)
         self._chk(lib().zlh_conservation(self.h, level, out))
         return out
+
+    # ---- learned closure ----
+    def mlp_set(self, model):
+        f = lambda v: np.ascontiguousarray(v, np.float32)
+        self._chk(lib().zlh_mlp_set(self.h, f(model.params), f(model.mean), f(model.std), f(model.fmin),
+                                    f(model.fmax), float(model.nu_scale)))
+
+    def mlp_gradient(self, params, x, t):
+        f = lambda v: np.ascontiguousarray(v, np.float32)
+        g = np.zeros(len(params), np.float32)
+        loss = np.zeros(1, np.float32)
+        self._chk(lib().zlh_mlp_gradient(self.h, f(params), f(x), f(t), len(t), g, loss))
+        return float(loss[0]), g
+
+    def mlp_train(self, params, x, t, steps, batch, lr, seed=20261202):
+        f = lambda v: np.ascontiguousarray(v, np.float32)
+        p = f(params).copy()
+        hist = np.zeros(steps, np.float32)
+        self._chk(lib().zlh_mlp_train(self.h, f(x), f(t), len(t), steps, batch, lr, seed, p, hist))
+        return p, hist
+
+    def mlp_eval(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros(len(x), np.float32)
+        self._chk(lib().zlh_mlp_eval(self.h, x, len(x), y))
+        return y
+
+    def closure_features(self, level=0):
+        n = self.n(level)
+        x = np.zeros(n * 9, np.float32)
+        nu = np.zeros(n, np.float32)
+        fb = np.zeros(n, np.int32)
+        self._chk(lib().zlh_closure_features(self.h, level, x, nu, fb))
+        return x.reshape(n, 9), nu, fb
 
     def synchronize(self):
         self._chk(lib().zlh_synchronize(self.h))
