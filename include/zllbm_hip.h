@@ -147,8 +147,12 @@ zlh_status zlh_mlp_gradient(zlh_ctx *ctx, const float *params, const float *x, c
  * loss_history (may be null) receives each minibatch loss */
 zlh_status zlh_mlp_train(zlh_ctx *ctx, const float *x, const float *t, int64_t n, int steps, int batch,
                          double lr, uint64_t seed, float *params, float *loss_history);
-/* network output (nu_t / nu_scale) of normalised inputs with the parameters set by zlh_mlp_set */
-zlh_status zlh_mlp_eval(zlh_ctx *ctx, const float *x, int64_t n, float *y);
+/* network forms: FP32 on the vector units (the FP64 solver's collision) and BF16 operands with
+ * FP32 accumulation on the matrix cores, 64 nodes per wavefront (the FP32 solver's collision) */
+enum { ZLH_MLP_FP32 = 0, ZLH_MLP_BF16_MFMA = 1 };
+/* network output (nu_t / nu_scale) of normalised inputs [n][9] with the parameters set by
+ * zlh_mlp_set, in the given form */
+zlh_status zlh_mlp_eval(zlh_ctx *ctx, const float *x, int64_t n, int form, float *y);
 /* per node of a level (key order): raw closure inputs [n][9], the closure's nu_t and whether the
  * node falls back to Smagorinsky, from the current post-stream populations (synchronising) */
 zlh_status zlh_closure_features(zlh_ctx *ctx, int level, float *features, float *nu_t, int32_t *fallback);

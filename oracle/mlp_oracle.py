@@ -116,3 +116,22 @@ def smagorinsky_nu_t(f, tau0, cs, dims=3):
     q = np.sqrt(2.0 * np.einsum("nab,nab->n", pi, pi))
     tau_t = 0.5 * (np.sqrt(tau0 ** 2 + 18 * cs ** 2 * q) - tau0)
     return tau_t / 3.0
+
+
+def bf16_round(x):
+    """Round to the nearest BF16 value (ties to even), returned as float64."""
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward_bf16(p, X):
+    """The network with BF16 operands and wide accumulation, as the matrix-core form evaluates
+    it: inputs, W1, b1, W2, b2 and the layer-1 activations rounded to BF16 (the biases enter the
+    products as weights of a constant input 1); w3, b3 and the accumulations exact."""
+    W1, b1, W2, b2, w3, b3 = unpack(p)
+    Z1 = bf16_round(X) @ bf16_round(W1).T + bf16_round(b1)
+    H1 = bf16_round(silu(Z1))
+    Z2 = H1 @ bf16_round(W2).T + bf16_round(b2)
+    return softplus(silu(Z2) @ w3 + b3)

@@ -44,3 +44,24 @@ def test_argument_validation_without_device():
     assert L.zlh_last_error(None) == b"null context"
     assert L.zlh_step_level(None, 0, 1) == -1
     assert L.zlh_build(None) == -1
+
+
+def _declared_arity():
+    """Parameter count of every function declared in include/*.h."""
+    out = {}
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            for name, args in re.findall(r"\b(zlh_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", text, flags=re.S):
+                args = args.strip()
+                out[name] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def test_binding_arity_matches_header():
+    L = hip.lib()
+    arity = _declared_arity()
+    for name in hip.EXPORTED:
+        assert name in arity, name
+        assert len(getattr(L, name).argtypes) == arity[name], name

@@ -137,6 +137,22 @@ def test_const_force_fp64():
         assert_close(s.f(0), o.f(0))
 
 
+@pytest.mark.parametrize("dims", [2, 3])
+@pytest.mark.parametrize("compressible", [True, False])
+@pytest.mark.parametrize("coll", [hip.SRT, hip.MRT])
+def test_forced_les_fp64(dims, compressible, coll):
+    """Forcing (Guo terms, force-corrected stress) with the Smagorinsky closure, both equilibrium
+    forms, on a periodic channel with bounce-back walls."""
+    grid = g.uniform_grid(dims, (8, 12, 6)[:dims])
+    bc = sc.all_bc(dims, sc.PER)
+    bc[g.YMIN] = (sc.BB, (0, 0, 0))
+    bc[g.YMAX] = (sc.BB, (0.02, 0, 0))
+    o, s = run_pair(grid, 60, init=sc.tgv_fields(grid, 0.03, noise=1e-3), collision=(coll,), nu0=0.01, bc=bc,
+                    has_force=True, const_force=(2e-5, -1e-5, 5e-6)[:dims], les=1, cs=0.15,
+                    compressible=compressible)
+    assert_close(s.f(0), o.f(0))
+
+
 def _macros(f, dims):
     cx, cy, cz = (np.array(v, dtype=np.float64) for v in _lat(dims))
     rho = f.sum(1)

@@ -100,3 +100,25 @@ def test_filtered_samples_on_synthetic_field():
     div = kx * uh[0] + ky * uh[1] + kz * uh[2]
     assert np.max(np.abs(div)) < 1e-9 * np.max(np.abs(uh[0]))
     np.testing.assert_allclose(np.sqrt(np.mean(uf ** 2)), 0.05 / np.sqrt(3.0), rtol=1e-12)
+
+
+def test_bf16_round_known_values():
+    v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 3.0e-5, 65504.0], np.float32)
+    r = mo.bf16_round(v)
+    # 1 + 2^-8 is a tie and rounds to even (1.0); 1 + 3 * 2^-8 rounds up to 1 + 2^-6
+    np.testing.assert_array_equal(r[:4], [1.0, 1.0, 1.0 + 2 ** -6, -2.5])
+    assert abs(r[4] - 3.0e-5) <= 2 ** -8 * 3.0e-5
+    assert r[5] == 65536.0
+    # a BF16 value has at most 8 significant bits
+    m, _ = np.frexp(r)
+    assert np.all((m * 2 ** 8) == np.round(m * 2 ** 8))
+
+
+def test_forward_bf16_close_to_exact():
+    rng = np.random.default_rng(3)
+    p = rng.normal(0, 0.3, mo.NPARAM)
+    X = rng.normal(size=(200, 9))
+    y, _ = mo.forward(p, X)
+    yb = mo.forward_bf16(p, X)
+    rel = np.abs(yb - y) / np.abs(y)
+    assert 0 < rel.max() < 3e-2 and np.median(rel) < 5e-3

@@ -143,7 +143,9 @@ def filtered_samples(u_fine, ratio, tau0_coarse):
     nu_t = np.maximum(0.0, -np.einsum("...ab,...ab->...", tau_d, S) / np.maximum(2.0 * SS, 1e-30))
     rho = np.ones(SS.shape)
     pi = -(2.0 / 3.0) * rho[..., None] * (tau0_coarse + 3.0 * nu_t)[..., None] * S6
-    x = features_from_stress(pi.reshape(-1, 6), rho.ravel(), ub.reshape(-1, 3), tau0_coarse)
+    # the collision evaluates the inputs in FP32; keep them so, so that the training range
+    # bounds compare exactly
+    x = features_from_stress(pi.reshape(-1, 6), rho.ravel(), ub.reshape(-1, 3), tau0_coarse).astype(np.float32)
     return x, nu_t.ravel(), pi.reshape(-1, 6)
 
 

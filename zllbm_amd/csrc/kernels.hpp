@@ -38,6 +38,14 @@ struct StepParams {
     const MlpDev *mlp;     // learned closure (les == 2)
 };
 
+// Populations of a level are brick-major, [brick][q][64 lanes]: the Q x 64 values of one brick
+// are one contiguous run (Q * 256 B in FP32), so a wave's pulls from its own and its 26
+// neighbour bricks touch few pages and its store is a single run.  slot = brick * 64 + lane.
+template <int Q>
+__host__ __device__ __forceinline__ int64_t pop_at(int q, int64_t slot) {
+    return ((slot >> 6) * Q + q) * 64 + (slot & 63);
+}
+
 template <typename T>
 struct BrickLevel {
     int64_t nbrick;
@@ -49,23 +57,6 @@ struct BrickLevel {
     const int32_t *auxbase;   // [nbrick]
     const uint32_t *aux_missing;  // [naux]
     const int32_t *aux_slot;  // [naux]
-};
-
-template <int D>
-struct MrtTable {
-    static constexpr int Q = Lattice<D>::Q;
-    int m[Q][Q];
-    int n2[Q];
-    constexpr MrtTable() : m(), n2() {
-        for (int k = 0; k < Q; ++k) {
-            int s = 0;
-            for (int i = 0; i < Q; ++i) {
-                m[k][i] = Lattice<D>::moment(k, i);
-                s += m[k][i] * m[k][i];
-            }
-            n2[k] = s;
-        }
-    }
 };
 
 // shift applied to stored populations: FP32 stores f - w_i
@@ -119,118 +110,277 @@ __device__ __forceinline__ void macro(const C (&f)[Lattice<D>::Q], const C *F, C
     }
 }
 
-// One node collision on stored populations (in place): func_collision_node_
-// (lbm_solver_interface.cpp:266-300), Smagorinsky SGS time (les_models/lbm_les_smagorinsky.cpp:22-94),
-// SRT (lbm_collision_model_srt.cpp:20-39) and MRT (lbm_collision_model_mrt.cpp:114-143).
-// The MRT transform is applied as M^T diag(1/|m_k|^2) (S M neq - D M F dt) with the constant
-// integer basis, which equals M^{-1} S M (f - feq) - M^{-1} D M F dt.
-template <int D, bool SHIFT, int COLL, typename C>
-__device__ __forceinline__ void collide(C (&f)[Lattice<D>::Q], const C *F, const StepParams &p,
-                                        C &rho_out, C (&u)[3]) {
+// sub-grid model code of a collision variant: 0 none, 1 Smagorinsky with tau0 + tau_t (physical),
+// 2 learned closure, 3 Smagorinsky with the reference's tau_eff = tau_t
+__host__ __device__ constexpr int les_code(int les, int les_tau_mode) {
+    return les == 1 ? (les_tau_mode ? 1 : 3) : les;
+}
+
+// relaxation time of a node: tau0, or with a sub-grid model (les_code) from the non-equilibrium
+// stress pi = (xx, yy, zz, xy, xz, yz): Smagorinsky (les_models/lbm_les_smagorinsky.cpp:22-94;
+// the reference sets tau_eff = tau_t, lbm_solver_interface.cpp:276-295) or the learned closure
+template <typename C>
+__device__ __forceinline__ C relaxation_time(int les, const C *pi, C rho, C drho, const C (&u)[3], const StepParams &p,
+                                             const MlpOps *ops) {
+    const C t0 = C(p.tau0);
+    if (!les) return t0;
+    if (les != 2) {
+        const C pi2 = pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2] + C(2) * (pi[3] * pi[3] + pi[4] * pi[4] + pi[5] * pi[5]);
+        const C tau_t = C(0.5) * (sqrt(t0 * t0 + C(p.cs18) * sqrt(C(2) * pi2)) - t0);
+        return les == 1 ? t0 + tau_t : tau_t;
+    }
+    if constexpr (std::is_same<C, float>::value) {
+        // BF16 MFMA network across the wavefront (callers keep every lane here)
+        return t0 + mlp_tau_t_wave(*p.mlp, ops ? *ops : p.mlp->ops, pi, rho, drho, u, t0, C(p.cs18));
+    } else {
+        bool fb;
+        return t0 + mlp_tau_t<C>(*p.mlp, pi, rho, drho, u, t0, C(p.cs18), &fb);
+    }
+}
+
+// forcing correction of the non-equilibrium stress (CalSgsRelaxationTimeWithForce,
+// les_models/lbm_les_smagorinsky.cpp:48-94)
+template <int D, typename C>
+__device__ __forceinline__ void stress_force_terms(C *pi, const C *F, const C (&u)[3], C dt) {
+    pi[0] += dt * F[0] * u[0];
+    pi[1] += dt * F[1] * u[1];
+    pi[3] += C(0.5) * dt * (F[0] * u[0] + F[1] * u[1]);
+    if (D == 3) {
+        pi[2] += dt * F[2] * u[2];
+        pi[4] += C(0.5) * dt * (F[0] * u[0] + F[2] * u[2]);
+        pi[5] += C(0.5) * dt * (F[1] * u[1] + F[2] * u[2]);
+    }
+}
+
+// non-equilibrium stress sum_q c c (f - feq) summed per direction (xx, yy, zz, xy, xz, yz)
+template <int D, bool SHIFT, typename C>
+__device__ __forceinline__ void neq_stress(const C (&f)[Lattice<D>::Q], C rho, C drho, const C (&u)[3], int compressible,
+                                           C *pi) {
+    using L = Lattice<D>;
+    const C usq = u[0] * u[0] + u[1] * u[1] + (D == 3 ? u[2] * u[2] : C(0));
+#pragma unroll
+    for (int c = 0; c < 6; ++c) pi[c] = C(0);
+#pragma unroll
+    for (int q = 1; q < L::Q; ++q) {
+        const C d = f[q] - feq_stored<D, SHIFT, C>(q, rho, drho, u[0], u[1], u[2], usq, compressible);
+        const int ax = L::cx[q], ay = L::cy[q], az = L::cz[q];
+        if (ax * ax) pi[0] += d;
+        if (ay * ay) pi[1] += d;
+        if (ax * ay) pi[3] += C(ax * ay) * d;
+        if (D == 3) {
+            if (az * az) pi[2] += d;
+            if (ax * az) pi[4] += C(ax * az) * d;
+            if (ay * az) pi[5] += C(ay * az) * d;
+        }
+    }
+}
+
+// MRT collision in moment space (LbmMrtCollisionOpt::CollisionOperator,
+// lbm_collision_model_mrt.cpp:114-143: f* = f - M^-1 S M (f - feq) + M^-1 (I - S/2) M F_i dt).
+// The moments m = M f come from pair sums/differences of opposite populations; M feq, M F_i and
+// the stress sum_q c c (f - feq) are closed forms in the moments (MomentSpace), so no per-
+// direction equilibrium is formed; the post-collision moments return through
+// f* = M^T diag(1/|M_k|^2) m*.  Stored (shifted) populations map to shifted moments
+// m~ = m - M w, and M^-1 (M w) = w keeps the stored form exact.
+template <int D, bool SHIFT, typename C, int VAR>
+__device__ __forceinline__ void collide_mrt(C (&f)[Lattice<D>::Q], const C *F, const StepParams &p, C &rho_out,
+                                            C (&u)[3], const MlpOps *ops) {
     using L = Lattice<D>;
     constexpr int Q = L::Q;
+    constexpr MomentSpace<D> S{};
+    constexpr int NP = S.NP;
+    const int les = VAR >= 0 ? (VAR >> 1) : les_code(p.les, p.les_tau_mode);
+    const bool forced = VAR >= 0 ? (VAR & 1) != 0 : p.has_force != 0;
+    const C dt = C(p.dt);
+    C ps[NP], pd[NP];
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        ps[j] = f[S.rep[j]] + f[S.opp[j]];
+        pd[j] = f[S.rep[j]] - f[S.opp[j]];
+    }
+    C mm[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        C acc = S.even[k] ? C(S.m[k][0]) * f[0] : C(0);
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            const int c = S.m[k][S.rep[j]];
+            const C v = S.even[k] ? ps[j] : pd[j];
+            if (c == 1) acc += v;
+            else if (c == -1) acc -= v;
+            else if (c != 0) acc += C(c) * v;
+        }
+        mm[k] = acc;
+    }
+    // macroscopic variables (CalMacro*/CalMacroForce*, lbm_solver_interface.cpp:477-592): rows 0
+    // and 3, 5 (, 7) are the density and the momentum
+    C rho, drho, base;
+    if (SHIFT) { drho = mm[0]; rho = C(1) + drho; base = drho; }
+    else { rho = mm[0]; drho = rho - C(1); base = rho; }
+    C j[3] = {mm[3], mm[5], D == 3 ? mm[7] : C(0)};
+    if (forced) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) j[a] += C(0.5) * F[a] * dt;
+    }
+    const C rhoc = p.compressible ? rho : C(1);
+    {
+        const C inv = C(1) / rhoc;
+        u[0] = p.compressible ? j[0] * inv : j[0];
+        u[1] = p.compressible ? j[1] * inv : j[1];
+        u[2] = D == 3 ? (p.compressible ? j[2] * inv : j[2]) : C(0);
+    }
+    rho_out = rho;
+    const C uu[3][3] = {{u[0] * u[0], u[0] * u[1], u[0] * u[2]},
+                        {u[1] * u[0], u[1] * u[1], u[1] * u[2]},
+                        {u[2] * u[0], u[2] * u[1], u[2] * u[2]}};
+    // non-equilibrium moments of the rows that relax (s_k != 0)
+    C nm[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        nm[k] = C(0);
+        if (!L::viscous_row[k] && L::s_base[k] == 0.) continue;
+        C poly = C(0);
+#pragma unroll
+        for (int x = 0; x < D; ++x) {
+            if (S.b[k][x] != 0.) poly += C(S.b[k][x]) * u[x];
+#pragma unroll
+            for (int y = 0; y < D; ++y) {
+                const double cxy = S.g[k][x][y] - (x == y ? 1.5 * S.a[k] : 0.);
+                if (cxy != 0.) poly += C(cxy) * uu[x][y];
+            }
+        }
+        C meq = rhoc * poly;
+        if (S.a[k] != 0.) meq += C(S.a[k]) * base;
+        nm[k] = mm[k] - meq;
+    }
+    C pi[6];
+    if (les == 3) {
+        // the reference's tau_eff = tau_t divides by a stress that is exactly zero at equilibrium;
+        // the per-direction sum reproduces that zero (and the resulting non-finite state)
+        neq_stress<D, SHIFT, C>(f, rho, drho, u, p.compressible, pi);
+        if (forced) stress_force_terms<D, C>(pi, F, u, dt);
+    } else if (les) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            C acc = C(0);
+#pragma unroll
+            for (int k = 0; k < Q; ++k)
+                if (S.pi[c][k] != 0. && (L::viscous_row[k] || L::s_base[k] != 0.)) acc += C(S.pi[c][k]) * nm[k];
+            pi[c] = acc;
+        }
+        if (forced) stress_force_terms<D, C>(pi, F, u, dt);
+    }
+    C uF = C(0);
+    if (forced) uF = u[0] * F[0] + u[1] * F[1] + (D == 3 ? u[2] * F[2] : C(0));
+    // post-collision moments over |M_k|^2; the rows with fixed rates first, so that only the
+    // viscous rows' moments stay live across the sub-grid model
+    C y[Q];
+    auto post = [&](int k, C sk, bool relaxes) {
+        C v = relaxes ? mm[k] - sk * nm[k] : mm[k];
+        if (forced) {
+            // Guo forcing moment b.F - 3 (u.F) a + 2 u.g.F
+            C fm = C(0);
+#pragma unroll
+            for (int x = 0; x < D; ++x) {
+                if (S.b[k][x] != 0.) fm += C(S.b[k][x]) * F[x];
+#pragma unroll
+                for (int z = 0; z < D; ++z)
+                    if (S.g[k][x][z] != 0.) fm += C(2. * S.g[k][x][z]) * u[x] * F[z];
+            }
+            if (S.a[k] != 0.) fm -= C(3. * S.a[k]) * uF;
+            v += (relaxes ? C(1) - C(0.5) * sk : C(1)) * dt * fm;
+        }
+        y[k] = v * C(1.0 / double(S.n2[k]));
+    };
+#pragma unroll
+    for (int k = 0; k < Q; ++k)
+        if (!L::viscous_row[k]) post(k, C(L::s_base[k]), L::s_base[k] != 0.);
+    const C tau = les ? relaxation_time<C>(les, pi, rho, drho, u, p, ops) : C(p.tau0);
+    const C s_nu = C(1) / tau;
+#pragma unroll
+    for (int k = 0; k < Q; ++k)
+        if (L::viscous_row[k]) post(k, s_nu, true);
+    // back to populations: even rows give the pair mean, odd rows the pair difference
+    {
+        C acc = C(0);
+#pragma unroll
+        for (int k = 0; k < Q; ++k)
+            if (S.even[k] && S.m[k][0] != 0) acc += C(S.m[k][0]) * y[k];
+        f[0] = acc;
+    }
+#pragma unroll
+    for (int jp = 0; jp < NP; ++jp) {
+        C e = C(0), o = C(0);
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+            const int c = S.m[k][S.rep[jp]];
+            if (c == 0) continue;
+            C &acc = S.even[k] ? e : o;
+            if (c == 1) acc += y[k];
+            else if (c == -1) acc -= y[k];
+            else acc += C(c) * y[k];
+        }
+        f[S.rep[jp]] = e + o;
+        f[S.opp[jp]] = e - o;
+    }
+}
+
+// One node collision on stored populations (in place): func_collision_node_
+// (lbm_solver_interface.cpp:266-300) with SRT (lbm_collision_model_srt.cpp:20-39) or MRT
+// (collide_mrt).  VAR = 2 * les_code + forced fixes the sub-grid model and the forcing at compile time
+// (the bulk kernel then keeps only the registers its variant needs); VAR < 0 reads both from p.
+template <int D, bool SHIFT, int COLL, typename C, int VAR = -1>
+__device__ __forceinline__ void collide(C (&f)[Lattice<D>::Q], const C *F, const StepParams &p,
+                                        C &rho_out, C (&u)[3], const MlpOps *ops = nullptr) {
+    if constexpr (COLL == 1) {
+        collide_mrt<D, SHIFT, C, VAR>(f, F, p, rho_out, u, ops);
+        return;
+    }
+    using L = Lattice<D>;
+    constexpr int Q = L::Q;
+    const int les = VAR >= 0 ? (VAR >> 1) : les_code(p.les, p.les_tau_mode);
+    const bool forced = VAR >= 0 ? (VAR & 1) != 0 : p.has_force != 0;
     const C dt = C(p.dt);
     C rho, drho;
-    macro<D, SHIFT, C>(f, F, dt, p.has_force, p.compressible, rho, drho, u);
+    macro<D, SHIFT, C>(f, F, dt, forced, p.compressible, rho, drho, u);
     rho_out = rho;
     C usq = u[0] * u[0] + u[1] * u[1] + (D == 3 ? u[2] * u[2] : C(0));
     C neq[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q)
         neq[q] = f[q] - feq_stored<D, SHIFT, C>(q, rho, drho, u[0], u[1], u[2], usq, p.compressible);
-    C fi[Q];
-    if (p.has_force) {
-        C uF = u[0] * F[0] + u[1] * F[1] + (D == 3 ? u[2] * F[2] : C(0));
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            C cF = C(L::cx[q]) * F[0] + C(L::cy[q]) * F[1] + (D == 3 ? C(L::cz[q]) * F[2] : C(0));
-            C cu = C(L::cx[q]) * u[0] + C(L::cy[q]) * u[1] + (D == 3 ? C(L::cz[q]) * u[2] : C(0));
-            fi[q] = C(L::w[q]) * (C(3) * (cF - uF) + C(9) * cu * cF);
-        }
-    }
     C tau = C(p.tau0);
-    if (p.les) {
-        C pxx = 0, pxy = 0, pyy = 0, pzz = 0, pxz = 0, pyz = 0;
+    if (les) {
+        C pi[6] = {C(0), C(0), C(0), C(0), C(0), C(0)};
 #pragma unroll
         for (int q = 1; q < Q; ++q) {
             const int ax = L::cx[q], ay = L::cy[q], az = L::cz[q];
-            if (ax * ax) pxx += neq[q];
-            if (ay * ay) pyy += neq[q];
-            if (ax * ay) pxy += C(ax * ay) * neq[q];
+            if (ax * ax) pi[0] += neq[q];
+            if (ay * ay) pi[1] += neq[q];
+            if (ax * ay) pi[3] += C(ax * ay) * neq[q];
             if (D == 3) {
-                if (az * az) pzz += neq[q];
-                if (ax * az) pxz += C(ax * az) * neq[q];
-                if (ay * az) pyz += C(ay * az) * neq[q];
+                if (az * az) pi[2] += neq[q];
+                if (ax * az) pi[4] += C(ax * az) * neq[q];
+                if (ay * az) pi[5] += C(ay * az) * neq[q];
             }
         }
-        if (p.has_force) {
-            pxx += dt * F[0] * u[0];
-            pyy += dt * F[1] * u[1];
-            pxy += C(0.5) * dt * (F[0] * u[0] + F[1] * u[1]);
-            if (D == 3) {
-                pzz += dt * F[2] * u[2];
-                pxz += C(0.5) * dt * (F[0] * u[0] + F[2] * u[2]);
-                pyz += C(0.5) * dt * (F[1] * u[1] + F[2] * u[2]);
-            }
-        }
-        const C t0 = C(p.tau0);
-        if (p.les == 1) {
-            C pi2 = pxx * pxx + pyy * pyy + pzz * pzz + C(2) * (pxy * pxy + pxz * pxz + pyz * pyz);
-            C tau_t = C(0.5) * (sqrt(t0 * t0 + C(p.cs18) * sqrt(C(2) * pi2)) - t0);
-            tau = p.les_tau_mode ? t0 + tau_t : tau_t;
-        } else {
-            const C pi[6] = {pxx, pyy, pzz, pxy, pxz, pyz};
-            bool fb;
-            tau = t0 + mlp_tau_t<C>(*p.mlp, pi, rho, drho, u, t0, C(p.cs18), &fb);
-        }
+        if (forced) stress_force_terms<D, C>(pi, F, u, dt);
+        tau = relaxation_time<C>(les, pi, rho, drho, u, p, ops);
     }
-    if (COLL == 0) {
-        const C inv_tau = C(1) / tau;
-        const C fcoef = (C(1) - C(0.5) * inv_tau) * dt;
+    const C inv_tau = C(1) / tau;
+    const C fcoef = (C(1) - C(0.5) * inv_tau) * dt;
+    C uF = C(0);
+    if (forced) uF = u[0] * F[0] + u[1] * F[1] + (D == 3 ? u[2] * F[2] : C(0));
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            C v = f[q] - neq[q] * inv_tau;
-            if (p.has_force) v += fcoef * fi[q];
-            f[q] = v;
+    for (int q = 0; q < Q; ++q) {
+        C v = f[q] - neq[q] * inv_tau;
+        if (forced) {
+            // Guo term w [3 (c - u).F + 9 (c.u)(c.F)] (CalForcingTerming3D, lbm_solver_interface.cpp:443-470)
+            const C cF = C(L::cx[q]) * F[0] + C(L::cy[q]) * F[1] + (D == 3 ? C(L::cz[q]) * F[2] : C(0));
+            const C cu = C(L::cx[q]) * u[0] + C(L::cy[q]) * u[1] + (D == 3 ? C(L::cz[q]) * u[2] : C(0));
+            v += fcoef * C(L::w[q]) * (C(3) * (cF - uF) + C(9) * cu * cF);
         }
-    } else {
-        constexpr MrtTable<D> T{};
-        const C s_nu = C(1) / tau;
-        C r[Q];
-#pragma unroll
-        for (int k = 0; k < Q; ++k) {
-            C mk = C(0);
-#pragma unroll
-            for (int i = 0; i < Q; ++i) {
-                if (T.m[k][i] == 1) mk += neq[i];
-                else if (T.m[k][i] == -1) mk -= neq[i];
-                else if (T.m[k][i] != 0) mk += C(T.m[k][i]) * neq[i];
-            }
-            const C sk = L::viscous_row[k] ? s_nu : C(L::s_base[k]);
-            C v = sk * mk;
-            if (p.has_force) {
-                C mf = C(0);
-#pragma unroll
-                for (int i = 0; i < Q; ++i) {
-                    if (T.m[k][i] == 1) mf += fi[i];
-                    else if (T.m[k][i] == -1) mf -= fi[i];
-                    else if (T.m[k][i] != 0) mf += C(T.m[k][i]) * fi[i];
-                }
-                v -= (C(1) - C(0.5) * sk) * dt * mf;
-            }
-            r[k] = v * C(1.0 / double(T.n2[k]));
-        }
-#pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            C acc = C(0);
-#pragma unroll
-            for (int k = 0; k < Q; ++k) {
-                if (T.m[k][i] == 1) acc += r[k];
-                else if (T.m[k][i] == -1) acc -= r[k];
-                else if (T.m[k][i] != 0) acc += C(T.m[k][i]) * r[k];
-            }
-            f[i] -= acc;
-        }
+        f[q] = v;
     }
 }
 
@@ -244,24 +394,26 @@ __device__ __forceinline__ int64_t xcd_group(int64_t ngroups) {
     return g < ngroups ? g : -1;
 }
 
-// Fused pull-stream + collide over all nodes of a level (one wavefront per brick).
-template <int D, typename T, typename C, bool SHIFT, int COLL>
-__global__ __launch_bounds__(256) void k_stream_collide(BrickLevel<T> lv, const T *__restrict__ fin,
-                                                        T *__restrict__ fout, T *__restrict__ aux_f,
-                                                        StepParams p) {
+// Fused pull-stream + collide of one brick by one wavefront.
+template <int D, typename T, typename C, bool SHIFT, int COLL, int VAR>
+__device__ __forceinline__ void stream_collide_brick(const BrickLevel<T> &lv, const T *__restrict__ fin,
+                                                     T *__restrict__ fout, T *__restrict__ aux_f, const StepParams &p,
+                                                     int64_t b, const MlpOps *ops) {
     using L = Lattice<D>;
     using B = Brick<D>;
     constexpr int Q = L::Q;
     constexpr int E = B::edge;
     constexpr int NN = B::nnbr;
-    const int64_t g = xcd_group((lv.nbrick + 3) >> 2);
-    if (g < 0) return;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    const int64_t b = g * 4 + wave;
-    if (b >= lv.nbrick) return;
+    // the FP32 learned closure is a wavefront-wide MFMA product: there every lane stays to the
+    // collision (empty lanes on the rest state) and only the stores are predicated
+    constexpr bool kWave = mlp_on_mfma<C, VAR>();
+    int lane = threadIdx.x & 63;
+    // in the persistent loop the lane is made opaque per brick, which keeps its stencil offsets
+    // from being hoisted out of the loop (they would hold ~50 registers across it)
+    if constexpr (kWave) asm volatile("" : "+v"(lane));
     const uint64_t occ = lv.occ[b];
-    if (!((occ >> lane) & 1ull)) return;
+    const bool has_node = (occ >> lane) & 1ull;
+    if (!kWave && !has_node) return;
     const uint64_t am = lv.auxmask[b];
     const bool is_aux = (am >> lane) & 1ull;
     int64_t ai = 0;
@@ -275,9 +427,9 @@ __global__ __launch_bounds__(256) void k_stream_collide(BrickLevel<T> lv, const 
     for (int k = 0; k < NN; ++k) nb[k] = lv.nbr[b * NN + k];
     const int lx = B::lane_coord(lane, 0), ly = B::lane_coord(lane, 1);
     const int lz = D == 3 ? B::lane_coord(lane, 2) : 0;
-    const int64_t ns = lv.nslot;
     C f[Q];
-    f[0] = C(fin[b * 64 + lane]);
+    const T *__restrict__ own = fin + b * Q * 64 + lane;
+    f[0] = kWave && !has_node ? C(0) : C(own[0]);
 #pragma unroll
     for (int q = 1; q < Q; ++q) {
         const int sx = lx - L::cx[q], sy = ly - L::cy[q], sz = lz - L::cz[q];
@@ -293,19 +445,58 @@ __global__ __launch_bounds__(256) void k_stream_collide(BrickLevel<T> lv, const 
                           : (ey ? (ex ? nb[base + dx + dy] : nb[base + dy])
                                 : (ex ? nb[base + dx] : nb[base]));
         const int sl = B::lane_of(sx & (E - 1), sy & (E - 1), sz & (E - 1));
-        if (missing & (1u << q)) f[q] = aux_f[q * lv.naux + ai];
-        else f[q] = C(fin[q * ns + (int64_t)sb * 64 + sl]);
+        if (kWave && !has_node) f[q] = C(0);
+        else if (missing & (1u << q)) f[q] = aux_f[q * lv.naux + ai];
+        else f[q] = C(fin[((int64_t)sb * Q + q) * 64 + sl]);
     }
     if (is_aux) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) aux_f[q * lv.naux + ai] = T(f[q]);
-        return;
+        if (!kWave) return;
     }
     C F[3] = {C(p.const_force[0]), C(p.const_force[1]), C(p.const_force[2])};
     C rho, u[3];
-    collide<D, SHIFT, COLL, C>(f, F, p, rho, u);
+    collide<D, SHIFT, COLL, C, VAR>(f, F, p, rho, u, ops);
+    if (kWave && (!has_node || is_aux)) return;
+    // streaming stores: the next substep reads these values through other CUs' caches
+    T *__restrict__ dst = fout + b * Q * 64 + lane;
 #pragma unroll
-    for (int q = 0; q < Q; ++q) fout[q * ns + b * 64 + lane] = T(f[q]);
+    for (int q = 0; q < Q; ++q) __builtin_nontemporal_store(T(f[q]), dst + q * 64);
+}
+
+// Fused pull-stream + collide over all nodes of a level, one wavefront per brick, four bricks
+// per workgroup.
+template <int D, typename T, typename C, bool SHIFT, int COLL, int VAR>
+__global__ __launch_bounds__(256) void k_stream_collide(BrickLevel<T> lv, const T *__restrict__ fin,
+                                                        T *__restrict__ fout, T *__restrict__ aux_f,
+                                                        StepParams p) {
+    const int64_t g = xcd_group((lv.nbrick + 3) >> 2);
+    if (g < 0) return;
+    const int64_t b = g * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (b >= lv.nbrick) return;
+    stream_collide_brick<D, T, C, SHIFT, COLL, VAR>(lv, fin, fout, aux_f, p, b, nullptr);
+}
+
+// The same with the FP32 learned closure: each workgroup stages the network's operand images in
+// LDS once and its four waves then take four rounds of four adjacent bricks (16 per workgroup),
+// which amortises the staging to a few percent of the population traffic.
+// Register budget: 4 waves per SIMD unforced, 3 forced (whose Guo terms would spill at 4).
+constexpr int kMlpBricksPerGroup = 16;
+template <int D, typename T, typename C, bool SHIFT, int COLL, int VAR>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((VAR & 1) ? 3 : 4)))
+void k_stream_collide_mlp(BrickLevel<T> lv, const T *__restrict__ fin, T *__restrict__ fout, T *__restrict__ aux_f,
+                          StepParams p) {
+    static_assert(mlp_on_mfma<C, VAR>(), "kernel of the FP32 learned closure");
+    __shared__ MlpOps ops;
+    const int64_t g = xcd_group((lv.nbrick + kMlpBricksPerGroup - 1) / kMlpBricksPerGroup);
+    if (g < 0) return;                       // uniform per workgroup: before the barrier
+    stage_mlp_ops(ops, p.mlp->ops);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+#pragma unroll 1
+    for (int r = 0; r < kMlpBricksPerGroup / 4; ++r) {
+        const int64_t b = g * kMlpBricksPerGroup + r * 4 + wave;
+        if (b < lv.nbrick) stream_collide_brick<D, T, C, SHIFT, COLL, VAR>(lv, fin, fout, aux_f, p, b, &ops);
+    }
 }
 
 // Collision of the aux nodes (deferred from the previous substep), writing post-collision
@@ -317,19 +508,23 @@ __global__ __launch_bounds__(256) void k_aux_collide(BrickLevel<T> lv, const T *
                                                      StepParams p) {
     constexpr int Q = Lattice<D>::Q;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= lv.naux) return;
+    // lanes past the end run the collision on the rest state (FP32 learned closure is
+    // wavefront-wide) and store nothing
+    const bool active = i < lv.naux;
+    if (!mlp_on_mfma<C, -1>() && !active) return;
     C f[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) f[q] = C(aux_f[q * lv.naux + i]);
+    for (int q = 0; q < Q; ++q) f[q] = active ? C(aux_f[q * lv.naux + i]) : C(SHIFT ? 0. : Lattice<D>::w[q]);
     C F[3] = {C(p.const_force[0]), C(p.const_force[1]), C(p.const_force[2])};
-    if (p.has_force && aux_force) {
+    if (active && p.has_force && aux_force) {
         F[0] += C(aux_force[i * 3]); F[1] += C(aux_force[i * 3 + 1]); F[2] += C(aux_force[i * 3 + 2]);
     }
     C rho, u[3];
     collide<D, SHIFT, COLL, C>(f, F, p, rho, u);
+    if (!active) return;
     const int64_t slot = lv.aux_slot[i];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) fa[q * lv.nslot + slot] = T(f[q]);
+    for (int q = 0; q < Q; ++q) fa[pop_at<Q>(q, slot)] = T(f[q]);
     aux_macro[i * 4] = double(rho);
     aux_macro[i * 4 + 1] = double(u[0]);
     aux_macro[i * 4 + 2] = double(u[1]);

@@ -95,6 +95,56 @@ constexpr int opposite(int q) {
     return -1;
 }
 
+// Moment-space tables of the MRT basis (all compile-time):
+//   pairs (rep, opp) of opposite directions and the parity of each row under c -> -c (even rows
+//   see only f_q + f_opp, odd rows only f_q - f_opp);
+//   equilibrium moments of feq = w rho (1 + 3 c.u + 4.5 (c.u)^2 - 1.5 u^2):
+//     M feq = rho (a + b.u + u.C.u), a = M w, b_a = 3 M (w c_a), g_ab = 4.5 M (w c_a c_b),
+//     C_ab = g_ab - 1.5 delta_ab a;
+//   Guo forcing moments M F_i = b.F - 3 (u.F) a + 2 u.g.F;
+//   second moments of a population vector from its moments: sum_q c_a c_b f_q = sum_k pi_ab,k m_k
+//     (pi_ab,k = sum_q c_a c_b M_kq / |M_k|^2, as M^-1 = M^T diag(1/|M_k|^2)).
+template <int D>
+struct MomentSpace {
+    using L = Lattice<D>;
+    static constexpr int Q = L::Q;
+    static constexpr int NP = (Q - 1) / 2;
+    int rep[NP], opp[NP];
+    bool even[Q];
+    int m[Q][Q];
+    int n2[Q];
+    double a[Q], b[Q][3], g[Q][3][3], pi[6][Q];
+    constexpr MomentSpace() : rep(), opp(), even(), m(), n2(), a(), b(), g(), pi() {
+        int np = 0;
+        for (int q = 1; q < Q; ++q) {
+            const int o = opposite<D>(q);
+            if (q < o) { rep[np] = q; opp[np] = o; ++np; }
+        }
+        const int ab[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+        for (int k = 0; k < Q; ++k) {
+            int s = 0;
+            even[k] = true;
+            for (int q = 0; q < Q; ++q) {
+                m[k][q] = L::moment(k, q);
+                s += m[k][q] * m[k][q];
+                if (L::moment(k, opposite<D>(q)) != m[k][q]) even[k] = false;
+            }
+            n2[k] = s;
+            for (int q = 0; q < Q; ++q) {
+                const int c[3] = {L::cx[q], L::cy[q], L::cz[q]};
+                const double mw = m[k][q] * L::w[q];
+                a[k] += mw;
+                for (int x = 0; x < 3; ++x) {
+                    b[k][x] += 3. * mw * c[x];
+                    for (int y = 0; y < 3; ++y) g[k][x][y] += 4.5 * mw * c[x] * c[y];
+                }
+                for (int j = 0; j < 6; ++j) pi[j][k] += double(c[ab[j][0]] * c[ab[j][1]] * m[k][q]);
+            }
+            for (int j = 0; j < 6; ++j) pi[j][k] /= double(s);
+        }
+    }
+};
+
 // brick geometry: 4x4x4 (3D) or 8x8 (2D) nodes = 64 lanes = one wavefront, lane = low 6 Morton bits
 template <int D> struct Brick;
 template <> struct Brick<2> {

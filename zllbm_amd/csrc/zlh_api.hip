@@ -163,24 +163,30 @@ __global__ void k_upload(BrickLevel<T> lv, const double *f_in, int64_t n, const 
                          const int32_t *node_aux, T *fa, T *fb, T *aux_f, StepParams p) {
     constexpr int Q = Lattice<D>::Q;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    // lanes without a bulk node run the collision on the rest state (the FP32 learned closure
+    // is wavefront-wide) and store nothing
+    const bool active = i < n;
+    if (!mlp_on_mfma<C, -1>() && !active) return;
     C f[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) f[q] = C(f_in[i * Q + q] - (SHIFT ? Lattice<D>::w[q] : 0.));
-    const int32_t ai = node_aux[i];
+    for (int q = 0; q < Q; ++q)
+        f[q] = active ? C(f_in[i * Q + q] - (SHIFT ? Lattice<D>::w[q] : 0.)) : C(SHIFT ? 0. : Lattice<D>::w[q]);
+    const int32_t ai = active ? node_aux[i] : -1;
     if (ai >= 0) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) aux_f[q * lv.naux + ai] = T(f[q]);
-        return;
     }
-    const int64_t slot = node_slot[i];
+    const int64_t slot = active ? node_slot[i] : 0;
+    if (active && ai < 0) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) fb[q * lv.nslot + slot] = T(f[q]);
+        for (int q = 0; q < Q; ++q) fb[pop_at<Q>(q, slot)] = T(f[q]);
+    }
     C F[3] = {C(p.const_force[0]), C(p.const_force[1]), C(p.const_force[2])};
     C rho, u[3];
     collide<D, SHIFT, COLL, C>(f, F, p, rho, u);
+    if (!active || ai >= 0) return;
 #pragma unroll
-    for (int q = 0; q < Q; ++q) fa[q * lv.nslot + slot] = T(f[q]);
+    for (int q = 0; q < Q; ++q) fa[pop_at<Q>(q, slot)] = T(f[q]);
 }
 
 // post-stream populations of every node, [Q][n] double in key order (unshifted)
@@ -202,7 +208,7 @@ __global__ void k_materialize(BrickLevel<T> lv, const int32_t *node_slot, const 
         if (ai >= 0) {
             v = double(aux_f[q * lv.naux + ai]);
         } else if (fresh || q == 0) {
-            v = double(last[q * lv.nslot + slot]);
+            v = double(last[pop_at<Q>(q, slot)]);
         } else {
             const int sx = lx - L::cx[q], sy = ly - L::cy[q], sz = lz - L::cz[q];
             const int ox = sx < 0 ? -1 : (sx >= B::edge ? 1 : 0);
@@ -211,7 +217,7 @@ __global__ void k_materialize(BrickLevel<T> lv, const int32_t *node_slot, const 
             const int k = D == 3 ? (oz + 1) * 9 + (oy + 1) * 3 + (ox + 1) : (oy + 1) * 3 + (ox + 1);
             const int64_t sb = lv.nbr[b * B::nnbr + k];
             const int sl = B::lane_of(sx & (B::edge - 1), sy & (B::edge - 1), sz & (B::edge - 1));
-            v = double(last[q * lv.nslot + sb * 64 + sl]);
+            v = double(last[pop_at<Q>(q, sb * 64 + sl)]);
         }
         out[q * n + i] = v + (SHIFT ? L::w[q] : 0.);
     }
@@ -261,7 +267,7 @@ __global__ void k_bc_bounce_back(int64_t nb, const int32_t *node_ai, const int32
     for (int k = 0; k < idx.n; ++k) {
         const int o = idx.inv[k];
         const double cu = L::cx[o] * ux + L::cy[o] * uy + (D == 3 ? L::cz[o] * uz : 0.);
-        aux_f[idx.ind[k] * naux + ai] = T(double(fa[o * nslot + slot]) - 6. * rho0 * L::w[o] * cu);
+        aux_f[idx.ind[k] * naux + ai] = T(double(fa[pop_at<L::Q>(o, slot)]) - 6. * rho0 * L::w[o] * cu);
     }
 }
 
@@ -371,6 +377,7 @@ struct zlh_ctx {
     DevBuf lattice_d;
     DevBuf mlp_d;           // MlpDev of the learned closure
     bool mlp_set = false;
+    int n_cu = 0;           // compute units of the context's device
 
     zlh_status fail(zlh_status code, const char *fmt, ...) {
         char buf[1024];
@@ -407,6 +414,15 @@ zlh_status upload(zlh_ctx *ctx, DevBuf &b, const std::vector<X> &v) {
 }
 
 inline unsigned blocks(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+// FP32 -> BF16 bit pattern, round to nearest even (NaN stays NaN)
+inline uint16_t bf16_bits(float v) {
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
 
 StepParams level_params(const zlh_ctx *ctx, int level) {
     StepParams p{};
@@ -471,6 +487,35 @@ int64_t find_key(const std::vector<uint64_t> &keys, uint64_t k) {
 // ==============================================================================================
 namespace {
 
+// bulk kernel variant: collision operator x (sub-grid model, forcing)
+template <int D, typename T, typename C, bool SHIFT, int COLL>
+void launch_stream_collide_v(zlh_ctx *ctx, int var, unsigned grid, const BrickLevel<T> &v, T *fa, T *fb, T *aux_f,
+                             const StepParams &sp) {
+#define ZLH_SC(V)                                                                                             \
+    case V:                                                                                                   \
+        if constexpr (mlp_on_mfma<C, V>()) {                                                                  \
+            const int64_t ng = (v.nbrick + kMlpBricksPerGroup - 1) / kMlpBricksPerGroup;                     \
+            hipLaunchKernelGGL((k_stream_collide_mlp<D, T, C, SHIFT, COLL, V>), dim3((unsigned)((ng + 7) / 8 * 8)), \
+                               dim3(256), 0, ctx->stream, v, fa, fb, aux_f, sp);                              \
+        } else {                                                                                              \
+            hipLaunchKernelGGL((k_stream_collide<D, T, C, SHIFT, COLL, V>), dim3(grid), dim3(256), 0, ctx->stream, \
+                               v, fa, fb, aux_f, sp);                                                         \
+        }                                                                                                     \
+        break;
+    switch (var) {
+        ZLH_SC(0) ZLH_SC(1) ZLH_SC(2) ZLH_SC(3) ZLH_SC(4) ZLH_SC(5) ZLH_SC(6) ZLH_SC(7)
+        default: break;
+    }
+#undef ZLH_SC
+}
+
+template <int D, typename T, typename C, bool SHIFT>
+void launch_stream_collide(zlh_ctx *ctx, int coll, int var, unsigned grid, const BrickLevel<T> &v, T *fa, T *fb,
+                           T *aux_f, const StepParams &sp) {
+    if (coll == ZLH_SRT) launch_stream_collide_v<D, T, C, SHIFT, 0>(ctx, var, grid, v, fa, fb, aux_f, sp);
+    else launch_stream_collide_v<D, T, C, SHIFT, 1>(ctx, var, grid, v, fa, fb, aux_f, sp);
+}
+
 template <int D, typename T, typename C, bool SHIFT>
 zlh_status launch_step_kernels(zlh_ctx *ctx, int level, int coll) {
     LevelState &L = ctx->lv[level];
@@ -494,12 +539,8 @@ zlh_status launch_step_kernels(zlh_ctx *ctx, int level, int coll) {
         ZLH_TRY(hipEventCreate(&e1));
         ZLH_TRY(hipEventRecord(e0, ctx->stream));
     }
-    if (coll == ZLH_SRT)
-        hipLaunchKernelGGL((k_stream_collide<D, T, C, SHIFT, 0>), dim3(grid), dim3(256), 0, ctx->stream, v, fa, fb,
-                           aux_f, L.sp);
-    else
-        hipLaunchKernelGGL((k_stream_collide<D, T, C, SHIFT, 1>), dim3(grid), dim3(256), 0, ctx->stream, v, fa, fb,
-                           aux_f, L.sp);
+    launch_stream_collide<D, T, C, SHIFT>(ctx, coll, 2 * les_code(L.sp.les, L.sp.les_tau_mode) + (L.sp.has_force ? 1 : 0),
+                                          grid, v, fa, fb, aux_f, L.sp);
     if (ctx->timing) {
         ZLH_TRY(hipEventRecord(e1, ctx->stream));
         ctx->events.emplace_back(e0, e1);
@@ -784,6 +825,10 @@ zlh_status zlh_create(const zlh_solver_desc *desc, zlh_ctx **out) {
     ctx->q = desc->dims == 2 ? 9 : 19;
     ctx->lv.resize(desc->max_level + 1);
     if (hipStreamCreate(&ctx->stream) != hipSuccess) return ZLH_EDEVICE;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ctx->n_cu < 1)
+        return ZLH_EDEVICE;
     if (dalloc(ctx.get(), ctx->mlp_d, sizeof(MlpDev))) return ZLH_ENOMEM;
     *out = ctx.release();
     return ZLH_OK;
@@ -1121,6 +1166,20 @@ zlh_status zlh_mlp_set(zlh_ctx *ctx, const float *params, const float *mean, con
         m.fmax[i] = fmax[i];
     }
     m.nu_scale = (float)nu_scale;
+    // operand images of the wavefront MFMA network (layout in mlp.hpp)
+    const float *W1 = m.p, *b1 = W1 + kMlpH * kMlpF, *W2 = b1 + kMlpH, *b2 = W2 + kMlpH * kMlpH, *w3 = b2 + kMlpH;
+    MlpOps &o = m.ops;
+    for (int l = 0; l < 64; ++l) {
+        const int i = l % 32, h = l / 32;
+        for (int e = 0; e < 8; ++e) {
+            const int k = 8 * h + e;
+            o.w1a[l][e] = bf16_bits(k < kMlpF ? W1[i * kMlpF + k] : (k == kMlpF ? b1[i] : 0.f));
+            for (int st = 0; st < 2; ++st) o.w2a[st][l][e] = bf16_bits(W2[i * kMlpH + mfma_row(h, 8 * st + e)]);
+            o.w2a[2][l][e] = bf16_bits(k == 0 ? b2[i] : 0.f);
+        }
+    }
+    for (int h = 0; h < 2; ++h)
+        for (int r = 0; r < 16; ++r) o.w3r[h][r] = w3[mfma_row(h, r)];
     ZLH_TRY(hipMemcpyAsync(ctx->mlp_d.p, &m, sizeof(m), hipMemcpyHostToDevice, ctx->stream));
     ZLH_TRY(hipStreamSynchronize(ctx->stream));
     ctx->mlp_set = true;
@@ -1186,17 +1245,22 @@ zlh_status zlh_mlp_train(zlh_ctx *ctx, const float *x, const float *t, int64_t n
     return ZLH_OK;
 }
 
-zlh_status zlh_mlp_eval(zlh_ctx *ctx, const float *x, int64_t n, float *y) {
+zlh_status zlh_mlp_eval(zlh_ctx *ctx, const float *x, int64_t n, int form, float *y) {
     if (!ctx) return ZLH_EINVAL;
     if (!ctx->mlp_set) return ctx->fail(ZLH_ESTATE, "learned closure not set");
-    if (!x || !y || n < 1) return ctx->fail(ZLH_EINVAL, "bad evaluation arguments");
+    if (!x || !y || n < 1 || (form != ZLH_MLP_FP32 && form != ZLH_MLP_BF16_MFMA))
+        return ctx->fail(ZLH_EINVAL, "bad evaluation arguments");
     DevBuf dx, dy;
     zlh_status s;
     if ((s = dalloc(ctx, dx, sizeof(float) * kMlpF * n, false))) return s;
     if ((s = dalloc(ctx, dy, sizeof(float) * n, false))) return s;
     ZLH_TRY(hipMemcpyAsync(dx.p, x, sizeof(float) * kMlpF * n, hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_mlp_eval, dim3(blocks(n)), dim3(256), 0, ctx->stream, ctx->mlp_d.as<MlpDev>(), dx.as<float>(),
-                       n, dy.as<float>());
+    if (form == ZLH_MLP_BF16_MFMA)
+        hipLaunchKernelGGL(k_mlp_eval_wave, dim3(blocks(n)), dim3(256), 0, ctx->stream, ctx->mlp_d.as<MlpDev>(),
+                           dx.as<float>(), n, dy.as<float>());
+    else
+        hipLaunchKernelGGL(k_mlp_eval, dim3(blocks(n)), dim3(256), 0, ctx->stream, ctx->mlp_d.as<MlpDev>(),
+                           dx.as<float>(), n, dy.as<float>());
     ZLH_TRY(hipGetLastError());
     ZLH_TRY(hipMemcpyAsync(y, dy.p, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
     ZLH_TRY(hipStreamSynchronize(ctx->stream));

@@ -21,6 +21,7 @@ FP64, FP32_SHIFTED = 0, 1
 SRT, MRT = 0, 1
 LES_NONE, LES_SMAGORINSKY, LES_MLP = 0, 1, 2
 LES_TAU_REFERENCE, LES_TAU_PHYSICAL = 0, 1
+MLP_FP32, MLP_BF16_MFMA = 0, 1
 
 
 class SolverDesc(C.Structure):
@@ -82,8 +83,8 @@ def lib():
         F32 = D(np.float32)
         L.zlh_mlp_set.argtypes = [P, F32, F32, F32, F32, F32, C.c_double]
         L.zlh_mlp_gradient.argtypes = [P, F32, F32, F32, C.c_int64, F32, F32]
-        L.zlh_mlp_train.argtypes = [P, F32, F32, F32, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_uint64, F32, F32]
-        L.zlh_mlp_eval.argtypes = [P, F32, C.c_int64, F32]
+        L.zlh_mlp_train.argtypes = [P, F32, F32, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_uint64, F32, F32]
+        L.zlh_mlp_eval.argtypes = [P, F32, C.c_int64, C.c_int, F32]
         L.zlh_closure_features.argtypes = [P, C.c_int, F32, F32, D(np.int32)]
         _lib = L
     return _lib
@@ -244,10 +245,11 @@ class Solver:
         self._chk(lib().zlh_mlp_train(self.h, f(x), f(t), len(t), steps, batch, lr, seed, p, hist))
         return p, hist
 
-    def mlp_eval(self, x):
+    def mlp_eval(self, x, form=None):
+        """Network output of normalised inputs; form MLP_FP32 (default) or MLP_BF16_MFMA."""
         x = np.ascontiguousarray(x, np.float32)
         y = np.zeros(len(x), np.float32)
-        self._chk(lib().zlh_mlp_eval(self.h, x, len(x), y))
+        self._chk(lib().zlh_mlp_eval(self.h, x, len(x), MLP_FP32 if form is None else form, y))
         return y
 
     def closure_features(self, level=0):
