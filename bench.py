@@ -1,16 +1,18 @@
 """Benchmark of the MI355X lattice Boltzmann time step (one JSON line on stdout).
 
-Workloads (synthetic data, sizes from the headline configurations):
-  tgv256  D3Q19 MRT + Smagorinsky (Cs 0.1), 256^3-cell periodic Taylor-Green vortex (u0 0.05)
-          with seeded Gaussian noise 1e-3, FP32 shifted populations.
-  amr4    D3Q19 MRT + Smagorinsky on a 4-level box-refined periodic grid (32^3 background,
-          finest level 256^3-equivalent), FP32 shifted populations.
+Workloads (synthetic data, zllbm_amd.workloads):
+  amr4    (default) D3Q19 MRT-LES decaying turbulence on 4 levels over a periodic 32^3 box
+          (256^3-equivalent), refined where |vorticity| exceeds its 80th percentile; FP32 shifted
+          populations with the learned closure (BF16 matrix-core network, Smagorinsky fallback).
+          Compared in the same run with the FP64 Smagorinsky form on the same grid (MLUPS), and
+          with the FP64 form of the same closure (velocity and density L2).
+  tgv256  D3Q19 MRT + Smagorinsky (Cs 0.1), 256^3-cell periodic Taylor-Green vortex, FP32.
 
-value = whole-job MLUPS (lattice-node updates per second / 1e6, every node of every level
-counted once per update it receives), with populations resident in HBM.
-roofline: algorithmic bytes of the fused stream-collide kernel, 2 * Q * 4 bytes per node update,
-divided by its average launch time measured with HIP events on the library stream.
-cpu_baseline: the FP64 CPU oracle (same algorithm) on a bounded sample on the host cores.
+value = whole-job MLUPS (lattice-node updates per second / 1e6; a level-l node counts 2^l
+updates per background step), populations resident in HBM.
+roofline: algorithmic bytes of the fused stream-collide kernel on the finest level, 2 * Q * 4
+bytes per node update, over its average launch time from HIP events on the library stream.
+cpu_baseline: the FP64 CPU oracle (same algorithm, OpenMP over nodes) on a bounded sample.
 """
 import argparse
 import json
@@ -24,39 +26,60 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "MLUPS + % HBM roofline, D3Q19 MRT-LES AMR; FP32+learned vs FP64 plain"
+DEFAULT_CLOSURE = os.path.join(ROOT, "zllbm_amd", "data", "closure_turbulence.npz")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_workload(name, hip, g):
-    from tests import scenarios as sc
-    if name == "tgv256":
-        n = 256
-        grid = g.uniform_grid(3, (n, n, n))
-        nu = 0.05 * (n / (2 * np.pi)) / 1600.0     # Re = u0 / (k nu) = 1600
-        kw = dict(nu0=nu, bc=sc.all_bc(3, sc.PER), collision=(hip.MRT,), les=hip.LES_SMAGORINSKY,
-                  les_tau_mode=hip.LES_TAU_PHYSICAL, cs=0.1, precision=hip.FP32_SHIFTED)
-        init = sc.tgv_fields(grid, 0.05, noise=1e-3)
-        desc = "D3Q19 MRT-LES (Cs 0.1) TGV 256^3 periodic, u0 0.05 + 1e-3 noise, FP32 shifted"
-        return grid, kw, [init], desc
-    raise SystemExit("unknown workload " + name)
+def make_solver(hip, W, precision, les, model=None):
+    kw = dict(W.solver)
+    kw.update(les=les)
+    s = hip.Solver(W.grid, precision=precision, **kw).build()
+    if model is not None:
+        s.mlp_set(model)
+    for lvl, (rho, u) in enumerate(W.init):
+        s.init_macro(lvl, rho, u)
+    return s
 
 
-def cpu_baseline(seconds=15.0):
+def timed(s, W, warmup, steps):
+    s.time_marching(warmup)
+    s.synchronize()
+    s.timing_enable(True)
+    t0 = time.perf_counter()
+    s.time_marching(steps)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    finest = W.grid.max_level
+    k_ms, k_n = s.timing_read(finest)
+    s.timing_enable(False)
+    return wall, k_ms, k_n
+
+
+def state(s, W):
+    rho, u = [], []
+    for l in range(W.grid.max_level + 1):
+        r, v = s.macros(l)
+        rho.append(r)
+        u.append(v)
+    return np.concatenate(rho), np.concatenate(u)
+
+
+def cpu_baseline(model, seconds=15.0):
+    """FP64 oracle on a bounded sample of the same workload: 2 levels over a 24^3 box."""
     from oracle.zlbm_oracle import Oracle, lib
-    from zllbm_amd import grid as g
-    from tests import scenarios as sc
+    from zllbm_amd import workloads
     threads = os.cpu_count() or 1
     lib().zo_set_threads(threads)
-    n = 32
-    grid = g.uniform_grid(3, (n, n, n))
-    kw = dict(nu0=0.05 * (256 / (2 * np.pi)) / 1600.0, bc=sc.all_bc(3, sc.PER), collision=(1,), les=1,
-              les_tau_mode=1, cs=0.1)
-    o = Oracle(grid, **kw)
-    rho, u = sc.tgv_fields(grid, 0.05, noise=1e-3)
-    o.init_macro(0, rho, u)
+    W = workloads.decaying_turbulence(24, 2, buffer=2, les=2 if model is not None else 1)
+    o = Oracle(W.grid, **W.solver)
+    if model is not None:
+        o.set_mlp(model.params.astype(np.float64), model.mean, model.std, model.fmin, model.fmax, model.nu_scale)
+    for l, (rho, u) in enumerate(W.init):
+        o.init_macro(l, rho, u)
     o.time_marching(1)
     steps = 0
     t0 = time.perf_counter()
@@ -64,9 +87,11 @@ def cpu_baseline(seconds=15.0):
         o.time_marching(1)
         steps += 1
     dt = time.perf_counter() - t0
-    mlups = grid.levels[0].n * steps / dt / 1e6
-    return {"value": round(mlups, 4), "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": "FP64 oracle, D3Q19 MRT-LES TGV 32^3 periodic (35,937 nodes), %d steps in %.1f s" % (steps, dt)}
+    return {"value": round(W.updates_per_step * steps / dt / 1e6, 4), "unit": "MLUPS", "cores": threads,
+            "kind": "port",
+            "sample": "FP64 oracle, D3Q19 MRT %s, decaying turbulence on 2 levels over a 24^3 box (%d updates "
+                      "per step), %d steps in %.1f s" % ("+ learned closure" if model is not None else "+ Smagorinsky",
+                                                         W.updates_per_step, steps, dt)}
 
 
 def main():
@@ -74,7 +99,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="tgv256")
+    ap.add_argument("--workload", default="amr4", choices=["amr4", "tgv256"])
+    ap.add_argument("--closure", default=DEFAULT_CLOSURE, help="trained closure (zllbm_amd.train_closure)")
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP64 runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (for counter calibration)")
@@ -92,49 +119,67 @@ def main():
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
-    from zllbm_amd import grid as g, hip
-    grid, kw, inits, desc = build_workload(args.workload, hip, g)
-    log("building", desc)
-    s = hip.Solver(grid, **kw).build()
-    for lvl, (rho, u) in enumerate(inits):
-        s.init_macro(lvl, rho, u)
-    nodes = [lev.n for lev in grid.levels]
-    updates_per_step = sum(n << l for l, n in enumerate(nodes))
-    log("nodes per level", nodes, "aux", [s.info(l)[1] for l in range(len(nodes))])
-    s.time_marching(args.warmup)
-    s.synchronize()
-    s.timing_enable(True)
-    t0 = time.perf_counter()
-    s.time_marching(args.steps)
-    s.synchronize()
-    t1 = time.perf_counter()
-    k_ms, k_n = s.timing_read()
-    s.timing_enable(False)
-    wall = t1 - t0
-    mlups = updates_per_step * args.steps / wall / 1e6
-    q = 19 if grid.dims == 3 else 9
-    bytes_per_update = 2 * q * (4 if kw.get("precision", 0) == hip.FP32_SHIFTED else 8)
-    avg_launch_s = k_ms / 1e3 / max(k_n, 1)
-    # one launch covers one level substep; the finest-level launch dominates for AMR
-    launch_nodes = updates_per_step * args.steps / max(k_n, 1)
-    achieved = launch_nodes * bytes_per_update / avg_launch_s / 1e9
-    rho, u = s.macros(0)
-    finite = bool(np.all(np.isfinite(rho)))
-    log("wall %.3f s, kernel %.3f ms x %d, finite=%s" % (wall, k_ms, k_n, finite))
-    out = {
-        "metric": "MLUPS + % HBM roofline, D3Q19 MRT-LES AMR; FP32+learned vs FP64 plain",
-        "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32" if kw.get("precision") == hip.FP32_SHIFTED else "fp64",
-        "data": "synthetic",
-        "config": {"workload": desc, "nodes_per_level": nodes, "updates_per_step": updates_per_step},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_stream_collide", "avg_launch_ms": round(avg_launch_s * 1e3, 4),
-                     "bytes_per_update": bytes_per_update},
-    }
+    from zllbm_amd import closure, hip, workloads
+    out = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic"}
+    model = None
+    if args.workload == "amr4":
+        W = workloads.decaying_turbulence(32, 4)
+        model = closure.ClosureModel.load(args.closure)
+        les = hip.LES_MLP
+    else:
+        W = workloads.tgv(256)
+        les = hip.LES_SMAGORINSKY
+    log("workload:", W.name, "nodes per level", W.nodes)
+    s = make_solver(hip, W, hip.FP32_SHIFTED, les, model)
+    log("aux nodes per level", [s.info(l)[1] for l in range(len(W.nodes))])
+    wall, k_ms, k_n = timed(s, W, args.warmup, args.steps)
+    mlups = W.updates_per_step * args.steps / wall / 1e6
+    q = 19
+    bpu = 2 * q * 4
+    finest = W.grid.max_level
+    avg_s = k_ms / 1e3 / max(k_n, 1)
+    achieved = W.nodes[finest] * bpu / avg_s / 1e9
+    log("FP32 %s: %.3f ms per step, finest-level kernel %.4f ms x %d" % (
+        "learned closure" if model is not None else "Smagorinsky", wall / args.steps * 1e3, avg_s * 1e3, k_n))
+    kernel = ("k_stream_collide_mlp" if model is not None else "k_stream_collide") + " (level %d)" % finest
+    out.update({"value": round(mlups, 2), "unit": "MLUPS", "ms_per_step": round(wall / args.steps * 1e3, 4),
+                "config": {"workload": W.name, "nodes_per_level": W.nodes, "updates_per_step": W.updates_per_step,
+                           "collision": "MRT", "closure": "learned (BF16 MFMA), Smagorinsky fallback"
+                           if model is not None else "Smagorinsky Cs 0.1", "populations": "FP32 shifted"},
+                "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+                             "avg_launch_ms": round(avg_s * 1e3, 4), "bytes_per_update": bpu,
+                             "updates_per_launch": W.nodes[finest]}})
+    rho32, u32 = state(s, W)
+    if model is not None:
+        fb = []
+        for l in range(len(W.nodes)):
+            fb.append(round(float(np.mean(s.closure_features(l)[2])), 4))
+        out["config"]["closure_fallback_fraction"] = fb
+    del s
+    if not args.no_compare and model is not None:
+        c_steps = max(20, args.steps // 2)
+        s64 = make_solver(hip, W, hip.FP64, hip.LES_SMAGORINSKY)
+        wall64, _, _ = timed(s64, W, 5, c_steps)
+        del s64
+        mlups64 = W.updates_per_step * c_steps / wall64 / 1e6
+        log("FP64 Smagorinsky: %.3f ms per step" % (wall64 / c_steps * 1e3))
+        s64 = make_solver(hip, W, hip.FP64, hip.LES_MLP, model)
+        s64.time_marching(args.warmup + args.steps)
+        rho64, u64 = state(s64, W)
+        del s64
+        l2u = float(np.sqrt(np.sum((u32 - u64) ** 2) / np.sum(u64 ** 2)))
+        l2r = float(np.sqrt(np.mean((rho32 - rho64) ** 2)))
+        log("L2 after %d steps: u %.3e, rho %.3e" % (args.warmup + args.steps, l2u, l2r))
+        out["comparison"] = {"fp64_smagorinsky_mlups": round(mlups64, 2),
+                             "fp64_smagorinsky_ms_per_step": round(wall64 / c_steps * 1e3, 4),
+                             "speedup_fp32_learned_vs_fp64_smagorinsky": round(mlups / mlups64, 3),
+                             "l2_u_vs_fp64": l2u, "l2_rho_vs_fp64": l2r,
+                             "l2_reference": "FP64 form of the same closure on the GPU (pinned to the CPU "
+                                             "oracle by the parity tests), %d steps" % (args.warmup + args.steps)}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(model)
     print(json.dumps(out), flush=True)
 
 

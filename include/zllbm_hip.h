@@ -126,9 +126,10 @@ zlh_status zlh_conservation(zlh_ctx *ctx, int level, double *out4);
 zlh_status zlh_level_info(zlh_ctx *ctx, int level, int64_t *n_nodes, int64_t *n_aux, int64_t *n_bricks);
 zlh_status zlh_synchronize(zlh_ctx *ctx);
 /* device time of the fused stream-collide kernel: enable recording, then read the total in
- * milliseconds and the number of launches since enabling (synchronising) */
+ * milliseconds and the number of launches since enabling, of one level or of all (level -1)
+ * (synchronising; no reference counterpart) */
 zlh_status zlh_timing_enable(zlh_ctx *ctx, int enable);
-zlh_status zlh_timing_read(zlh_ctx *ctx, double *total_ms, int64_t *launches);
+zlh_status zlh_timing_read(zlh_ctx *ctx, int level, double *total_ms, int64_t *launches);
 
 /* ---- learned sub-grid closure (no counterpart in the reference; replaces the Smagorinsky
  * relaxation time of les_models/lbm_les_smagorinsky.cpp:22-94 when desc.les = ZLH_LES_MLP) ----

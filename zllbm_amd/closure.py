@@ -70,31 +70,12 @@ def features_from_stress(pi6, rho, u, tau0):
 
 
 def random_fourier_velocity(grid, level=0, nmodes=64, kmin=1, kmax=8, urms=0.05 / np.sqrt(3.0), seed=20261202):
-    """Solenoidal random-mode field: nmodes integer wave vectors with kmin <= |k| <= kmax (in units of
-    2 pi / L), amplitudes from E(k) ~ k^(-5/3), uniform phases, scaled to the given rms speed."""
-    rng = np.random.default_rng(seed)
-    ks = []
-    while len(ks) < nmodes:
-        k = rng.integers(-kmax, kmax + 1, size=3)
-        r = np.linalg.norm(k)
-        if kmin <= r <= kmax:
-            ks.append(k.astype(np.float64))
-    ks = np.asarray(ks)
-    kn = np.linalg.norm(ks, axis=1)
-    amp = kn ** (-11.0 / 6.0)
-    dirs = rng.normal(size=(nmodes, 3))
-    dirs -= (np.sum(dirs * ks, 1) / kn ** 2)[:, None] * ks        # divergence free
-    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
-    phase = rng.uniform(0, 2 * np.pi, nmodes)
+    """Random-Fourier-mode velocity (zllbm_amd.fields) at the nodes of one level of a periodic
+    grid, rms speed per component `urms` over the box."""
+    from zllbm_amd.fields import fourier_modes
+    modes = fourier_modes(grid.cells, nmodes, kmin, kmax, urms, seed)
     xyz = grid.coords(level).astype(np.float64) / (1 << level) - 1.0
-    L = np.asarray(grid.cells, dtype=np.float64)
-    u = np.zeros((len(xyz), 3))
-    for m in range(nmodes):
-        arg = 2 * np.pi * (xyz @ (ks[m] / L)) + phase[m]
-        u += np.outer(amp[m] * np.cos(arg), dirs[m])
-    uniq = np.all(xyz < L, axis=1)          # rms over the unique nodes of the periodic box
-    u *= urms / np.sqrt(np.mean(np.sum(u[uniq] ** 2, 1)) / 3.0)
-    return u
+    return modes.velocity(xyz)
 
 
 def unique_field(grid, values, level=0):
@@ -128,19 +109,18 @@ def strain(u):
 
 def filtered_samples(u_fine, ratio, tau0_coarse):
     """A-priori samples from one fine velocity snapshot (n, n, n, 3): raw inputs [m, 9] and target
-    nu_t [m] in coarse lattice units, plus the coarse strain for the Smagorinsky comparison."""
+    nu_t [m] in coarse lattice units, plus the coarse stress for the Smagorinsky comparison."""
+    pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
     ub = box_filter(u_fine, ratio)
-    uu = box_filter(np.einsum("...a,...b->...ab", u_fine, u_fine), ratio)
-    tau = uu - np.einsum("...a,...b->...ab", ub, ub)
-    tr = np.trace(tau, axis1=-2, axis2=-1)
-    tau_d = tau - tr[..., None, None] * np.eye(3) / 3.0
     S6 = strain(ub)
-    S = np.empty(S6.shape[:-1] + (3, 3))
-    for k, (a, b) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]):
-        S[..., a, b] = S6[..., k]
-        S[..., b, a] = S6[..., k]
-    SS = np.einsum("...ab,...ab->...", S, S)
-    nu_t = np.maximum(0.0, -np.einsum("...ab,...ab->...", tau_d, S) / np.maximum(2.0 * SS, 1e-30))
+    tau6 = np.stack([box_filter(u_fine[..., a] * u_fine[..., b], ratio) - ub[..., a] * ub[..., b]
+                     for a, b in pairs], -1)
+    tr = tau6[..., 0] + tau6[..., 1] + tau6[..., 2]
+    for a in range(3):
+        tau6[..., a] -= tr / 3.0
+    w = np.array([1.0, 1.0, 1.0, 2.0, 2.0, 2.0])          # off-diagonal entries count twice
+    SS = np.sum(w * S6 * S6, -1)
+    nu_t = np.maximum(0.0, -np.sum(w * tau6 * S6, -1) / np.maximum(2.0 * SS, 1e-30))
     rho = np.ones(SS.shape)
     pi = -(2.0 / 3.0) * rho[..., None] * (tau0_coarse + 3.0 * nu_t)[..., None] * S6
     # the collision evaluates the inputs in FP32; keep them so, so that the training range

@@ -371,9 +371,10 @@ struct zlh_ctx {
     bool built = false, initialised = false;
     std::string err;
     bool timing = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
-    double timing_ms = 0.;
-    int64_t timing_launches = 0;
+    struct TimedLaunch { hipEvent_t start, stop; int level; };
+    std::vector<TimedLaunch> events;
+    double timing_ms[16] = {};       // per level
+    int64_t timing_launches[16] = {};
     DevBuf lattice_d;
     DevBuf mlp_d;           // MlpDev of the learned closure
     bool mlp_set = false;
@@ -543,7 +544,7 @@ zlh_status launch_step_kernels(zlh_ctx *ctx, int level, int coll) {
                                           grid, v, fa, fb, aux_f, L.sp);
     if (ctx->timing) {
         ZLH_TRY(hipEventRecord(e1, ctx->stream));
-        ctx->events.emplace_back(e0, e1);
+        ctx->events.push_back({e0, e1, level});
     }
     ZLH_TRY(hipGetLastError());
     // domain boundaries in the reference order: every minimum face, then every maximum face
@@ -837,7 +838,7 @@ zlh_status zlh_create(const zlh_solver_desc *desc, zlh_ctx **out) {
 void zlh_destroy(zlh_ctx *ctx) {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (auto &e : ctx->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    for (auto &e : ctx->events) { (void)hipEventDestroy(e.start); (void)hipEventDestroy(e.stop); }
     ctx->lv.clear();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1127,28 +1128,32 @@ zlh_status zlh_synchronize(zlh_ctx *ctx) {
 zlh_status zlh_timing_enable(zlh_ctx *ctx, int enable) {
     if (!ctx) return ZLH_EINVAL;
     ZLH_TRY(hipStreamSynchronize(ctx->stream));
-    for (auto &e : ctx->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    for (auto &e : ctx->events) { (void)hipEventDestroy(e.start); (void)hipEventDestroy(e.stop); }
     ctx->events.clear();
-    ctx->timing_ms = 0.;
-    ctx->timing_launches = 0;
+    for (int l = 0; l < 16; ++l) { ctx->timing_ms[l] = 0.; ctx->timing_launches[l] = 0; }
     ctx->timing = enable != 0;
     return ZLH_OK;
 }
 
-zlh_status zlh_timing_read(zlh_ctx *ctx, double *total_ms, int64_t *launches) {
+zlh_status zlh_timing_read(zlh_ctx *ctx, int level, double *total_ms, int64_t *launches) {
     if (!ctx) return ZLH_EINVAL;
+    if (level < -1 || level > ctx->d.max_level) return ctx->fail(ZLH_EINVAL, "bad level %d", level);
     ZLH_TRY(hipStreamSynchronize(ctx->stream));
     for (auto &e : ctx->events) {
         float ms = 0.f;
-        ZLH_TRY(hipEventElapsedTime(&ms, e.first, e.second));
-        ctx->timing_ms += ms;
-        ++ctx->timing_launches;
-        (void)hipEventDestroy(e.first);
-        (void)hipEventDestroy(e.second);
+        ZLH_TRY(hipEventElapsedTime(&ms, e.start, e.stop));
+        ctx->timing_ms[e.level] += ms;
+        ++ctx->timing_launches[e.level];
+        (void)hipEventDestroy(e.start);
+        (void)hipEventDestroy(e.stop);
     }
     ctx->events.clear();
-    if (total_ms) *total_ms = ctx->timing_ms;
-    if (launches) *launches = ctx->timing_launches;
+    double ms = 0.;
+    int64_t n = 0;
+    for (int l = 0; l <= ctx->d.max_level; ++l)
+        if (level < 0 || l == level) { ms += ctx->timing_ms[l]; n += ctx->timing_launches[l]; }
+    if (total_ms) *total_ms = ms;
+    if (launches) *launches = n;
     return ZLH_OK;
 }
 

@@ -216,6 +216,103 @@ def refined_box_grid(dims: int, cells: Sequence[int],
     return Grid(dims, cells, levels)
 
 
+def _node_closure(cells: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+    """Nodes of a cell set (array shape + 1 per axis): the closure (some incident cell in the set)
+    and the interior (every incident cell in the set)."""
+    import itertools
+    pad = np.pad(cells, 1)
+    shape = tuple(n + 1 for n in cells.shape)
+    anyc = np.zeros(shape, dtype=bool)
+    allc = np.ones(shape, dtype=bool)
+    for off in itertools.product((0, 1), repeat=cells.ndim):
+        sl = pad[tuple(slice(o, o + n) for o, n in zip(off, shape))]
+        anyc |= sl
+        allc &= sl
+    return anyc, allc
+
+
+def _rings(closure: np.ndarray, interior: np.ndarray, count: int) -> np.ndarray:
+    """Chebyshev ring index of the closure nodes from its boundary (0 on the boundary), for the
+    first `count` rings; 255 beyond and outside."""
+    from scipy import ndimage
+    ring = np.full(closure.shape, 255, dtype=np.uint8)
+    front = closure & ~interior
+    reached = front.copy()
+    ring[front] = 0
+    box = np.ones((3,) * closure.ndim, dtype=bool)
+    for r in range(1, count):
+        front = ndimage.binary_dilation(front, structure=box) & closure & ~reached
+        ring[front] = r
+        reached |= front
+    return ring
+
+
+def refined_mask_grid(dims: int, cells: Sequence[int], levels: int, refine, margin: int = 4,
+                      buffer: int = 2) -> Grid:
+    """Refinement by a cell criterion (the general form of refined_box_grid).
+
+    ``refine(k, centres)`` returns, for the cell centres of level k (background units, origin at
+    the domain minimum, array [n, dims]), which of them ask for level k+1.  A flagged cell is
+    refined if it lies at least ``margin`` level-k cells inside level k's region and inside the
+    domain faces; the flags grow by ``buffer`` cells (the reference's extension layers,
+    grid/grid_manager.h:97) within that allowance.  Interface layers and flags follow
+    refined_box_grid with the ring index (Chebyshev distance to the region boundary) in place of
+    the box distance.
+    """
+    from scipy import ndimage
+    cells = tuple(int(c) for c in cells)
+    box = np.ones((3,) * dims, dtype=bool)
+    regions = [np.ones(cells, dtype=bool)]            # level-k cells of level k's region
+    flagged: List[np.ndarray] = []
+    for k in range(levels - 1):
+        region = regions[k]
+        allowed = ndimage.binary_erosion(region, structure=box, iterations=margin, border_value=0)
+        axes = [(np.arange(n, dtype=np.float64) + 0.5) / (1 << k) for n in region.shape]
+        centres = np.stack([a.ravel() for a in np.meshgrid(*axes, indexing="ij")], 1)
+        flag = np.asarray(refine(k, centres), dtype=bool).reshape(region.shape) & allowed
+        if buffer:
+            flag = ndimage.binary_dilation(flag, structure=box, iterations=buffer) & allowed
+        if not flag.any():
+            raise ValueError("level %d: no cell is refined" % (k + 1))
+        flagged.append(flag)
+        fine = flag
+        for a in range(dims):
+            fine = np.repeat(fine, 2, axis=a)
+        regions.append(fine)
+    out: List[Level] = []
+    for k in range(levels):
+        lo = 1 << k
+        closure, interior = _node_closure(regions[k])
+        keep = closure.copy()
+        fl = np.zeros(closure.shape, dtype=np.int32)
+        lay_masks: Dict[Tuple[int, int], np.ndarray] = {}
+        if k < levels - 1:
+            c_clo, c_int = _node_closure(flagged[k])
+            ring = _rings(c_clo, c_int, 2)
+            keep &= ~c_clo | (ring <= 1)
+            lay_masks[(OUTER_C2F, 0)] = ring == 0
+            lay_masks[(OUTER_C2F, 1)] = ring == 1
+            # layer 1 is the outmost coarse-to-fine layer and the only ghost one
+            # (InstantiateOverlapLayerOfRefinementInterface, grid/grid_manager.cpp:413-568)
+            fl[ring == 1] |= FLAG_C2F_GHOST | FLAG_C2F0
+        if k > 0:
+            ring = _rings(closure, interior, 3)
+            for layer, r in ((2, 0), (1, 1), (0, 2)):
+                lay_masks[(OUTER_F2C, layer)] = ring == r
+            fl[ring == 0] |= FLAG_F2C0 | FLAG_F2C_GHOST
+            fl[ring == 1] |= FLAG_F2C_GHOST
+        idx = np.nonzero(keep)
+        keys = morton_encode(dims, *[(i + lo).astype(np.uint64) for i in idx])
+        flags = fl[idx]
+        order = np.argsort(keys)
+        lev = Level(keys[order], flags[order])
+        for kk, m in lay_masks.items():
+            li = np.nonzero(m & keep)
+            lev.layers[kk] = np.sort(morton_encode(dims, *[(i + lo).astype(np.uint64) for i in li]))
+        out.append(lev)
+    return Grid(dims, cells, out)
+
+
 def boundary_masks(grid: Grid, level: int) -> Dict[int, np.ndarray]:
     """Nodes on each domain face at ``level`` (CheckIfNodeOutsideCubicDomain,
     grid/grid_info.cpp:84-142)."""

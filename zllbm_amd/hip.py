@@ -79,7 +79,7 @@ def lib():
         L.zlh_level_info.argtypes = [P, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.zlh_synchronize.argtypes = [P]
         L.zlh_timing_enable.argtypes = [P, C.c_int]
-        L.zlh_timing_read.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.zlh_timing_read.argtypes = [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         F32 = D(np.float32)
         L.zlh_mlp_set.argtypes = [P, F32, F32, F32, F32, F32, C.c_double]
         L.zlh_mlp_gradient.argtypes = [P, F32, F32, F32, C.c_int64, F32, F32]
@@ -266,7 +266,8 @@ class Solver:
     def timing_enable(self, on=True):
         self._chk(lib().zlh_timing_enable(self.h, 1 if on else 0))
 
-    def timing_read(self):
+    def timing_read(self, level=-1):
+        """(total ms, launches) of the fused kernel since timing_enable, one level or all (-1)."""
         ms, n = C.c_double(), C.c_int64()
-        self._chk(lib().zlh_timing_read(self.h, C.byref(ms), C.byref(n)))
+        self._chk(lib().zlh_timing_read(self.h, level, C.byref(ms), C.byref(n)))
         return ms.value, n.value
