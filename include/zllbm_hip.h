@@ -110,6 +110,9 @@ zlh_status zlh_transfer_levels(zlh_ctx *ctx, int fine_level);
 /* background steps in the coarse-to-fine multi time stepping order (AmrManager::TimeMarching,
  * amr_manager.cpp:279-300; MultiTimeSteppingC2F, grid/grid_manager.cpp:997-1024) */
 zlh_status zlh_time_marching(zlh_ctx *ctx, int nsteps);
+/* zlh_time_marching replays pairs of background steps as captured HIP graphs (default on);
+ * off launches every kernel directly (no reference counterpart) */
+zlh_status zlh_graphs_enable(zlh_ctx *ctx, int enable);
 /* post-stream populations GridNodeLbm::f_ in key order [n][q] (synchronising) */
 zlh_status zlh_download_populations(zlh_ctx *ctx, int level, double *f);
 /* density and velocity of the post-stream populations, key order; u is [n][dims] (synchronising) */

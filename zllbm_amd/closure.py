@@ -107,9 +107,10 @@ def strain(u):
     return np.stack([S[..., 0, 0], S[..., 1, 1], S[..., 2, 2], S[..., 0, 1], S[..., 0, 2], S[..., 1, 2]], -1)
 
 
-def filtered_samples(u_fine, ratio, tau0_coarse):
-    """A-priori samples from one fine velocity snapshot (n, n, n, 3): raw inputs [m, 9] and target
-    nu_t [m] in coarse lattice units, plus the coarse stress for the Smagorinsky comparison."""
+def filtered_samples(u_fine, ratio, tau0_coarse, rho_fine=None):
+    """A-priori samples from one fine velocity snapshot (n, n, n, 3) and, if given, its density
+    (n, n, n): raw inputs [m, 9] and target nu_t [m] in coarse lattice units, plus the coarse
+    stress for the Smagorinsky comparison."""
     pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
     ub = box_filter(u_fine, ratio)
     S6 = strain(ub)
@@ -121,7 +122,7 @@ def filtered_samples(u_fine, ratio, tau0_coarse):
     w = np.array([1.0, 1.0, 1.0, 2.0, 2.0, 2.0])          # off-diagonal entries count twice
     SS = np.sum(w * S6 * S6, -1)
     nu_t = np.maximum(0.0, -np.sum(w * tau6 * S6, -1) / np.maximum(2.0 * SS, 1e-30))
-    rho = np.ones(SS.shape)
+    rho = np.ones(SS.shape) if rho_fine is None else box_filter(rho_fine, ratio)
     pi = -(2.0 / 3.0) * rho[..., None] * (tau0_coarse + 3.0 * nu_t)[..., None] * S6
     # the collision evaluates the inputs in FP32; keep them so, so that the training range
     # bounds compare exactly
@@ -141,13 +142,15 @@ def fit_model(solver, x_raw, nu_t, steps=3000, batch=4096, lr=2e-3, seed=2026120
     model with the training range of the raw inputs."""
     mean = x_raw.mean(0)
     std = x_raw.std(0)
-    std[std == 0] = 1.0
+    const = std == 0
+    std[const] = 1.0
     nu_scale = float(np.mean(nu_t)) if np.mean(nu_t) > 0 else 1.0
     xn = ((x_raw - mean) / std).astype(np.float32)
     t = (nu_t / nu_scale).astype(np.float32)
     params, hist = solver.mlp_train(init_params(seed), xn, t, steps, min(batch, len(t)), lr, seed)
     span = x_raw.max(0) - x_raw.min(0)
-    model = ClosureModel(params, mean.astype(np.float32), std.astype(np.float32),
-                         (x_raw.min(0) - margin * span).astype(np.float32),
-                         (x_raw.max(0) + margin * span).astype(np.float32), nu_scale)
+    # a feature without spread in the data does not bound the range
+    fmin = np.where(const, -np.inf, x_raw.min(0) - margin * span).astype(np.float32)
+    fmax = np.where(const, np.inf, x_raw.max(0) + margin * span).astype(np.float32)
+    model = ClosureModel(params, mean.astype(np.float32), std.astype(np.float32), fmin, fmax, nu_scale)
     return model, hist

@@ -131,7 +131,8 @@ __device__ __forceinline__ C relaxation_time(int les, const C *pi, C rho, C drho
     }
     if constexpr (std::is_same<C, float>::value) {
         // BF16 MFMA network across the wavefront (callers keep every lane here)
-        return t0 + mlp_tau_t_wave(*p.mlp, ops ? *ops : p.mlp->ops, pi, rho, drho, u, t0, C(p.cs18));
+        // ops: the operand images, in LDS in the bulk kernel (its address space must stay visible)
+        return t0 + mlp_tau_t_wave(*ops, pi, rho, drho, u, t0, C(p.cs18));
     } else {
         bool fb;
         return t0 + mlp_tau_t<C>(*p.mlp, pi, rho, drho, u, t0, C(p.cs18), &fb);
@@ -404,13 +405,9 @@ __device__ __forceinline__ void stream_collide_brick(const BrickLevel<T> &lv, co
     constexpr int Q = L::Q;
     constexpr int E = B::edge;
     constexpr int NN = B::nnbr;
-    // the FP32 learned closure is a wavefront-wide MFMA product: there every lane stays to the
-    // collision (empty lanes on the rest state) and only the stores are predicated
-    constexpr bool kWave = mlp_on_mfma<C, VAR>();
-    int lane = threadIdx.x & 63;
-    // in the persistent loop the lane is made opaque per brick, which keeps its stencil offsets
-    // from being hoisted out of the loop (they would hold ~50 registers across it)
-    if constexpr (kWave) asm volatile("" : "+v"(lane));
+    static_assert(!mlp_on_mfma<C, VAR>(), "the FP32 learned closure runs in k_stream_collide_mlp");
+    constexpr bool kWave = false;
+    const int lane = threadIdx.x & 63;
     const uint64_t occ = lv.occ[b];
     const bool has_node = (occ >> lane) & 1ull;
     if (!kWave && !has_node) return;
@@ -477,25 +474,135 @@ __global__ __launch_bounds__(256) void k_stream_collide(BrickLevel<T> lv, const 
     stream_collide_brick<D, T, C, SHIFT, COLL, VAR>(lv, fin, fout, aux_f, p, b, nullptr);
 }
 
-// The same with the FP32 learned closure: each workgroup stages the network's operand images in
-// LDS once and its four waves then take four rounds of four adjacent bricks (16 per workgroup),
-// which amortises the staging to a few percent of the population traffic.
+// Read-only brick tables of a level (the kernel takes them as no-alias parameters: it stores into
+// the population arrays inside its loop, and only provably unclobbered tables are read through
+// the scalar cache, whose loads do not wait on the vector memory counter).
+struct BrickTables {
+    const uint64_t *__restrict__ occ;
+    const uint64_t *__restrict__ auxmask;
+    const int32_t *__restrict__ auxbase;
+    const int32_t *__restrict__ nbr;
+    const uint32_t *__restrict__ aux_missing;
+};
+
+// Pull of brick b (every lane, every direction) into a per-wave LDS buffer [Q][64] with one
+// direct-to-LDS dword load per direction (gfx950 global_load_lds_dword: lane i lands at
+// buf + 4 i).  Absent sources of deferred nodes load from the aux array, lanes without a node
+// load their own slot (a valid address).
+template <int D, typename T>
+__device__ __forceinline__ void pull_to_lds(const BrickTables &t, int64_t naux, const T *__restrict__ fin,
+                                            const T *__restrict__ aux_f, int64_t b, int lane, T *buf) {
+    using L = Lattice<D>;
+    using B = Brick<D>;
+    constexpr int Q = L::Q;
+    constexpr int E = B::edge;
+    constexpr int NN = B::nnbr;
+    static_assert(sizeof(T) == 4, "dword staging");
+    const bool has_node = (t.occ[b] >> lane) & 1ull;
+    const uint64_t am = t.auxmask[b];
+    uint32_t missing = 0;
+    int64_t ai = 0;
+    if ((am >> lane) & 1ull) {
+        ai = t.auxbase[b] + __popcll(am & ((1ull << lane) - 1ull));
+        missing = t.aux_missing[ai];
+    }
+    // consume the mask here: its wait then precedes the direct-to-LDS loads instead of landing
+    // between them (where the in-order counter would make each wait for the ones before)
+    asm volatile("" : "+v"(missing));
+    int32_t nb[NN];
+#pragma unroll
+    for (int k = 0; k < NN; ++k) nb[k] = t.nbr[b * NN + k];
+    const int lx = B::lane_coord(lane, 0), ly = B::lane_coord(lane, 1);
+    const int lz = D == 3 ? B::lane_coord(lane, 2) : 0;
+    // addresses by selection, no branches: a branch between two of these loads makes the
+    // compiler wait on the vector memory counter there
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const T *src = fin + (b * Q + q) * 64 + lane;
+        if (q > 0) {
+            const int sx = lx - L::cx[q], sy = ly - L::cy[q], sz = lz - L::cz[q];
+            const bool ex = L::cx[q] != 0 && (sx < 0 || sx >= E);
+            const bool ey = L::cy[q] != 0 && (sy < 0 || sy >= E);
+            const bool ez = D == 3 && L::cz[q] != 0 && (sz < 0 || sz >= E);
+            constexpr int base = D == 3 ? 13 : 4;
+            const int dx = -L::cx[q], dy = -3 * L::cy[q], dz = D == 3 ? -9 * L::cz[q] : 0;
+            const int sb = ez ? (ey ? (ex ? nb[base + dx + dy + dz] : nb[base + dy + dz])
+                                    : (ex ? nb[base + dx + dz] : nb[base + dz]))
+                              : (ey ? (ex ? nb[base + dx + dy] : nb[base + dy])
+                                    : (ex ? nb[base + dx] : nb[base]));
+            const T *pulled = fin + ((int64_t)sb * Q + q) * 64 + B::lane_of(sx & (E - 1), sy & (E - 1), sz & (E - 1));
+            const T *deferred = aux_f + q * naux + ai;
+            src = has_node ? (((missing >> q) & 1u) ? deferred : pulled) : src;
+        }
+        __builtin_amdgcn_global_load_lds((const void *)src, (__attribute__((address_space(3))) void *)(buf + q * 64),
+                                         4, 0, 0);
+    }
+}
+
+// The same with the FP32 learned closure.  Each workgroup stages the network's operand images
+// in LDS once; its four waves then take four rounds of four adjacent bricks (16 per workgroup),
+// and each wave pulls its next brick into LDS while it collides the current one, so the pull's
+// latency hides behind the network's arithmetic at the occupancy its registers allow.  Within
+// a round the order is: read the staged brick, write deferred nodes' post-stream values, issue
+// the next pull, collide, store; the store group is unconditional so that the next round's wait
+// (in-order vector memory counter) covers the pull and not these stores.
 // Register budget: 4 waves per SIMD unforced, 3 forced (whose Guo terms would spill at 4).
 constexpr int kMlpBricksPerGroup = 16;
 template <int D, typename T, typename C, bool SHIFT, int COLL, int VAR>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((VAR & 1) ? 3 : 4)))
-void k_stream_collide_mlp(BrickLevel<T> lv, const T *__restrict__ fin, T *__restrict__ fout, T *__restrict__ aux_f,
-                          StepParams p) {
+void k_stream_collide_mlp(int64_t nbrick, int64_t naux, const uint64_t *__restrict__ occ,
+                          const uint64_t *__restrict__ auxmask, const int32_t *__restrict__ auxbase,
+                          const int32_t *__restrict__ nbr, const uint32_t *__restrict__ aux_missing,
+                          const T *__restrict__ fin, T *__restrict__ fout, T *__restrict__ aux_f, StepParams p) {
     static_assert(mlp_on_mfma<C, VAR>(), "kernel of the FP32 learned closure");
+    const BrickTables t{occ, auxmask, auxbase, nbr, aux_missing};
+    using L = Lattice<D>;
+    constexpr int Q = L::Q;
     __shared__ MlpOps ops;
-    const int64_t g = xcd_group((lv.nbrick + kMlpBricksPerGroup - 1) / kMlpBricksPerGroup);
+    __shared__ T stage[4][Q * 64];
+    const int64_t g = xcd_group((nbrick + kMlpBricksPerGroup - 1) / kMlpBricksPerGroup);
     if (g < 0) return;                       // uniform per workgroup: before the barrier
     stage_mlp_ops(ops, p.mlp->ops);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    T *buf = stage[wave];
+    int64_t b = g * kMlpBricksPerGroup + wave;
+    if (b >= nbrick) return;                 // uniform per wave; no barrier follows
+    pull_to_lds<D, T>(t, naux, fin, aux_f, b, threadIdx.x & 63, buf);
 #pragma unroll 1
     for (int r = 0; r < kMlpBricksPerGroup / 4; ++r) {
-        const int64_t b = g * kMlpBricksPerGroup + r * 4 + wave;
-        if (b < lv.nbrick) stream_collide_brick<D, T, C, SHIFT, COLL, VAR>(lv, fin, fout, aux_f, p, b, &ops);
+        int lane = threadIdx.x & 63;
+        // opaque per brick: keeps the lane's stencil offsets from being hoisted out of the loop
+        asm volatile("" : "+v"(lane));
+        C f[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) f[q] = C(buf[q * 64 + lane]);
+        const bool has_node = (t.occ[b] >> lane) & 1ull;
+        const uint64_t am = t.auxmask[b];
+        if ((am >> lane) & 1ull) {
+            // deferred node: post-stream values to the aux array
+            const int64_t ai = t.auxbase[b] + __popcll(am & ((1ull << lane) - 1ull));
+#pragma unroll
+            for (int q = 0; q < Q; ++q) aux_f[q * naux + ai] = T(f[q]);
+        }
+        // the buffer's reads have returned before it is refilled
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        const int64_t bn = b + 4;
+        const bool more = r + 1 < kMlpBricksPerGroup / 4 && bn < nbrick;
+        if (more) pull_to_lds<D, T>(t, naux, fin, aux_f, bn, lane, buf);
+        if (!has_node) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) f[q] = C(0);          // rest state (shifted storage)
+        }
+        C F[3] = {C(p.const_force[0]), C(p.const_force[1]), C(p.const_force[2])};
+        C rho, u[3];
+        collide<D, SHIFT, COLL, C, VAR>(f, F, p, rho, u, &ops);
+        // every lane stores: empty slots are never read, and deferred slots are rewritten by the
+        // next substep's aux collision before the fused kernel reads them
+        T *__restrict__ dst = fout + b * Q * 64 + lane;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) __builtin_nontemporal_store(T(f[q]), dst + q * 64);
+        if (!more) break;
+        b = bn;
     }
 }
 
@@ -520,7 +627,7 @@ __global__ __launch_bounds__(256) void k_aux_collide(BrickLevel<T> lv, const T *
         F[0] += C(aux_force[i * 3]); F[1] += C(aux_force[i * 3 + 1]); F[2] += C(aux_force[i * 3 + 2]);
     }
     C rho, u[3];
-    collide<D, SHIFT, COLL, C>(f, F, p, rho, u);
+    collide<D, SHIFT, COLL, C>(f, F, p, rho, u, p.mlp ? &p.mlp->ops : nullptr);
     if (!active) return;
     const int64_t slot = lv.aux_slot[i];
 #pragma unroll

@@ -24,10 +24,19 @@ __host__ __device__ constexpr int mfma_row(int h, int r) { return (r >> 2) * 8 +
 //                  layer-1 accumulator order, so activations stay in their registers); step 2
 //                  holds b2[l % 32] in slot 0 (its B slot holds 1)
 //   w3r[h][r]    = w3[mfma_row(h, r)] (FP32)
+// plus the input normalisation, training range, output scale and b3, so that the fused kernel
+// reads nothing of the network from global memory after its first store
 struct alignas(16) MlpOps {
     uint16_t w1a[64][8];
     uint16_t w2a[3][64][8];
     float w3r[2][16];
+    float mean[kMlpF];
+    float inv_std[kMlpF];
+    float fmin[kMlpF];
+    float fmax[kMlpF];
+    float nu_scale;
+    float b3;
+    float pad[2];
 };
 
 // device-resident closure: operand images, parameters (flat order W1, b1, W2, b2, w3, b3), input
@@ -138,13 +147,20 @@ __device__ __forceinline__ float siluf_(float z) {
     return z * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
 }
 
+// value held by lane l ^ 32 (v_permlane32_swap exchanges the halves of a register pair; unlike
+// ds_bpermute it is no LDS instruction, so it never waits on direct-to-LDS loads in flight)
+__device__ __forceinline__ float swap_halves(float v) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(__lane_id() < 32 ? r[1] : r[0]);
+}
+
 // network output (nu_t / nu_scale) of this lane's normalised inputs; o may live in LDS
 __device__ __forceinline__ float mlp_forward_wave(const MlpOps &o, const float *xn, float b3) {
     const int lane = __lane_id();
     const int h = lane >> 5;
     float px[kMlpF];
 #pragma unroll
-    for (int k = 0; k < kMlpF; ++k) px[k] = __shfl_xor(xn[k], 32, 64);
+    for (int k = 0; k < kMlpF; ++k) px[k] = swap_halves(xn[k]);
     // layer-1 B operands: column block 0 = nodes 0..31, block 1 = nodes 32..63; lane l carries
     // K slots 8 h .. 8 h + 7 of node l % 32 (block 0) or 32 + l % 32 (block 1); slot 9 = 1 (bias)
     bf16x8_t x0, x1;
@@ -188,26 +204,26 @@ __device__ __forceinline__ float mlp_forward_wave(const MlpOps &o, const float *
         p1 += ww * siluf_(y1[r]);
     }
     // node l % 32 + 32 h: own half of its block plus the other half's, held by lane l ^ 32
-    const float recv = __shfl_xor(h ? p0 : p1, 32, 64);
+    const float recv = swap_halves(h ? p0 : p1);
     return softplusf_((h ? p1 : p0) + recv + b3);
 }
 
 // mlp_tau_t for the FP32 solver, evaluated by the whole wavefront
-__device__ __forceinline__ float mlp_tau_t_wave(const MlpDev &m, const MlpOps &o, const float *pi, float rho,
-                                                float drho, const float *u, float tau0, float cs18) {
+__device__ __forceinline__ float mlp_tau_t_wave(const MlpOps &o, const float *pi, float rho, float drho,
+                                                const float *u, float tau0, float cs18) {
     float x[kMlpF], xn[kMlpF];
     closure_features<float>(pi, rho, drho, u, tau0, x);
     bool inside = true;
 #pragma unroll
     for (int i = 0; i < kMlpF; ++i) {
-        inside = inside && x[i] >= m.fmin[i] && x[i] <= m.fmax[i];
-        xn[i] = (x[i] - m.mean[i]) * m.inv_std[i];
+        inside = inside && x[i] >= o.fmin[i] && x[i] <= o.fmax[i];
+        xn[i] = (x[i] - o.mean[i]) * o.inv_std[i];
     }
-    const float net = mlp_forward_wave(o, xn, m.p[kMlpParams - 1]);
+    const float net = mlp_forward_wave(o, xn, o.b3);
     const float q = sqrtf(2.f * (pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2] +
                                  2.f * (pi[3] * pi[3] + pi[4] * pi[4] + pi[5] * pi[5])));
     const float smag = 0.5f * (sqrtf(tau0 * tau0 + cs18 * q) - tau0);
-    return inside ? 3.f * m.nu_scale * net : smag;
+    return inside ? 3.f * o.nu_scale * net : smag;
 }
 
 // copy of the operand images into LDS by one workgroup (followed by a barrier)
@@ -227,7 +243,7 @@ __global__ __launch_bounds__(256) void k_mlp_eval_wave(const MlpDev *m, const fl
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     float xn[kMlpF];
     for (int i = 0; i < kMlpF; ++i) xn[i] = k < n ? X[k * kMlpF + i] : 0.f;
-    const float out = mlp_forward_wave(ops, xn, m->p[kMlpParams - 1]);
+    const float out = mlp_forward_wave(ops, xn, ops.b3);
     if (k < n) y[k] = out;
 }
 

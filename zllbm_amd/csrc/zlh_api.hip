@@ -183,7 +183,7 @@ __global__ void k_upload(BrickLevel<T> lv, const double *f_in, int64_t n, const 
     }
     C F[3] = {C(p.const_force[0]), C(p.const_force[1]), C(p.const_force[2])};
     C rho, u[3];
-    collide<D, SHIFT, COLL, C>(f, F, p, rho, u);
+    collide<D, SHIFT, COLL, C>(f, F, p, rho, u, p.mlp ? &p.mlp->ops : nullptr);
     if (!active || ai >= 0) return;
 #pragma unroll
     for (int q = 0; q < Q; ++q) fa[pop_at<Q>(q, slot)] = T(f[q]);
@@ -379,6 +379,14 @@ struct zlh_ctx {
     DevBuf mlp_d;           // MlpDev of the learned closure
     bool mlp_set = false;
     int n_cu = 0;           // compute units of the context's device
+    // two background steps captured as one graph, per combination of the levels' buffer
+    // indices (two steps flip every level's buffers an even number of times, so replays chain)
+    bool use_graphs = true;
+    std::map<uint32_t, hipGraphExec_t> graph_exec;
+    void drop_graphs() {
+        for (auto &g : graph_exec) (void)hipGraphExecDestroy(g.second);
+        graph_exec.clear();
+    }
 
     zlh_status fail(zlh_status code, const char *fmt, ...) {
         char buf[1024];
@@ -497,7 +505,8 @@ void launch_stream_collide_v(zlh_ctx *ctx, int var, unsigned grid, const BrickLe
         if constexpr (mlp_on_mfma<C, V>()) {                                                                  \
             const int64_t ng = (v.nbrick + kMlpBricksPerGroup - 1) / kMlpBricksPerGroup;                     \
             hipLaunchKernelGGL((k_stream_collide_mlp<D, T, C, SHIFT, COLL, V>), dim3((unsigned)((ng + 7) / 8 * 8)), \
-                               dim3(256), 0, ctx->stream, v, fa, fb, aux_f, sp);                              \
+                               dim3(256), 0, ctx->stream, v.nbrick, v.naux, v.occ, v.auxmask, v.auxbase, v.nbr, \
+                               v.aux_missing, fa, fb, aux_f, sp);                                             \
         } else {                                                                                              \
             hipLaunchKernelGGL((k_stream_collide<D, T, C, SHIFT, COLL, V>), dim3(grid), dim3(256), 0, ctx->stream, \
                                v, fa, fb, aux_f, sp);                                                         \
@@ -839,6 +848,7 @@ void zlh_destroy(zlh_ctx *ctx) {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &e : ctx->events) { (void)hipEventDestroy(e.start); (void)hipEventDestroy(e.stop); }
+    ctx->drop_graphs();
     ctx->lv.clear();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -904,6 +914,7 @@ zlh_status zlh_ib_set_markers(zlh_ctx *ctx, int level, const double *xyz, const 
 
 zlh_status zlh_build(zlh_ctx *ctx) {
     if (!ctx) return ZLH_EINVAL;
+    ctx->drop_graphs();
     for (int l = 0; l <= ctx->d.max_level; ++l)
         if (!ctx->lv[l].have) return ctx->fail(ZLH_ESTATE, "level %d nodes not set", l);
     zlh_status s;
@@ -985,8 +996,11 @@ zlh_status zlh_transfer_levels(zlh_ctx *ctx, int fine_level) {
     return launch_transfers(ctx, fine_level);
 }
 
-zlh_status zlh_time_marching(zlh_ctx *ctx, int nsteps) {
-    if (!ctx) return ZLH_EINVAL;
+}  // extern "C"
+
+namespace {
+
+zlh_status march_launches(zlh_ctx *ctx, int nsteps) {
     // level order: a level step is followed by two substeps of the next finer level
     std::vector<int> order;
     std::function<void(int)> rec = [&](int l) {
@@ -1005,6 +1019,63 @@ zlh_status zlh_time_marching(zlh_ctx *ctx, int nsteps) {
             if (l > 0 && tsl[l] % 2 == 0 && (s = zlh_transfer_levels(ctx, l))) return s;
         }
     }
+    return ZLH_OK;
+}
+
+// capture two background steps into an executable graph (the host-side state advances as if
+// they ran; the caller launches the graph at once)
+zlh_status capture_two_steps(zlh_ctx *ctx, hipGraphExec_t *exec) {
+    ZLH_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const zlh_status s = march_launches(ctx, 2);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (s) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return s;
+    }
+    if (e != hipSuccess) return ctx->fail(ZLH_EDEVICE, "graph capture: %s", hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return ctx->fail(ZLH_EDEVICE, "graph instantiation: %s", hipGetErrorString(ei));
+    return ZLH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+zlh_status zlh_time_marching(zlh_ctx *ctx, int nsteps) {
+    if (!ctx) return ZLH_EINVAL;
+    if (!ctx->built) return ctx->fail(ZLH_ESTATE, "not built");
+    if (nsteps < 0) return ctx->fail(ZLH_EINVAL, "negative step count");
+    // launch-bound multi-level steps replay a captured graph of two background steps; timing
+    // (per-launch events) runs the launches directly
+    if (!ctx->use_graphs || ctx->timing) return march_launches(ctx, nsteps);
+    int t = 0;
+    while (nsteps - t >= 2) {
+        uint32_t key = 0;
+        for (int l = 0; l <= ctx->d.max_level; ++l) key |= (uint32_t)(ctx->lv[l].cur & 1) << l;
+        auto it = ctx->graph_exec.find(key);
+        if (it == ctx->graph_exec.end()) {
+            hipGraphExec_t g = nullptr;
+            zlh_status s = capture_two_steps(ctx, &g);   // advances the host state
+            if (s) return s;
+            ctx->graph_exec[key] = g;
+            ZLH_TRY(hipGraphLaunch(g, ctx->stream));
+        } else {
+            ZLH_TRY(hipGraphLaunch(it->second, ctx->stream));
+            // host state after two steps: same buffer indices, post-stream data current
+            for (auto &L : ctx->lv) L.fresh = false;
+        }
+        t += 2;
+    }
+    return t < nsteps ? march_launches(ctx, nsteps - t) : ZLH_OK;
+}
+
+zlh_status zlh_graphs_enable(zlh_ctx *ctx, int enable) {
+    if (!ctx) return ZLH_EINVAL;
+    ctx->use_graphs = enable != 0;
+    if (!ctx->use_graphs) ctx->drop_graphs();
     return ZLH_OK;
 }
 
@@ -1185,6 +1256,15 @@ zlh_status zlh_mlp_set(zlh_ctx *ctx, const float *params, const float *mean, con
     }
     for (int h = 0; h < 2; ++h)
         for (int r = 0; r < 16; ++r) o.w3r[h][r] = w3[mfma_row(h, r)];
+    for (int i = 0; i < kMlpF; ++i) {
+        o.mean[i] = m.mean[i];
+        o.inv_std[i] = m.inv_std[i];
+        o.fmin[i] = m.fmin[i];
+        o.fmax[i] = m.fmax[i];
+    }
+    o.nu_scale = m.nu_scale;
+    o.b3 = m.p[kMlpParams - 1];
+    o.pad[0] = o.pad[1] = 0.f;
     ZLH_TRY(hipMemcpyAsync(ctx->mlp_d.p, &m, sizeof(m), hipMemcpyHostToDevice, ctx->stream));
     ZLH_TRY(hipStreamSynchronize(ctx->stream));
     ctx->mlp_set = true;

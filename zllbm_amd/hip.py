@@ -42,7 +42,7 @@ EXPORTED = (
     "zlh_step_level", "zlh_transfer_levels", "zlh_time_marching", "zlh_download_populations", "zlh_download_macros",
     "zlh_download_source_table", "zlh_ib_download_forces", "zlh_conservation", "zlh_level_info",
     "zlh_synchronize", "zlh_timing_enable", "zlh_timing_read", "zlh_mlp_set", "zlh_mlp_gradient",
-    "zlh_mlp_train", "zlh_mlp_eval", "zlh_closure_features",
+    "zlh_mlp_train", "zlh_mlp_eval", "zlh_closure_features", "zlh_graphs_enable",
 )
 
 
@@ -86,6 +86,7 @@ def lib():
         L.zlh_mlp_train.argtypes = [P, F32, F32, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_uint64, F32, F32]
         L.zlh_mlp_eval.argtypes = [P, F32, C.c_int64, C.c_int, F32]
         L.zlh_closure_features.argtypes = [P, C.c_int, F32, F32, D(np.int32)]
+        L.zlh_graphs_enable.argtypes = [P, C.c_int]
         _lib = L
     return _lib
 
@@ -259,6 +260,9 @@ class Solver:
         fb = np.zeros(n, np.int32)
         self._chk(lib().zlh_closure_features(self.h, level, x, nu, fb))
         return x.reshape(n, 9), nu, fb
+
+    def graphs_enable(self, on=True):
+        self._chk(lib().zlh_graphs_enable(self.h, 1 if on else 0))
 
     def synchronize(self):
         self._chk(lib().zlh_synchronize(self.h))
