@@ -31,10 +31,12 @@ __device__ __forceinline__ void macro_feq_double(const double *f, int compressib
 }
 
 // NodeInfoCoarse2fine (lbm_grid_interface.cpp:570-588) + PostStreamCoarse2Fine
-// (lbm_collision_models.cpp:244-251) for every coarse node that feeds an interpolation stencil
+// (lbm_collision_models.cpp:244-251) for every coarse node that feeds an interpolation stencil.
+// The rescaled populations are stored node-major, [n_src][Q], in the populations' storage form
+// (FP64, or FP32 shifted by w), so that each stencil contributor is one contiguous read.
 template <int D, typename T, bool SHIFT>
 __global__ void k_c2f_sources(int64_t n_src, const int32_t *src_ai, const T *aux_f, int64_t naux, double tau_ratio,
-                              int compressible, double *src_f) {
+                              int compressible, T *src_f) {
     constexpr int Q = Lattice<D>::Q;
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n_src) return;
@@ -42,32 +44,61 @@ __global__ void k_c2f_sources(int64_t n_src, const int32_t *src_ai, const T *aux
     double f[Q], feq[Q], rho, u[3];
     for (int q = 0; q < Q; ++q) f[q] = double(aux_f[q * naux + j]) + (SHIFT ? Lattice<D>::w[q] : 0.);
     macro_feq_double<D>(f, compressible, rho, u, feq);
-    for (int q = 0; q < Q; ++q) src_f[q * n_src + r] = feq[q] + tau_ratio * (f[q] - feq[q]);
+    for (int q = 0; q < Q; ++q)
+        src_f[r * Q + q] = T(feq[q] + tau_ratio * (f[q] - feq[q]) - (SHIFT ? Lattice<D>::w[q] : 0.));
 }
 
 // InterpolationLagrangian{2D,3D} (grid/grid_interpolation.cpp:224-304, 433-525): each ghost node
-// is the weighted sum of its contributors, accumulated in stencil order
+// is the weighted sum of its contributors, accumulated in stencil order.  A contributor is one
+// 8-byte word (source row, FP32 weight: the tensor-product Lagrange weights are dyadic rationals,
+// exact in FP32).  Four threads per ghost node, each over 5 (D2Q9: 5 and 4) directions, and
+// contributors loaded 8 at a time, so the ~16 dependent loads of a node overlap.
+constexpr int kGatherDirs = 5;
 template <int D, typename T, bool SHIFT>
-__global__ void k_c2f_gather(int64_t n_tgt, const int32_t *tgt_ai, const int32_t *begin, const int32_t *src,
-                             const double *coeff, const double *src_f, int64_t n_src, T *aux_f, int64_t naux) {
+__global__ void k_c2f_gather(int64_t n_tgt, const int32_t *__restrict__ tgt_ai, const int32_t *__restrict__ begin,
+                             const int2 *__restrict__ contrib, const T *__restrict__ src_f, T *__restrict__ aux_f,
+                             int64_t naux) {
     constexpr int Q = Lattice<D>::Q;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int NQ = kGatherDirs;
+    constexpr int PARTS = (Q + NQ - 1) / NQ;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = idx / PARTS;
+    const int q0 = (int)(idx - t * PARTS) * NQ;
     if (t >= n_tgt) return;
-    double acc[Q];
-    for (int q = 0; q < Q; ++q) acc[q] = 0.;
-    for (int32_t k = begin[t]; k < begin[t + 1]; ++k) {
-        const double c = coeff[k];
-        const int32_t s = src[k];
-        if (s >= 0) {
-            for (int q = 0; q < Q; ++q) acc[q] = acc[q] + c * src_f[q * n_src + s];
-        } else {
-            const int64_t fi = -(int64_t)s - 1;
-            for (int q = 0; q < Q; ++q)
-                acc[q] = acc[q] + c * (double(aux_f[q * naux + fi]) + (SHIFT ? Lattice<D>::w[q] : 0.));
+    double acc[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) acc[i] = 0.;
+    const int32_t k0 = begin[t], k1 = begin[t + 1];
+    for (int32_t kb = k0; kb < k1; kb += 8) {
+        int2 ck[8];
+        T vk[8][NQ];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) ck[j] = contrib[kb + j < k1 ? kb + j : k1 - 1];   // clamped: in bounds
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                const int q = q0 + i < Q ? q0 + i : Q - 1;
+                // select the address, then load once (a select of two loads becomes a branch
+                // with a wait inside it)
+                const T *ptr = ck[j].x >= 0 ? src_f + (int64_t)ck[j].x * Q + q : aux_f + q * naux + (-(int64_t)ck[j].x - 1);
+                vk[j][i] = *ptr;
+            }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if (kb + j >= k1) break;
+            const double c = (double)__int_as_float(ck[j].y);
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                const double wq = SHIFT ? Lattice<D>::w[q0 + i < Q ? q0 + i : 0] : 0.;
+                acc[i] = acc[i] + c * (double(vk[j][i]) + wq);
+            }
         }
     }
     const int64_t ai = tgt_ai[t];
-    for (int q = 0; q < Q; ++q) aux_f[q * naux + ai] = T(acc[q] - (SHIFT ? Lattice<D>::w[q] : 0.));
+#pragma unroll
+    for (int i = 0; i < NQ; ++i)
+        if (q0 + i < Q) aux_f[(q0 + i) * naux + ai] = T(acc[i] - (SHIFT ? Lattice<D>::w[q0 + i] : 0.));
 }
 
 // NodeInfoFine2Coarse (lbm_grid_interface.cpp:594-610) + PostStreamFine2Coarse
@@ -396,10 +427,17 @@ inline zlh_status upload_transfer(zlh_ctx *ctx, int level) {
     X.n_f2c = (int64_t)f2c_c.size();
     if ((s = upload(ctx, X.c2f_target, tgt))) return s;
     if ((s = upload(ctx, X.c2f_begin, begin))) return s;
-    if ((s = upload(ctx, X.c2f_src, src))) return s;
-    if ((s = upload(ctx, X.c2f_coeff, P.coeff))) return s;
+    std::vector<int2> packed(src.size());
+    for (size_t k = 0; k < src.size(); ++k) {
+        const float cf = (float)P.coeff[k];
+        if ((double)cf != P.coeff[k]) return ctx->fail(ZLH_EDEVICE, "interpolation weight not exact in FP32");
+        int32_t bits;
+        std::memcpy(&bits, &cf, 4);
+        packed[k] = make_int2(src[k], bits);
+    }
+    if ((s = upload(ctx, X.c2f_contrib, packed))) return s;
     if ((s = upload(ctx, X.src_coarse_ai, src_ai))) return s;
-    if ((s = dalloc(ctx, X.src_f, sizeof(double) * ctx->q * std::max<int64_t>(X.n_src, 1)))) return s;
+    if ((s = dalloc(ctx, X.src_f, ctx->tsize() * ctx->q * std::max<int64_t>(X.n_src, 1)))) return s;
     if ((s = upload(ctx, X.f2c_coarse_ai, f2c_c))) return s;
     if ((s = upload(ctx, X.f2c_fine_ai, f2c_f))) return s;
     return ZLH_OK;
@@ -450,10 +488,10 @@ zlh_status launch_transfers_t(zlh_ctx *ctx, int level) {
     if (X.n_c2f > 0) {
         if (X.n_src > 0)
             hipLaunchKernelGGL((k_c2f_sources<D, T, SHIFT>), dim3(blocks(X.n_src)), dim3(256), 0, ctx->stream, X.n_src,
-                               X.src_coarse_ai.as<int32_t>(), C.aux_f_d.as<T>(), C.naux, c2f, comp, X.src_f.as<double>());
-        hipLaunchKernelGGL((k_c2f_gather<D, T, SHIFT>), dim3(blocks(X.n_c2f)), dim3(256), 0, ctx->stream, X.n_c2f,
-                           X.c2f_target.as<int32_t>(), X.c2f_begin.as<int32_t>(), X.c2f_src.as<int32_t>(),
-                           X.c2f_coeff.as<double>(), X.src_f.as<double>(), X.n_src, F.aux_f_d.as<T>(), F.naux);
+                               X.src_coarse_ai.as<int32_t>(), C.aux_f_d.as<T>(), C.naux, c2f, comp, X.src_f.as<T>());
+        hipLaunchKernelGGL((k_c2f_gather<D, T, SHIFT>), dim3(blocks(X.n_c2f * ((ctx->q + kGatherDirs - 1) / kGatherDirs))),
+                           dim3(256), 0, ctx->stream, X.n_c2f, X.c2f_target.as<int32_t>(), X.c2f_begin.as<int32_t>(),
+                           X.c2f_contrib.as<int2>(), X.src_f.as<T>(), F.aux_f_d.as<T>(), F.naux);
     }
     if (X.n_f2c > 0)
         hipLaunchKernelGGL((k_f2c<D, T, SHIFT>), dim3(blocks(X.n_f2c)), dim3(256), 0, ctx->stream, X.n_f2c,

@@ -540,15 +540,15 @@ __device__ __forceinline__ void pull_to_lds(const BrickTables &t, int64_t naux, 
 }
 
 // The same with the FP32 learned closure.  Each workgroup stages the network's operand images
-// in LDS once; its four waves then take four rounds of four adjacent bricks (16 per workgroup),
-// and each wave pulls its next brick into LDS while it collides the current one, so the pull's
+// in LDS once; its four waves then take ROUNDS rounds of four adjacent bricks (4 on large
+// levels; 1 on small ones, where the grid would not fill the chip otherwise), and each wave
+// pulls its next brick into LDS while it collides the current one, so the pull's
 // latency hides behind the network's arithmetic at the occupancy its registers allow.  Within
 // a round the order is: read the staged brick, write deferred nodes' post-stream values, issue
 // the next pull, collide, store; the store group is unconditional so that the next round's wait
 // (in-order vector memory counter) covers the pull and not these stores.
 // Register budget: 4 waves per SIMD unforced, 3 forced (whose Guo terms would spill at 4).
-constexpr int kMlpBricksPerGroup = 16;
-template <int D, typename T, typename C, bool SHIFT, int COLL, int VAR>
+template <int D, typename T, typename C, bool SHIFT, int COLL, int VAR, int ROUNDS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((VAR & 1) ? 3 : 4)))
 void k_stream_collide_mlp(int64_t nbrick, int64_t naux, const uint64_t *__restrict__ occ,
                           const uint64_t *__restrict__ auxmask, const int32_t *__restrict__ auxbase,
@@ -560,16 +560,17 @@ void k_stream_collide_mlp(int64_t nbrick, int64_t naux, const uint64_t *__restri
     constexpr int Q = L::Q;
     __shared__ MlpOps ops;
     __shared__ T stage[4][Q * 64];
-    const int64_t g = xcd_group((nbrick + kMlpBricksPerGroup - 1) / kMlpBricksPerGroup);
+    constexpr int64_t per_group = 4 * ROUNDS;
+    const int64_t g = xcd_group((nbrick + per_group - 1) / per_group);
     if (g < 0) return;                       // uniform per workgroup: before the barrier
     stage_mlp_ops(ops, p.mlp->ops);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     T *buf = stage[wave];
-    int64_t b = g * kMlpBricksPerGroup + wave;
+    int64_t b = g * per_group + wave;
     if (b >= nbrick) return;                 // uniform per wave; no barrier follows
     pull_to_lds<D, T>(t, naux, fin, aux_f, b, threadIdx.x & 63, buf);
 #pragma unroll 1
-    for (int r = 0; r < kMlpBricksPerGroup / 4; ++r) {
+    for (int r = 0; r < ROUNDS; ++r) {
         int lane = threadIdx.x & 63;
         // opaque per brick: keeps the lane's stencil offsets from being hoisted out of the loop
         asm volatile("" : "+v"(lane));
@@ -587,7 +588,7 @@ void k_stream_collide_mlp(int64_t nbrick, int64_t naux, const uint64_t *__restri
         // the buffer's reads have returned before it is refilled
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
         const int64_t bn = b + 4;
-        const bool more = r + 1 < kMlpBricksPerGroup / 4 && bn < nbrick;
+        const bool more = r + 1 < ROUNDS && bn < nbrick;
         if (more) pull_to_lds<D, T>(t, naux, fin, aux_f, bn, lane, buf);
         if (!has_node) {
 #pragma unroll
@@ -608,26 +609,35 @@ void k_stream_collide_mlp(int64_t nbrick, int64_t naux, const uint64_t *__restri
 
 // Collision of the aux nodes (deferred from the previous substep), writing post-collision
 // values into the buffer the next fused kernel reads, plus rho/u (GridNodeLbm::rho_/velocity_).
-template <int D, typename T, typename C, bool SHIFT, int COLL>
+// VAR as in the fused kernel; with the FP32 learned closure the workgroup stages the network in
+// LDS first.
+template <int D, typename T, typename C, bool SHIFT, int COLL, int VAR>
 __global__ __launch_bounds__(256) void k_aux_collide(BrickLevel<T> lv, const T *__restrict__ aux_f,
                                                      const double *__restrict__ aux_force,
                                                      T *__restrict__ fa, double *__restrict__ aux_macro,
                                                      StepParams p) {
     constexpr int Q = Lattice<D>::Q;
+    constexpr bool kWave = mlp_on_mfma<C, VAR>();
+    const MlpOps *ops = nullptr;
+    if constexpr (kWave) {
+        __shared__ MlpOps s_ops;
+        stage_mlp_ops(s_ops, p.mlp->ops);
+        ops = &s_ops;
+    }
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     // lanes past the end run the collision on the rest state (FP32 learned closure is
     // wavefront-wide) and store nothing
     const bool active = i < lv.naux;
-    if (!mlp_on_mfma<C, -1>() && !active) return;
+    if (!kWave && !active) return;
     C f[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) f[q] = active ? C(aux_f[q * lv.naux + i]) : C(SHIFT ? 0. : Lattice<D>::w[q]);
     C F[3] = {C(p.const_force[0]), C(p.const_force[1]), C(p.const_force[2])};
-    if (active && p.has_force && aux_force) {
+    if (active && (VAR & 1) && aux_force) {
         F[0] += C(aux_force[i * 3]); F[1] += C(aux_force[i * 3 + 1]); F[2] += C(aux_force[i * 3 + 2]);
     }
     C rho, u[3];
-    collide<D, SHIFT, COLL, C>(f, F, p, rho, u, p.mlp ? &p.mlp->ops : nullptr);
+    collide<D, SHIFT, COLL, C, VAR>(f, F, p, rho, u, ops);
     if (!active) return;
     const int64_t slot = lv.aux_slot[i];
 #pragma unroll

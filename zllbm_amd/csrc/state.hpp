@@ -37,10 +37,10 @@ struct TransferTables {
     int64_t n_f2c = 0;          // coarse nodes copied from the fine level
     DevBuf c2f_target;          // [n_c2f] fine aux index
     DevBuf c2f_begin;           // [n_c2f + 1] offsets into the contributor list
-    DevBuf c2f_src;             // contributor: >= 0 coarse source row, < 0: -(fine aux index) - 1
-    DevBuf c2f_coeff;           // contributor weight (double)
+    DevBuf c2f_contrib;         // contributor {source, FP32 weight bits}: source >= 0 coarse source
+                                // row, < 0: -(fine aux index) - 1
     DevBuf src_coarse_ai;       // [n_src] coarse aux index of each source row
-    DevBuf src_f;               // [Q][n_src] rescaled coarse populations (double, unshifted)
+    DevBuf src_f;               // [n_src][Q] rescaled coarse populations (storage form)
     DevBuf f2c_coarse_ai;       // [n_f2c]
     DevBuf f2c_fine_ai;         // [n_f2c] fine aux index of the coincident node
 };

@@ -503,10 +503,18 @@ void launch_stream_collide_v(zlh_ctx *ctx, int var, unsigned grid, const BrickLe
 #define ZLH_SC(V)                                                                                             \
     case V:                                                                                                   \
         if constexpr (mlp_on_mfma<C, V>()) {                                                                  \
-            const int64_t ng = (v.nbrick + kMlpBricksPerGroup - 1) / kMlpBricksPerGroup;                     \
-            hipLaunchKernelGGL((k_stream_collide_mlp<D, T, C, SHIFT, COLL, V>), dim3((unsigned)((ng + 7) / 8 * 8)), \
-                               dim3(256), 0, ctx->stream, v.nbrick, v.naux, v.occ, v.auxmask, v.auxbase, v.nbr, \
-                               v.aux_missing, fa, fb, aux_f, sp);                                             \
+            /* 4 rounds once the grid still gives every CU 4 workgroups */                                \
+            if (v.nbrick >= (int64_t)16 * 4 * ctx->n_cu) {                                                   \
+                const int64_t ng = (v.nbrick + 15) / 16;                                                      \
+                hipLaunchKernelGGL((k_stream_collide_mlp<D, T, C, SHIFT, COLL, V, 4>), dim3((unsigned)((ng + 7) / 8 * 8)), \
+                                   dim3(256), 0, ctx->stream, v.nbrick, v.naux, v.occ, v.auxmask, v.auxbase, v.nbr, \
+                                   v.aux_missing, fa, fb, aux_f, sp);                                         \
+            } else {                                                                                          \
+                const int64_t ng = (v.nbrick + 3) / 4;                                                        \
+                hipLaunchKernelGGL((k_stream_collide_mlp<D, T, C, SHIFT, COLL, V, 1>), dim3((unsigned)((ng + 7) / 8 * 8)), \
+                                   dim3(256), 0, ctx->stream, v.nbrick, v.naux, v.occ, v.auxmask, v.auxbase, v.nbr, \
+                                   v.aux_missing, fa, fb, aux_f, sp);                                         \
+            }                                                                                                 \
         } else {                                                                                              \
             hipLaunchKernelGGL((k_stream_collide<D, T, C, SHIFT, COLL, V>), dim3(grid), dim3(256), 0, ctx->stream, \
                                v, fa, fb, aux_f, sp);                                                         \
@@ -517,6 +525,21 @@ void launch_stream_collide_v(zlh_ctx *ctx, int var, unsigned grid, const BrickLe
         default: break;
     }
 #undef ZLH_SC
+}
+
+template <int D, typename T, typename C, bool SHIFT, int COLL>
+void launch_aux_collide_v(zlh_ctx *ctx, int var, LevelState &L, const BrickLevel<T> &v, T *aux_f,
+                          const double *aux_force, T *fa) {
+#define ZLH_AC(V)                                                                                             \
+    case V:                                                                                                   \
+        hipLaunchKernelGGL((k_aux_collide<D, T, C, SHIFT, COLL, V>), dim3(blocks(L.naux)), dim3(256), 0,       \
+                           ctx->stream, v, aux_f, aux_force, fa, L.aux_macro_d.as<double>(), L.sp);           \
+        break;
+    switch (var) {
+        ZLH_AC(0) ZLH_AC(1) ZLH_AC(2) ZLH_AC(3) ZLH_AC(4) ZLH_AC(5) ZLH_AC(6) ZLH_AC(7)
+        default: break;
+    }
+#undef ZLH_AC
 }
 
 template <int D, typename T, typename C, bool SHIFT>
@@ -533,13 +556,10 @@ zlh_status launch_step_kernels(zlh_ctx *ctx, int level, int coll) {
     T *fa = L.f_d[L.cur].as<T>(), *fb = L.f_d[L.cur ^ 1].as<T>();
     T *aux_f = L.aux_f_d.as<T>();
     const double *aux_force = L.aux_force_d.as<double>();
+    const int var = 2 * les_code(L.sp.les, L.sp.les_tau_mode) + (L.sp.has_force ? 1 : 0);
     if (L.naux > 0) {
-        if (coll == ZLH_SRT)
-            hipLaunchKernelGGL((k_aux_collide<D, T, C, SHIFT, 0>), dim3(blocks(L.naux)), dim3(256), 0, ctx->stream,
-                               v, aux_f, aux_force, fa, L.aux_macro_d.as<double>(), L.sp);
-        else
-            hipLaunchKernelGGL((k_aux_collide<D, T, C, SHIFT, 1>), dim3(blocks(L.naux)), dim3(256), 0, ctx->stream,
-                               v, aux_f, aux_force, fa, L.aux_macro_d.as<double>(), L.sp);
+        if (coll == ZLH_SRT) launch_aux_collide_v<D, T, C, SHIFT, 0>(ctx, var, L, v, aux_f, aux_force, fa);
+        else launch_aux_collide_v<D, T, C, SHIFT, 1>(ctx, var, L, v, aux_f, aux_force, fa);
     }
     const int64_t groups = (L.nbrick + 3) / 4;
     const unsigned grid = (unsigned)(((groups + 7) / 8) * 8);
@@ -549,8 +569,7 @@ zlh_status launch_step_kernels(zlh_ctx *ctx, int level, int coll) {
         ZLH_TRY(hipEventCreate(&e1));
         ZLH_TRY(hipEventRecord(e0, ctx->stream));
     }
-    launch_stream_collide<D, T, C, SHIFT>(ctx, coll, 2 * les_code(L.sp.les, L.sp.les_tau_mode) + (L.sp.has_force ? 1 : 0),
-                                          grid, v, fa, fb, aux_f, L.sp);
+    launch_stream_collide<D, T, C, SHIFT>(ctx, coll, var, grid, v, fa, fb, aux_f, L.sp);
     if (ctx->timing) {
         ZLH_TRY(hipEventRecord(e1, ctx->stream));
         ctx->events.push_back({e0, e1, level});
