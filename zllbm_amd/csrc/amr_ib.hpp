@@ -49,50 +49,76 @@ __global__ void k_c2f_sources(int64_t n_src, const int32_t *src_ai, const T *aux
 }
 
 // InterpolationLagrangian{2D,3D} (grid/grid_interpolation.cpp:224-304, 433-525): each ghost node
-// is the weighted sum of its contributors, accumulated in stencil order.  A contributor is one
-// 8-byte word (source row, FP32 weight: the tensor-product Lagrange weights are dyadic rationals,
-// exact in FP32).  Four threads per ghost node, each over 5 (D2Q9: 5 and 4) directions, and
-// contributors loaded 8 at a time, so the ~16 dependent loads of a node overlap.
+// is the weighted sum of its contributors, accumulated in stencil order.  A workgroup takes a
+// group of consecutive (Morton-ordered, so neighbouring) ghost nodes: it first copies the union
+// of their coarse source rows into LDS with coalesced loads (neighbours share most of their
+// 4^d stencils), then each ghost node is summed from LDS by four threads of 5 (D2Q9: 5 and 4)
+// directions.  A contributor is one 8-byte word {LDS row or fine fallback, FP32 weight}: the
+// tensor-product Lagrange weights are dyadic rationals, exact in FP32.
 constexpr int kGatherDirs = 5;
 template <int D, typename T, bool SHIFT>
-__global__ void k_c2f_gather(int64_t n_tgt, const int32_t *__restrict__ tgt_ai, const int32_t *__restrict__ begin,
-                             const int2 *__restrict__ contrib, const T *__restrict__ src_f, T *__restrict__ aux_f,
-                             int64_t naux) {
+__global__ __launch_bounds__(256) void k_c2f_gather(int64_t n_tgt, int group, int max_rows,
+                                                    const int32_t *__restrict__ tgt_ai,
+                                                    const int32_t *__restrict__ begin,
+                                                    const int2 *__restrict__ contrib,
+                                                    const int32_t *__restrict__ rows_begin,
+                                                    const int32_t *__restrict__ rows, const T *__restrict__ src_f,
+                                                    T *__restrict__ aux_f, int64_t naux) {
     constexpr int Q = Lattice<D>::Q;
     constexpr int NQ = kGatherDirs;
     constexpr int PARTS = (Q + NQ - 1) / NQ;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t t = idx / PARTS;
-    const int q0 = (int)(idx - t * PARTS) * NQ;
-    if (t >= n_tgt) return;
+    extern __shared__ unsigned char lds_raw[];
+    T *rowbuf = reinterpret_cast<T *>(lds_raw);                                  // [max_rows][Q]
+    int2 *cbuf = reinterpret_cast<int2 *>(lds_raw + (((size_t)max_rows * Q * sizeof(T) + 15) & ~(size_t)15));
+    const int64_t gi = blockIdx.x;
+    const int64_t t_first = gi * group;
+    const int64_t t_end = t_first + group < n_tgt ? t_first + group : n_tgt;
+    const int32_t c0 = begin[t_first], nc = begin[t_end] - c0;
+    const int32_t r0 = rows_begin[gi], nr = rows_begin[gi + 1] - r0;
+    // staging: every load of a thread is issued before its first LDS write (a load followed by
+    // its store, repeated, would wait once per element)
+    for (int rb = 0; rb < nr; rb += 256) {
+        const int r = rb + (int)threadIdx.x;
+        if (r < nr) {
+            const T *row = src_f + (int64_t)rows[r0 + r] * Q;
+            T v[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) v[q] = row[q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) rowbuf[r * Q + q] = v[q];
+        }
+    }
+    for (int cb = 0; cb < nc; cb += 4 * 256) {
+        int2 w[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int e = cb + j * 256 + (int)threadIdx.x;
+            w[j] = contrib[c0 + (e < nc ? e : nc - 1)];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int e = cb + j * 256 + (int)threadIdx.x;
+            if (e < nc) cbuf[e] = w[j];
+        }
+    }
+    __syncthreads();
+    const int local = threadIdx.x / PARTS;
+    const int q0 = (threadIdx.x - local * PARTS) * NQ;
+    const int64_t t = t_first + local;
+    if (local >= group || t >= n_tgt) return;
     double acc[NQ];
 #pragma unroll
     for (int i = 0; i < NQ; ++i) acc[i] = 0.;
-    const int32_t k0 = begin[t], k1 = begin[t + 1];
-    for (int32_t kb = k0; kb < k1; kb += 8) {
-        int2 ck[8];
-        T vk[8][NQ];
+    const int32_t k1 = begin[t + 1] - c0;
+    for (int32_t k = begin[t] - c0; k < k1; ++k) {
+        const int2 cw = cbuf[k];
+        const double c = (double)__int_as_float(cw.y);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) ck[j] = contrib[kb + j < k1 ? kb + j : k1 - 1];   // clamped: in bounds
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-#pragma unroll
-            for (int i = 0; i < NQ; ++i) {
-                const int q = q0 + i < Q ? q0 + i : Q - 1;
-                // select the address, then load once (a select of two loads becomes a branch
-                // with a wait inside it)
-                const T *ptr = ck[j].x >= 0 ? src_f + (int64_t)ck[j].x * Q + q : aux_f + q * naux + (-(int64_t)ck[j].x - 1);
-                vk[j][i] = *ptr;
-            }
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            if (kb + j >= k1) break;
-            const double c = (double)__int_as_float(ck[j].y);
-#pragma unroll
-            for (int i = 0; i < NQ; ++i) {
-                const double wq = SHIFT ? Lattice<D>::w[q0 + i < Q ? q0 + i : 0] : 0.;
-                acc[i] = acc[i] + c * (double(vk[j][i]) + wq);
-            }
+        for (int i = 0; i < NQ; ++i) {
+            if (q0 + i >= Q) break;
+            const int q = q0 + i;
+            const double v = cw.x >= 0 ? double(rowbuf[cw.x * Q + q]) : double(aux_f[q * naux + (-(int64_t)cw.x - 1)]);
+            acc[i] = acc[i] + c * (v + (SHIFT ? Lattice<D>::w[q] : 0.));
         }
     }
     const int64_t ai = tgt_ai[t];
@@ -427,14 +453,51 @@ inline zlh_status upload_transfer(zlh_ctx *ctx, int level) {
     X.n_f2c = (int64_t)f2c_c.size();
     if ((s = upload(ctx, X.c2f_target, tgt))) return s;
     if ((s = upload(ctx, X.c2f_begin, begin))) return s;
+    // gather groups: consecutive ghost nodes, their source rows staged in LDS; the group is
+    // halved until the largest row set fits the LDS budget
+    const int Qn = ctx->q;
+    const size_t lds_budget = 48 * 1024;
     std::vector<int2> packed(src.size());
-    for (size_t k = 0; k < src.size(); ++k) {
-        const float cf = (float)P.coeff[k];
-        if ((double)cf != P.coeff[k]) return ctx->fail(ZLH_EDEVICE, "interpolation weight not exact in FP32");
-        int32_t bits;
-        std::memcpy(&bits, &cf, 4);
-        packed[k] = make_int2(src[k], bits);
+    std::vector<int32_t> group_begin, group_rows;
+    int group = 256 / ((Qn + kGatherDirs - 1) / kGatherDirs);
+    for (;; group /= 2) {
+        group_begin.assign(1, 0);
+        group_rows.clear();
+        int max_rows = 0;
+        int64_t max_contrib = 0;
+        const int64_t nt = (int64_t)tgt.size();
+        for (int64_t g0 = 0; g0 < nt; g0 += group) {
+            std::map<int32_t, int32_t> local;
+            for (int64_t t = g0; t < std::min<int64_t>(nt, g0 + group); ++t)
+                for (int32_t k = begin[t]; k < begin[t + 1]; ++k) {
+                    const float cf = (float)P.coeff[k];
+                    if ((double)cf != P.coeff[k]) return ctx->fail(ZLH_EDEVICE, "interpolation weight not exact in FP32");
+                    int32_t bits;
+                    std::memcpy(&bits, &cf, 4);
+                    int32_t sx = src[k];
+                    if (sx >= 0) {
+                        auto it = local.find(sx);
+                        if (it == local.end()) {
+                            it = local.emplace(sx, (int32_t)local.size()).first;
+                            group_rows.push_back(sx);
+                        }
+                        sx = it->second;
+                    }
+                    packed[k] = make_int2(sx, bits);
+                }
+            max_rows = std::max<int>(max_rows, (int)local.size());
+            max_contrib = std::max<int64_t>(max_contrib, begin[std::min<int64_t>(nt, g0 + group)] - begin[g0]);
+            group_begin.push_back((int32_t)group_rows.size());
+        }
+        X.max_rows = max_rows;
+        X.lds_bytes = (((size_t)max_rows * Qn * ctx->tsize() + 15) & ~(size_t)15) + (size_t)max_contrib * 8;
+        if (X.lds_bytes <= lds_budget) break;
+        if (group == 1) return ctx->fail(ZLH_EDEVICE, "interpolation stencil too large for LDS");
     }
+    X.group = group;
+    X.n_groups = (int64_t)group_begin.size() - 1;
+    if ((s = upload(ctx, X.group_rows_begin, group_begin))) return s;
+    if ((s = upload(ctx, X.group_rows, group_rows))) return s;
     if ((s = upload(ctx, X.c2f_contrib, packed))) return s;
     if ((s = upload(ctx, X.src_coarse_ai, src_ai))) return s;
     if ((s = dalloc(ctx, X.src_f, ctx->tsize() * ctx->q * std::max<int64_t>(X.n_src, 1)))) return s;
@@ -489,9 +552,11 @@ zlh_status launch_transfers_t(zlh_ctx *ctx, int level) {
         if (X.n_src > 0)
             hipLaunchKernelGGL((k_c2f_sources<D, T, SHIFT>), dim3(blocks(X.n_src)), dim3(256), 0, ctx->stream, X.n_src,
                                X.src_coarse_ai.as<int32_t>(), C.aux_f_d.as<T>(), C.naux, c2f, comp, X.src_f.as<T>());
-        hipLaunchKernelGGL((k_c2f_gather<D, T, SHIFT>), dim3(blocks(X.n_c2f * ((ctx->q + kGatherDirs - 1) / kGatherDirs))),
-                           dim3(256), 0, ctx->stream, X.n_c2f, X.c2f_target.as<int32_t>(), X.c2f_begin.as<int32_t>(),
-                           X.c2f_contrib.as<int2>(), X.src_f.as<T>(), F.aux_f_d.as<T>(), F.naux);
+        hipLaunchKernelGGL((k_c2f_gather<D, T, SHIFT>), dim3((unsigned)X.n_groups), dim3(256), X.lds_bytes,
+                           ctx->stream, X.n_c2f, X.group, X.max_rows,
+                           X.c2f_target.as<int32_t>(), X.c2f_begin.as<int32_t>(), X.c2f_contrib.as<int2>(),
+                           X.group_rows_begin.as<int32_t>(), X.group_rows.as<int32_t>(), X.src_f.as<T>(),
+                           F.aux_f_d.as<T>(), F.naux);
     }
     if (X.n_f2c > 0)
         hipLaunchKernelGGL((k_f2c<D, T, SHIFT>), dim3(blocks(X.n_f2c)), dim3(256), 0, ctx->stream, X.n_f2c,

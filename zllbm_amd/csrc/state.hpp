@@ -37,8 +37,14 @@ struct TransferTables {
     int64_t n_f2c = 0;          // coarse nodes copied from the fine level
     DevBuf c2f_target;          // [n_c2f] fine aux index
     DevBuf c2f_begin;           // [n_c2f + 1] offsets into the contributor list
-    DevBuf c2f_contrib;         // contributor {source, FP32 weight bits}: source >= 0 coarse source
-                                // row, < 0: -(fine aux index) - 1
+    DevBuf c2f_contrib;         // contributor {source, FP32 weight bits}: source >= 0 row of the
+                                // group's staged rows, < 0: -(fine aux index) - 1
+    int group = 0;              // ghost nodes per gather workgroup (consecutive, Morton order)
+    int64_t n_groups = 0;
+    int max_rows = 0;           // most source rows one group stages in LDS
+    size_t lds_bytes = 0;       // rows plus contributor words of the largest group
+    DevBuf group_rows_begin;    // [n_groups + 1] offsets into group_rows
+    DevBuf group_rows;          // source rows (into src_f) of each group, first use order
     DevBuf src_coarse_ai;       // [n_src] coarse aux index of each source row
     DevBuf src_f;               // [n_src][Q] rescaled coarse populations (storage form)
     DevBuf f2c_coarse_ai;       // [n_f2c]
