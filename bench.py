@@ -34,9 +34,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+MLP_OPERANDS = "fp16"
+
+
 def make_solver(hip, W, precision, les, model=None):
     kw = dict(W.solver)
     kw.update(les=les)
+    if les == hip.LES_MLP:
+        kw.update(mlp_operands=hip.MLP_OPERANDS_FP16 if MLP_OPERANDS == "fp16" else hip.MLP_OPERANDS_BF16)
     s = hip.Solver(W.grid, precision=precision, **kw).build()
     if model is not None:
         s.mlp_set(model)
@@ -68,6 +73,21 @@ def state(s, W):
         rho.append(r)
         u.append(v)
     return np.concatenate(rho), np.concatenate(u)
+
+
+def l2_curve(hip, W, les, model, steps, every):
+    """Relative velocity L2 and density RMS between the FP32 and FP64 forms of the same model,
+    stepped in lockstep from the same initial state, every `every` steps."""
+    a = make_solver(hip, W, hip.FP32_SHIFTED, les, model)
+    b = make_solver(hip, W, hip.FP64, les, model)
+    out = []
+    for t in range(every, steps + 1, every):
+        a.time_marching(every)
+        b.time_marching(every)
+        ra, ua = state(a, W)
+        rb, ub = state(b, W)
+        out.append((t, float(np.sqrt(np.sum((ua - ub) ** 2) / np.sum(ub ** 2))), float(np.sqrt(np.mean((ra - rb) ** 2)))))
+    return out
 
 
 def cpu_baseline(model, seconds=15.0):
@@ -104,10 +124,16 @@ def main():
     ap.add_argument("--workload", default="amr4", choices=["amr4", "tgv256"])
     ap.add_argument("--closure", default=DEFAULT_CLOSURE, help="trained closure (zllbm_amd.train_closure)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 runs")
+    ap.add_argument("--mlp-operands", default="fp16", choices=["fp16", "bf16"],
+                    help="operands of the FP32 solver's matrix-core closure network")
+    ap.add_argument("--l2-steps", type=int, default=1000)
+    ap.add_argument("--l2-every", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (for counter calibration)")
     args = ap.parse_args()
+    global MLP_OPERANDS
+    MLP_OPERANDS = args.mlp_operands
     if args.calibrate_copy:
         import torch
         n = args.calibrate_copy * (1 << 20) // 4
@@ -135,7 +161,7 @@ def main():
     log("workload:", W.name, "nodes per level", W.nodes)
     s = make_solver(hip, W, hip.FP32_SHIFTED, les, model)
     log("aux nodes per level", [s.info(l)[1] for l in range(len(W.nodes))])
-    wall, k_ms, k_n, done = timed(s, W, args.warmup, args.steps)
+    wall, k_ms, k_n, _ = timed(s, W, args.warmup, args.steps)
     mlups = W.updates_per_step * args.steps / wall / 1e6
     q = 19
     bpu = 2 * q * 4
@@ -147,13 +173,12 @@ def main():
     kernel = ("k_stream_collide_mlp" if model is not None else "k_stream_collide") + " (level %d)" % finest
     out.update({"value": round(mlups, 2), "unit": "MLUPS", "ms_per_step": round(wall / args.steps * 1e3, 4),
                 "config": {"workload": W.name, "nodes_per_level": W.nodes, "updates_per_step": W.updates_per_step,
-                           "collision": "MRT", "closure": "learned (BF16 MFMA), Smagorinsky fallback"
+                           "collision": "MRT", "closure": "learned (%s-operand MFMA), Smagorinsky fallback" % args.mlp_operands.upper()
                            if model is not None else "Smagorinsky Cs 0.1", "populations": "FP32 shifted"},
                 "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
                              "avg_launch_ms": round(avg_s * 1e3, 4), "bytes_per_update": bpu,
                              "updates_per_launch": W.nodes[finest]}})
-    rho32, u32 = state(s, W)
     if model is not None:
         fb = []
         for l in range(len(W.nodes)):
@@ -167,19 +192,22 @@ def main():
         del s64
         mlups64 = W.updates_per_step * c_steps / wall64 / 1e6
         log("FP64 Smagorinsky: %.3f ms per step" % (wall64 / c_steps * 1e3))
-        s64 = make_solver(hip, W, hip.FP64, hip.LES_MLP, model)
-        s64.time_marching(done)
-        rho64, u64 = state(s64, W)
-        del s64
-        l2u = float(np.sqrt(np.sum((u32 - u64) ** 2) / np.sum(u64 ** 2)))
-        l2r = float(np.sqrt(np.mean((rho32 - rho64) ** 2)))
-        log("L2 after %d steps: u %.3e, rho %.3e" % (done, l2u, l2r))
+        # accuracy: FP32 against FP64, same closure, in lockstep (the learned closure, and for
+        # reference the plain Smagorinsky pair, whose difference is FP32 rounding alone)
+        curve = l2_curve(hip, W, hip.LES_MLP, model, args.l2_steps, args.l2_every)
+        curve_s = l2_curve(hip, W, hip.LES_SMAGORINSKY, None, args.l2_steps, args.l2_every)
+        for (t, u1, r1), (_, u2, _) in zip(curve, curve_s):
+            log("L2(u) after %4d steps: learned FP32 vs FP64 %.3e, Smagorinsky FP32 vs FP64 %.3e" % (t, u1, u2))
         out["comparison"] = {"fp64_smagorinsky_mlups": round(mlups64, 2),
                              "fp64_smagorinsky_ms_per_step": round(wall64 / c_steps * 1e3, 4),
                              "speedup_fp32_learned_vs_fp64_smagorinsky": round(mlups / mlups64, 3),
-                             "l2_u_vs_fp64": l2u, "l2_rho_vs_fp64": l2r,
+                             "l2_u_vs_fp64_mean": float(np.mean([c[1] for c in curve])),
+                             "l2_u_vs_fp64_final": curve[-1][1], "l2_rho_vs_fp64_final": curve[-1][2],
+                             "l2_u_curve": [[t, round(u, 8)] for t, u, _ in curve],
+                             "l2_u_curve_smagorinsky_fp32_vs_fp64": [[t, round(u, 8)] for t, u, _ in curve_s],
                              "l2_reference": "FP64 form of the same closure on the GPU (pinned to the CPU "
-                                             "oracle by the parity tests), %d steps" % done}
+                                             "oracle by the parity tests), stepped in lockstep from the same "
+                                             "initial state"}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(model)
     print(json.dumps(out), flush=True)

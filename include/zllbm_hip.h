@@ -50,6 +50,9 @@ enum { ZLH_LES_NONE = 0, ZLH_LES_SMAGORINSKY = 1, ZLH_LES_MLP = 2 };
 /* Smagorinsky relaxation time: reference = tau_eff is the SGS increment alone (the shipped
  * behaviour of lbm_solver_interface.cpp:276-295), physical = tau0 + increment */
 enum { ZLH_LES_TAU_REFERENCE = 0, ZLH_LES_TAU_PHYSICAL = 1 };
+/* operands of the FP32 solver's matrix-core closure network (FP32 accumulation either way):
+ * BF16 (8-bit significand) or FP16 (11-bit significand, same rate) */
+enum { ZLH_MLP_OPERANDS_BF16 = 0, ZLH_MLP_OPERANDS_FP16 = 1 };
 
 /* Solver settings: SolverLbmInterface members set by the drivers (lbm_interface.h:288-474)
  * and the per-direction boundary settings (SetDomainBoundaryCondition, SetValues). */
@@ -65,7 +68,7 @@ typedef struct zlh_solver_desc {
     int32_t les_tau_mode;       /* ZLH_LES_TAU_* */
     double cs;                  /* Smagorinsky constant */
     int32_t has_force;          /* bool_forces_ */
-    int32_t pad0;
+    int32_t mlp_operands;       /* ZLH_MLP_OPERANDS_*, FP32 solver with ZLH_LES_MLP */
     double const_force[3];      /* k0ConstForce_ */
     int32_t collision[16];      /* per level ZLH_SRT / ZLH_MRT */
     int32_t bc_scheme[7];       /* per direction 1..6 */
@@ -153,7 +156,7 @@ zlh_status zlh_mlp_train(zlh_ctx *ctx, const float *x, const float *t, int64_t n
                          double lr, uint64_t seed, float *params, float *loss_history);
 /* network forms: FP32 on the vector units (the FP64 solver's collision) and BF16 operands with
  * FP32 accumulation on the matrix cores, 64 nodes per wavefront (the FP32 solver's collision) */
-enum { ZLH_MLP_FP32 = 0, ZLH_MLP_BF16_MFMA = 1 };
+enum { ZLH_MLP_FP32 = 0, ZLH_MLP_BF16_MFMA = 1, ZLH_MLP_FP16_MFMA = 2 };
 /* network output (nu_t / nu_scale) of normalised inputs [n][9] with the parameters set by
  * zlh_mlp_set, in the given form */
 zlh_status zlh_mlp_eval(zlh_ctx *ctx, const float *x, int64_t n, int form, float *y);

@@ -126,12 +126,25 @@ def bf16_round(x):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def forward_bf16(p, X):
-    """The network with BF16 operands and wide accumulation, as the matrix-core form evaluates
-    it: inputs, W1, b1, W2, b2 and the layer-1 activations rounded to BF16 (the biases enter the
-    products as weights of a constant input 1); w3, b3 and the accumulations exact."""
+def f16_round(x):
+    """Round to the nearest FP16 value (ties to even), returned as float64."""
+    return np.asarray(x, dtype=np.float64).astype(np.float16).astype(np.float64)
+
+
+def forward_lowp(p, X, rnd):
+    """The network with low-precision operands and wide accumulation, as the matrix-core form
+    evaluates it: inputs, W1, b1, W2, b2 and the layer-1 activations rounded by `rnd` (the biases
+    enter the products as weights of a constant input 1); w3, b3 and the accumulations exact."""
     W1, b1, W2, b2, w3, b3 = unpack(p)
-    Z1 = bf16_round(X) @ bf16_round(W1).T + bf16_round(b1)
-    H1 = bf16_round(silu(Z1))
-    Z2 = H1 @ bf16_round(W2).T + bf16_round(b2)
+    Z1 = rnd(X) @ rnd(W1).T + rnd(b1)
+    H1 = rnd(silu(Z1))
+    Z2 = H1 @ rnd(W2).T + rnd(b2)
     return softplus(silu(Z2) @ w3 + b3)
+
+
+def forward_bf16(p, X):
+    return forward_lowp(p, X, bf16_round)
+
+
+def forward_f16(p, X):
+    return forward_lowp(p, X, f16_round)

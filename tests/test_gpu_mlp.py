@@ -70,6 +70,25 @@ def test_eval_bf16_mfma_matches_emulation():
     assert np.median(np.abs(y - yref)) < 0.1 * np.median(np.abs(y - yexact))
 
 
+def test_eval_fp16_mfma_matches_emulation():
+    """FP16-operand form: against an FP64 evaluation of the same roundings, and about 8x closer
+    to the exact network than the BF16 form."""
+    s = _ctx()
+    X, _ = _data(1000)
+    p = closure.init_params(9)
+    m = closure.ClosureModel(p, np.zeros(9, np.float32), np.ones(9, np.float32), np.full(9, -1e30, np.float32),
+                             np.full(9, 1e30, np.float32), 1.0)
+    s.mlp_set(m)
+    y16 = s.mlp_eval(X[:993], hip.MLP_FP16_MFMA)
+    yb = s.mlp_eval(X[:993], hip.MLP_BF16_MFMA)
+    yref = mo.forward_f16(p.astype(np.float64), X[:993].astype(np.float64))
+    np.testing.assert_allclose(y16, yref, rtol=1e-3, atol=1e-6)
+    yexact, _ = mo.forward(p.astype(np.float64), X[:993].astype(np.float64))
+    e16 = np.median(np.abs(y16 - yexact) / np.abs(yexact))
+    eb = np.median(np.abs(yb - yexact) / np.abs(yexact))
+    assert e16 < eb / 4, (e16, eb)
+
+
 def test_eval_matches_reference():
     s = _ctx()
     X, T = _data(500)
@@ -112,15 +131,17 @@ def test_fused_closure_matches_oracle(coll):
     assert_close(s.f(0), o.f(0), rtol=2e-7)
 
 
-def test_fp32_learned_closure_tracks_fp64():
-    """FP32 solver (shifted populations, BF16 matrix-core network) against the FP64 oracle with
-    the same network in FP64."""
+@pytest.mark.parametrize("operands", [hip.MLP_OPERANDS_BF16, hip.MLP_OPERANDS_FP16])
+def test_fp32_learned_closure_tracks_fp64(operands):
+    """FP32 solver (shifted populations, matrix-core network) against the FP64 oracle with the
+    same network in FP64."""
     nu = 0.002
     grid, kw = sc.periodic_tgv(3, 16, nu=nu)
     m = _random_model(3 * nu + 0.5)
     o = Oracle(grid, collision=(hip.MRT,), les=2, cs=0.1, **kw)
     o.set_mlp(m.params.astype(np.float64), m.mean, m.std, m.fmin, m.fmax, m.nu_scale)
-    s = hip.Solver(grid, precision=hip.FP32_SHIFTED, collision=(hip.MRT,), les=hip.LES_MLP, cs=0.1, **kw).build()
+    s = hip.Solver(grid, precision=hip.FP32_SHIFTED, collision=(hip.MRT,), les=hip.LES_MLP, cs=0.1,
+                   mlp_operands=operands, **kw).build()
     s.mlp_set(m)
     init = sc.tgv_fields(grid, 0.05, noise=1e-3)
     o.init_macro(0, *init)
@@ -131,7 +152,7 @@ def test_fp32_learned_closure_tracks_fp64():
     rho_s, u_s = s.macros(0)
     l2_u = np.sqrt(np.sum((u_s - u_o) ** 2) / np.sum(u_o ** 2))
     l2_rho = np.sqrt(np.mean((rho_s - rho_o) ** 2))
-    print("FP32 + BF16 closure vs FP64: L2(u) %.2e, L2(rho) %.2e" % (l2_u, l2_rho))
+    print("FP32 + closure (operands %d) vs FP64: L2(u) %.2e, L2(rho) %.2e" % (operands, l2_u, l2_rho))
     assert l2_u < 1e-3, l2_u
     assert l2_rho < 1e-5, l2_rho
 
@@ -205,7 +226,8 @@ def test_learned_closure_beats_smagorinsky_a_priori():
     assert err_mlp < min(err_smag.values())
 
 
-def test_fp32_learned_closure_on_refined_grid():
+@pytest.mark.parametrize("operands", [hip.MLP_OPERANDS_BF16, hip.MLP_OPERANDS_FP16])
+def test_fp32_learned_closure_on_refined_grid(operands):
     """The pipelined FP32 closure kernel (direct-to-LDS pulls, deferred nodes, interfaces) on the
     3-level vorticity-refined grid against the FP64 oracle with the same network."""
     from zllbm_amd import workloads
@@ -214,7 +236,7 @@ def test_fp32_learned_closure_on_refined_grid():
     m = _random_model(3 * nu + 0.5)
     o = Oracle(W.grid, **W.solver)
     o.set_mlp(m.params.astype(np.float64), m.mean, m.std, m.fmin, m.fmax, m.nu_scale)
-    s = hip.Solver(W.grid, precision=hip.FP32_SHIFTED, **W.solver).build()
+    s = hip.Solver(W.grid, precision=hip.FP32_SHIFTED, mlp_operands=operands, **W.solver).build()
     s.mlp_set(m)
     for l, (rho, u) in enumerate(W.init):
         o.init_macro(l, rho, u)
@@ -228,5 +250,5 @@ def test_fp32_learned_closure_on_refined_grid():
         num += np.sum((u_s - u_o) ** 2)
         den += np.sum(u_o ** 2)
     l2 = np.sqrt(num / den)
-    print("FP32 + BF16 closure vs FP64 on 3 levels: L2(u) %.2e" % l2)
+    print("FP32 + closure (operands %d) vs FP64 on 3 levels: L2(u) %.2e" % (operands, l2))
     assert l2 < 1e-4

@@ -36,6 +36,7 @@ struct StepParams {
     int has_force;
     int compressible;
     const MlpDev *mlp;     // learned closure (les == 2)
+    int mlp_operands;      // 0 BF16, 1 FP16 operands of the FP32 matrix-core network
 };
 
 // Populations of a level are brick-major, [brick][q][64 lanes]: the Q x 64 values of one brick
@@ -111,9 +112,10 @@ __device__ __forceinline__ void macro(const C (&f)[Lattice<D>::Q], const C *F, C
 }
 
 // sub-grid model code of a collision variant: 0 none, 1 Smagorinsky with tau0 + tau_t (physical),
-// 2 learned closure, 3 Smagorinsky with the reference's tau_eff = tau_t
-__host__ __device__ constexpr int les_code(int les, int les_tau_mode) {
-    return les == 1 ? (les_tau_mode ? 1 : 3) : les;
+// 2 learned closure (BF16 operands in FP32), 3 Smagorinsky with the reference's tau_eff = tau_t,
+// 4 learned closure (FP16 operands in FP32)
+__host__ __device__ constexpr int les_code(int les, int les_tau_mode, int mlp_operands) {
+    return les == 1 ? (les_tau_mode ? 1 : 3) : (les == 2 ? (mlp_operands ? 4 : 2) : les);
 }
 
 // relaxation time of a node: tau0, or with a sub-grid model (les_code) from the non-equilibrium
@@ -124,15 +126,16 @@ __device__ __forceinline__ C relaxation_time(int les, const C *pi, C rho, C drho
                                              const MlpOps *ops) {
     const C t0 = C(p.tau0);
     if (!les) return t0;
-    if (les != 2) {
+    if (les == 1 || les == 3) {
         const C pi2 = pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2] + C(2) * (pi[3] * pi[3] + pi[4] * pi[4] + pi[5] * pi[5]);
         const C tau_t = C(0.5) * (sqrt(t0 * t0 + C(p.cs18) * sqrt(C(2) * pi2)) - t0);
         return les == 1 ? t0 + tau_t : tau_t;
     }
     if constexpr (std::is_same<C, float>::value) {
-        // BF16 MFMA network across the wavefront (callers keep every lane here)
-        // ops: the operand images, in LDS in the bulk kernel (its address space must stay visible)
-        return t0 + mlp_tau_t_wave(*ops, pi, rho, drho, u, t0, C(p.cs18));
+        // matrix-core network across the wavefront (callers keep every lane here); ops: the
+        // operand images, in LDS in the bulk kernel (its address space must stay visible)
+        if (les == 4) return t0 + mlp_tau_t_wave<OperandsF16>(*ops, pi, rho, drho, u, t0, C(p.cs18));
+        return t0 + mlp_tau_t_wave<OperandsBf16>(*ops, pi, rho, drho, u, t0, C(p.cs18));
     } else {
         bool fb;
         return t0 + mlp_tau_t<C>(*p.mlp, pi, rho, drho, u, t0, C(p.cs18), &fb);
@@ -190,7 +193,7 @@ __device__ __forceinline__ void collide_mrt(C (&f)[Lattice<D>::Q], const C *F, c
     constexpr int Q = L::Q;
     constexpr MomentSpace<D> S{};
     constexpr int NP = S.NP;
-    const int les = VAR >= 0 ? (VAR >> 1) : les_code(p.les, p.les_tau_mode);
+    const int les = VAR >= 0 ? (VAR >> 1) : les_code(p.les, p.les_tau_mode, p.mlp_operands);
     const bool forced = VAR >= 0 ? (VAR & 1) != 0 : p.has_force != 0;
     const C dt = C(p.dt);
     C ps[NP], pd[NP];
@@ -339,7 +342,7 @@ __device__ __forceinline__ void collide(C (&f)[Lattice<D>::Q], const C *F, const
     }
     using L = Lattice<D>;
     constexpr int Q = L::Q;
-    const int les = VAR >= 0 ? (VAR >> 1) : les_code(p.les, p.les_tau_mode);
+    const int les = VAR >= 0 ? (VAR >> 1) : les_code(p.les, p.les_tau_mode, p.mlp_operands);
     const bool forced = VAR >= 0 ? (VAR & 1) != 0 : p.has_force != 0;
     const C dt = C(p.dt);
     C rho, drho;

@@ -17,7 +17,8 @@ constexpr int kMlpParams = kMlpH * kMlpF + kMlpH + kMlpH * kMlpH + kMlpH + kMlpH
 // (v_mfma_f32_32x32x16_bf16: lane l holds C[i][l % 32] for i = 8 (r / 4) + 4 h + r % 4).
 __host__ __device__ constexpr int mfma_row(int h, int r) { return (r >> 2) * 8 + h * 4 + (r & 3); }
 
-// Per-lane operand images of the wavefront MFMA network, BF16 bit patterns (built on the host):
+// Per-lane operand images of the wavefront MFMA network, as BF16 and as FP16 bit patterns (built
+// on the host; w1a/w2a BF16, h1a/h2a the same entries in FP16):
 //   w1a[l][e]    = layer-1 A operand, K slot k = 8 (l / 32) + e of row l % 32: W1[l % 32][k] for
 //                  the 9 features, b1[l % 32] in slot 9 (its B slot holds 1), zero above
 //   w2a[s][l][e] = layer-2 A operand, K step s < 2: W2[l % 32][mfma_row(l / 32, 8 s + e)] (the
@@ -29,6 +30,8 @@ __host__ __device__ constexpr int mfma_row(int h, int r) { return (r >> 2) * 8 +
 struct alignas(16) MlpOps {
     uint16_t w1a[64][8];
     uint16_t w2a[3][64][8];
+    uint16_t h1a[64][8];
+    uint16_t h2a[3][64][8];
     float w3r[2][16];
     float mean[kMlpF];
     float inv_std[kMlpF];
@@ -124,22 +127,46 @@ __device__ __forceinline__ C mlp_tau_t(const MlpDev &m, const C *pi, C rho, C dr
 }
 
 // ---------------------------------------------------------------------------------------------
-// FP32 solver: the network of 64 nodes (one per lane) as BF16 MFMA products with FP32
-// accumulation.  Layer l computes H_l^T [32 x 64 nodes] = W_l [32 x K] * H_{l-1}^T in two
-// 32-node column blocks; every lane of the wavefront must call it (EXEC all ones).
+// FP32 solver: the network of 64 nodes (one per lane) as matrix-core products with BF16 or FP16
+// operands and FP32 accumulation.  Layer l computes H_l^T [32 x 64 nodes] = W_l [32 x K] *
+// H_{l-1}^T in two 32-node column blocks; every lane of the wavefront must call it (EXEC all
+// ones).  v_mfma_f32_32x32x16_bf16 and _f16 share one lane layout.
 // ---------------------------------------------------------------------------------------------
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
+// sub-grid codes of the learned closure: 2 with BF16 operands, 4 with FP16 (VAR = 2 code + forced)
 template <typename C, int VAR>
 __host__ __device__ constexpr bool mlp_on_mfma() {
-    return std::is_same<C, float>::value && (VAR < 0 || (VAR >> 1) == 2);   // VAR = 2 * les_code + forced
+    return std::is_same<C, float>::value && (VAR < 0 || (VAR >> 1) == 2 || (VAR >> 1) == 4);
 }
 
-__device__ __forceinline__ bf16x8_t load_bf16x8(const uint16_t *p) {
-    return __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u16x8_t *>(p));
-}
+struct OperandsBf16 {
+    typedef bf16x8_t vec;
+    typedef __bf16 elem;
+    static __device__ __forceinline__ vec load(const MlpOps &o, const uint16_t *bits) {
+        return __builtin_bit_cast(vec, *reinterpret_cast<const u16x8_t *>(bits));
+    }
+    static __device__ __forceinline__ const uint16_t *w1(const MlpOps &o, int lane) { return o.w1a[lane]; }
+    static __device__ __forceinline__ const uint16_t *w2(const MlpOps &o, int s, int lane) { return o.w2a[s][lane]; }
+    static __device__ __forceinline__ f32x16_t mfma(vec a, vec b, f32x16_t c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+struct OperandsF16 {
+    typedef f16x8_t vec;
+    typedef _Float16 elem;
+    static __device__ __forceinline__ vec load(const MlpOps &o, const uint16_t *bits) {
+        return __builtin_bit_cast(vec, *reinterpret_cast<const u16x8_t *>(bits));
+    }
+    static __device__ __forceinline__ const uint16_t *w1(const MlpOps &o, int lane) { return o.h1a[lane]; }
+    static __device__ __forceinline__ const uint16_t *w2(const MlpOps &o, int s, int lane) { return o.h2a[s][lane]; }
+    static __device__ __forceinline__ f32x16_t mfma(vec a, vec b, f32x16_t c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
 
 // SiLU for BF16 activations: hardware exp2 and reciprocal (about 1 ulp each), far below BF16's
 // rounding; large negative z gives exp2 -> inf, rcp -> 0
@@ -155,7 +182,10 @@ __device__ __forceinline__ float swap_halves(float v) {
 }
 
 // network output (nu_t / nu_scale) of this lane's normalised inputs; o may live in LDS
+template <typename OT>
 __device__ __forceinline__ float mlp_forward_wave(const MlpOps &o, const float *xn, float b3) {
+    typedef typename OT::vec vec;
+    typedef typename OT::elem elem;
     const int lane = __lane_id();
     const int h = lane >> 5;
     float px[kMlpF];
@@ -163,38 +193,38 @@ __device__ __forceinline__ float mlp_forward_wave(const MlpOps &o, const float *
     for (int k = 0; k < kMlpF; ++k) px[k] = swap_halves(xn[k]);
     // layer-1 B operands: column block 0 = nodes 0..31, block 1 = nodes 32..63; lane l carries
     // K slots 8 h .. 8 h + 7 of node l % 32 (block 0) or 32 + l % 32 (block 1); slot 9 = 1 (bias)
-    bf16x8_t x0, x1;
+    vec x0, x1;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         const float hi = e == 1 ? 1.f : 0.f;
         const float v0 = e == 0 ? (h ? px[8] : xn[0]) : (h ? hi : xn[e]);
         const float v1 = e == 0 ? (h ? xn[8] : px[0]) : (h ? hi : px[e]);
-        x0[e] = (__bf16)v0;
-        x1[e] = (__bf16)v1;
+        x0[e] = (elem)v0;
+        x1[e] = (elem)v1;
     }
     const f32x16_t zero = {};
     f32x16_t z0, z1;
     {
-        const bf16x8_t a1 = load_bf16x8(o.w1a[lane]);
-        z0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, x0, zero, 0, 0, 0);
-        z1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, x1, zero, 0, 0, 0);
+        const vec a1 = OT::load(o, OT::w1(o, lane));
+        z0 = OT::mfma(a1, x0, zero);
+        z1 = OT::mfma(a1, x1, zero);
     }
     // SiLU; accumulator r of this lane is hidden unit mfma_row(h, r), which is also the K slot
     // 8 h + r % 8 of layer-2 K step r / 8, so the activations stay in place
-    bf16x8_t g0[3], g1[3];
+    vec g0[3], g1[3];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        g0[r >> 3][r & 7] = (__bf16)siluf_(z0[r]);
-        g1[r >> 3][r & 7] = (__bf16)siluf_(z1[r]);
+        g0[r >> 3][r & 7] = (elem)siluf_(z0[r]);
+        g1[r >> 3][r & 7] = (elem)siluf_(z1[r]);
     }
 #pragma unroll
-    for (int e = 0; e < 8; ++e) g0[2][e] = g1[2][e] = (__bf16)(h == 0 && e == 0 ? 1.f : 0.f);
+    for (int e = 0; e < 8; ++e) g0[2][e] = g1[2][e] = (elem)(h == 0 && e == 0 ? 1.f : 0.f);
     f32x16_t y0 = zero, y1 = zero;
 #pragma unroll
     for (int st = 0; st < 3; ++st) {
-        const bf16x8_t a2 = load_bf16x8(o.w2a[st][lane]);
-        y0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, g0[st], y0, 0, 0, 0);
-        y1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, g1[st], y1, 0, 0, 0);
+        const vec a2 = OT::load(o, OT::w2(o, st, lane));
+        y0 = OT::mfma(a2, g0[st], y0);
+        y1 = OT::mfma(a2, g1[st], y1);
     }
     float p0 = 0.f, p1 = 0.f;
 #pragma unroll
@@ -209,6 +239,7 @@ __device__ __forceinline__ float mlp_forward_wave(const MlpOps &o, const float *
 }
 
 // mlp_tau_t for the FP32 solver, evaluated by the whole wavefront
+template <typename OT>
 __device__ __forceinline__ float mlp_tau_t_wave(const MlpOps &o, const float *pi, float rho, float drho,
                                                 const float *u, float tau0, float cs18) {
     float x[kMlpF], xn[kMlpF];
@@ -219,7 +250,7 @@ __device__ __forceinline__ float mlp_tau_t_wave(const MlpOps &o, const float *pi
         inside = inside && x[i] >= o.fmin[i] && x[i] <= o.fmax[i];
         xn[i] = (x[i] - o.mean[i]) * o.inv_std[i];
     }
-    const float net = mlp_forward_wave(o, xn, o.b3);
+    const float net = mlp_forward_wave<OT>(o, xn, o.b3);
     const float q = sqrtf(2.f * (pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2] +
                                  2.f * (pi[3] * pi[3] + pi[4] * pi[4] + pi[5] * pi[5])));
     const float smag = 0.5f * (sqrtf(tau0 * tau0 + cs18 * q) - tau0);
@@ -237,13 +268,14 @@ __device__ __forceinline__ void stage_mlp_ops(MlpOps &dst, const MlpOps &src) {
 }
 
 // batch inference on the MFMA network (FP32 solver form); every lane runs, inactive ones on zeros
+template <typename OT>
 __global__ __launch_bounds__(256) void k_mlp_eval_wave(const MlpDev *m, const float *X, int64_t n, float *y) {
     __shared__ MlpOps ops;
     stage_mlp_ops(ops, m->ops);
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     float xn[kMlpF];
     for (int i = 0; i < kMlpF; ++i) xn[i] = k < n ? X[k * kMlpF + i] : 0.f;
-    const float out = mlp_forward_wave(ops, xn, ops.b3);
+    const float out = mlp_forward_wave<OT>(ops, xn, ops.b3);
     if (k < n) y[k] = out;
 }
 

@@ -425,6 +425,31 @@ zlh_status upload(zlh_ctx *ctx, DevBuf &b, const std::vector<X> &v) {
 inline unsigned blocks(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
 // FP32 -> BF16 bit pattern, round to nearest even (NaN stays NaN)
+// FP32 -> FP16 bit pattern, round to nearest even, with subnormals, overflow to infinity
+inline uint16_t f16_bits(float v) {
+    uint32_t x;
+    std::memcpy(&x, &v, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const int32_t ef = (int32_t)((x >> 23) & 0xffu);
+    uint32_t m = x & 0x7fffffu;
+    if (ef == 0xff) return (uint16_t)(sign | 0x7c00u | (m ? 0x200u : 0u));
+    const int32_t e = ef - 127 + 15;
+    if (e >= 31) return (uint16_t)(sign | 0x7c00u);
+    if (e <= 0) {
+        if (e < -10) return (uint16_t)sign;
+        m |= 0x800000u;
+        const int shift = 14 - e;
+        uint32_t hm = m >> shift;
+        const uint32_t rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1);
+        if (rem > half || (rem == half && (hm & 1u))) ++hm;
+        return (uint16_t)(sign | hm);
+    }
+    uint32_t out = sign | ((uint32_t)e << 10) | (m >> 13);
+    const uint32_t rem = m & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (out & 1u))) ++out;     // a carry rounds into the exponent
+    return (uint16_t)out;
+}
+
 inline uint16_t bf16_bits(float v) {
     uint32_t u;
     std::memcpy(&u, &v, 4);
@@ -441,6 +466,7 @@ StepParams level_params(const zlh_ctx *ctx, int level) {
     p.cs18 = 18. * ctx->d.cs * ctx->d.cs;
     for (int a = 0; a < 3; ++a) p.const_force[a] = ctx->d.has_force ? ctx->d.const_force[a] : 0.;
     p.les = ctx->d.les == ZLH_LES_SMAGORINSKY ? 1 : (ctx->d.les == ZLH_LES_MLP ? 2 : 0);
+    p.mlp_operands = ctx->d.mlp_operands;
     p.mlp = ctx->mlp_d.as<MlpDev>();
     p.les_tau_mode = ctx->d.les_tau_mode;
     p.has_force = ctx->d.has_force;
@@ -521,7 +547,7 @@ void launch_stream_collide_v(zlh_ctx *ctx, int var, unsigned grid, const BrickLe
         }                                                                                                     \
         break;
     switch (var) {
-        ZLH_SC(0) ZLH_SC(1) ZLH_SC(2) ZLH_SC(3) ZLH_SC(4) ZLH_SC(5) ZLH_SC(6) ZLH_SC(7)
+        ZLH_SC(0) ZLH_SC(1) ZLH_SC(2) ZLH_SC(3) ZLH_SC(4) ZLH_SC(5) ZLH_SC(6) ZLH_SC(7) ZLH_SC(8) ZLH_SC(9)
         default: break;
     }
 #undef ZLH_SC
@@ -536,7 +562,7 @@ void launch_aux_collide_v(zlh_ctx *ctx, int var, LevelState &L, const BrickLevel
                            ctx->stream, v, aux_f, aux_force, fa, L.aux_macro_d.as<double>(), L.sp);           \
         break;
     switch (var) {
-        ZLH_AC(0) ZLH_AC(1) ZLH_AC(2) ZLH_AC(3) ZLH_AC(4) ZLH_AC(5) ZLH_AC(6) ZLH_AC(7)
+        ZLH_AC(0) ZLH_AC(1) ZLH_AC(2) ZLH_AC(3) ZLH_AC(4) ZLH_AC(5) ZLH_AC(6) ZLH_AC(7) ZLH_AC(8) ZLH_AC(9)
         default: break;
     }
 #undef ZLH_AC
@@ -556,7 +582,7 @@ zlh_status launch_step_kernels(zlh_ctx *ctx, int level, int coll) {
     T *fa = L.f_d[L.cur].as<T>(), *fb = L.f_d[L.cur ^ 1].as<T>();
     T *aux_f = L.aux_f_d.as<T>();
     const double *aux_force = L.aux_force_d.as<double>();
-    const int var = 2 * les_code(L.sp.les, L.sp.les_tau_mode) + (L.sp.has_force ? 1 : 0);
+    const int var = 2 * les_code(L.sp.les, L.sp.les_tau_mode, L.sp.mlp_operands) + (L.sp.has_force ? 1 : 0);
     if (L.naux > 0) {
         if (coll == ZLH_SRT) launch_aux_collide_v<D, T, C, SHIFT, 0>(ctx, var, L, v, aux_f, aux_force, fa);
         else launch_aux_collide_v<D, T, C, SHIFT, 1>(ctx, var, L, v, aux_f, aux_force, fa);
@@ -843,6 +869,7 @@ zlh_status zlh_create(const zlh_solver_desc *desc, zlh_ctx **out) {
     for (int l = 0; l <= desc->max_level; ++l)
         if (desc->collision[l] != ZLH_SRT && desc->collision[l] != ZLH_MRT) return ZLH_EINVAL;
     if (desc->les != ZLH_LES_NONE && desc->les != ZLH_LES_SMAGORINSKY && desc->les != ZLH_LES_MLP) return ZLH_EINVAL;
+    if (desc->mlp_operands != ZLH_MLP_OPERANDS_BF16 && desc->mlp_operands != ZLH_MLP_OPERANDS_FP16) return ZLH_EINVAL;
     for (int dir = 1; dir <= 2 * desc->dims; ++dir)
         if (desc->bc_scheme[dir] < ZLH_BC_BOUNCE_BACK || desc->bc_scheme[dir] > ZLH_BC_NEE) return ZLH_EINVAL;
     for (int a = 0; a < desc->dims; ++a)
@@ -1268,9 +1295,16 @@ zlh_status zlh_mlp_set(zlh_ctx *ctx, const float *params, const float *mean, con
         const int i = l % 32, h = l / 32;
         for (int e = 0; e < 8; ++e) {
             const int k = 8 * h + e;
-            o.w1a[l][e] = bf16_bits(k < kMlpF ? W1[i * kMlpF + k] : (k == kMlpF ? b1[i] : 0.f));
-            for (int st = 0; st < 2; ++st) o.w2a[st][l][e] = bf16_bits(W2[i * kMlpH + mfma_row(h, 8 * st + e)]);
+            const float a1 = k < kMlpF ? W1[i * kMlpF + k] : (k == kMlpF ? b1[i] : 0.f);
+            o.w1a[l][e] = bf16_bits(a1);
+            o.h1a[l][e] = f16_bits(a1);
+            for (int st = 0; st < 2; ++st) {
+                const float a2 = W2[i * kMlpH + mfma_row(h, 8 * st + e)];
+                o.w2a[st][l][e] = bf16_bits(a2);
+                o.h2a[st][l][e] = f16_bits(a2);
+            }
             o.w2a[2][l][e] = bf16_bits(k == 0 ? b2[i] : 0.f);
+            o.h2a[2][l][e] = f16_bits(k == 0 ? b2[i] : 0.f);
         }
     }
     for (int h = 0; h < 2; ++h)
@@ -1352,7 +1386,7 @@ zlh_status zlh_mlp_train(zlh_ctx *ctx, const float *x, const float *t, int64_t n
 zlh_status zlh_mlp_eval(zlh_ctx *ctx, const float *x, int64_t n, int form, float *y) {
     if (!ctx) return ZLH_EINVAL;
     if (!ctx->mlp_set) return ctx->fail(ZLH_ESTATE, "learned closure not set");
-    if (!x || !y || n < 1 || (form != ZLH_MLP_FP32 && form != ZLH_MLP_BF16_MFMA))
+    if (!x || !y || n < 1 || form < ZLH_MLP_FP32 || form > ZLH_MLP_FP16_MFMA)
         return ctx->fail(ZLH_EINVAL, "bad evaluation arguments");
     DevBuf dx, dy;
     zlh_status s;
@@ -1360,8 +1394,11 @@ zlh_status zlh_mlp_eval(zlh_ctx *ctx, const float *x, int64_t n, int form, float
     if ((s = dalloc(ctx, dy, sizeof(float) * n, false))) return s;
     ZLH_TRY(hipMemcpyAsync(dx.p, x, sizeof(float) * kMlpF * n, hipMemcpyHostToDevice, ctx->stream));
     if (form == ZLH_MLP_BF16_MFMA)
-        hipLaunchKernelGGL(k_mlp_eval_wave, dim3(blocks(n)), dim3(256), 0, ctx->stream, ctx->mlp_d.as<MlpDev>(),
-                           dx.as<float>(), n, dy.as<float>());
+        hipLaunchKernelGGL(k_mlp_eval_wave<OperandsBf16>, dim3(blocks(n)), dim3(256), 0, ctx->stream,
+                           ctx->mlp_d.as<MlpDev>(), dx.as<float>(), n, dy.as<float>());
+    else if (form == ZLH_MLP_FP16_MFMA)
+        hipLaunchKernelGGL(k_mlp_eval_wave<OperandsF16>, dim3(blocks(n)), dim3(256), 0, ctx->stream,
+                           ctx->mlp_d.as<MlpDev>(), dx.as<float>(), n, dy.as<float>());
     else
         hipLaunchKernelGGL(k_mlp_eval, dim3(blocks(n)), dim3(256), 0, ctx->stream, ctx->mlp_d.as<MlpDev>(),
                            dx.as<float>(), n, dy.as<float>());

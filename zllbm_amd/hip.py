@@ -21,7 +21,8 @@ FP64, FP32_SHIFTED = 0, 1
 SRT, MRT = 0, 1
 LES_NONE, LES_SMAGORINSKY, LES_MLP = 0, 1, 2
 LES_TAU_REFERENCE, LES_TAU_PHYSICAL = 0, 1
-MLP_FP32, MLP_BF16_MFMA = 0, 1
+MLP_FP32, MLP_BF16_MFMA, MLP_FP16_MFMA = 0, 1, 2
+MLP_OPERANDS_BF16, MLP_OPERANDS_FP16 = 0, 1
 
 
 class SolverDesc(C.Structure):
@@ -29,7 +30,7 @@ class SolverDesc(C.Structure):
         ("dims", C.c_int32), ("max_level", C.c_int32), ("precision", C.c_int32),
         ("compressible", C.c_int32), ("nu0", C.c_double), ("rho0", C.c_double),
         ("u0", C.c_double * 3), ("les", C.c_int32), ("les_tau_mode", C.c_int32),
-        ("cs", C.c_double), ("has_force", C.c_int32), ("pad0", C.c_int32),
+        ("cs", C.c_double), ("has_force", C.c_int32), ("mlp_operands", C.c_int32),
         ("const_force", C.c_double * 3), ("collision", C.c_int32 * 16),
         ("bc_scheme", C.c_int32 * 7), ("pad1", C.c_int32), ("bc_value", (C.c_double * 3) * 7),
         ("dom_min", C.c_uint32 * 3), ("dom_max", C.c_uint32 * 3),
@@ -95,7 +96,7 @@ def make_desc(grid: "_grid.Grid", *, nu0: float, precision: int = FP64, rho0: fl
               u0=(0., 0., 0.), collision: Sequence[int] = (SRT,), les: int = LES_NONE,
               les_tau_mode: int = LES_TAU_PHYSICAL, cs: float = 0.1, has_force: bool = False,
               const_force=(0., 0., 0.), bc: Optional[Dict[int, tuple]] = None,
-              compressible: bool = True) -> SolverDesc:
+              compressible: bool = True, mlp_operands: int = MLP_OPERANDS_BF16) -> SolverDesc:
     d = SolverDesc()
     d.dims = grid.dims
     d.max_level = grid.max_level
@@ -107,6 +108,7 @@ def make_desc(grid: "_grid.Grid", *, nu0: float, precision: int = FP64, rho0: fl
         d.const_force[a] = const_force[a] if a < len(const_force) else 0.
     d.les, d.les_tau_mode, d.cs = les, les_tau_mode, cs
     d.has_force = 1 if has_force else 0
+    d.mlp_operands = mlp_operands
     for l in range(16):
         d.collision[l] = collision[min(l, len(collision) - 1)]
     for dr, (scheme, val) in (bc or {}).items():
