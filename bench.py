@@ -192,6 +192,10 @@ def main():
         del s64
         mlups64 = W.updates_per_step * c_steps / wall64 / 1e6
         log("FP64 Smagorinsky: %.3f ms per step" % (wall64 / c_steps * 1e3))
+        s32 = make_solver(hip, W, hip.FP32_SHIFTED, hip.LES_SMAGORINSKY)
+        wall32, _, _, _ = timed(s32, W, 5, c_steps, kernel_steps=0)
+        del s32
+        log("FP32 Smagorinsky: %.3f ms per step" % (wall32 / c_steps * 1e3))
         # accuracy: FP32 against FP64, same closure, in lockstep (the learned closure, and for
         # reference the plain Smagorinsky pair, whose difference is FP32 rounding alone)
         curve = l2_curve(hip, W, hip.LES_MLP, model, args.l2_steps, args.l2_every)
@@ -201,6 +205,7 @@ def main():
         out["comparison"] = {"fp64_smagorinsky_mlups": round(mlups64, 2),
                              "fp64_smagorinsky_ms_per_step": round(wall64 / c_steps * 1e3, 4),
                              "speedup_fp32_learned_vs_fp64_smagorinsky": round(mlups / mlups64, 3),
+                             "fp32_smagorinsky_mlups": round(W.updates_per_step * c_steps / wall32 / 1e6, 2),
                              "l2_u_vs_fp64_mean": float(np.mean([c[1] for c in curve])),
                              "l2_u_vs_fp64_final": curve[-1][1], "l2_rho_vs_fp64_final": curve[-1][2],
                              "l2_u_curve": [[t, round(u, 8)] for t, u, _ in curve],
