@@ -130,6 +130,12 @@ zlh_status zlh_ib_download_forces(zlh_ctx *ctx, int level, double *force);
 zlh_status zlh_conservation(zlh_ctx *ctx, int level, double *out4);
 /* node counts: total nodes of a level, and how many take the deferred (aux) path */
 zlh_status zlh_level_info(zlh_ctx *ctx, int level, int64_t *n_nodes, int64_t *n_aux, int64_t *n_bricks);
+/* interpolation tables of a refined level (no reference counterpart): out[ZLH_XFER_INFO_N] =
+ * {ghost nodes, coarse source rows, fine-to-coarse nodes, gather workgroups, rows staged by all
+ * workgroups, contributors, contributors taken from a fine fallback node, LDS bytes of the largest
+ * workgroup}; zeros for level 0 */
+#define ZLH_XFER_INFO_N 8
+zlh_status zlh_transfer_info(zlh_ctx *ctx, int level, int64_t *out);
 zlh_status zlh_synchronize(zlh_ctx *ctx);
 /* device time of the fused stream-collide kernel: enable recording, then read the total in
  * milliseconds and the number of launches since enabling, of one level or of all (level -1)

@@ -31,100 +31,177 @@ __device__ __forceinline__ void macro_feq_double(const double *f, int compressib
 }
 
 // NodeInfoCoarse2fine (lbm_grid_interface.cpp:570-588) + PostStreamCoarse2Fine
-// (lbm_collision_models.cpp:244-251) for every coarse node that feeds an interpolation stencil.
-// The rescaled populations are stored node-major, [n_src][Q], in the populations' storage form
-// (FP64, or FP32 shifted by w), so that each stencil contributor is one contiguous read.
+// (lbm_collision_models.cpp:244-251) for every coarse node that feeds an interpolation stencil,
+// and a plain copy of every fine node that stands in for a missing coarse one.  The rows are
+// stored node-major, [n_src + n_fb][Q], in the populations' storage form (FP64, or FP32 shifted by
+// w), so that each stencil contributor is one contiguous row; both row sets are sorted by aux
+// index, so consecutive threads read neighbouring entries of each direction's plane.
 template <int D, typename T, bool SHIFT>
-__global__ void k_c2f_sources(int64_t n_src, const int32_t *src_ai, const T *aux_f, int64_t naux, double tau_ratio,
+__global__ void k_c2f_sources(int64_t n_src, int64_t n_fb, const int32_t *src_ai, const T *aux_c, int64_t naux_c,
+                              const int32_t *fb_ai, const T *aux_f, int64_t naux_f, double tau_ratio,
                               int compressible, T *src_f) {
     constexpr int Q = Lattice<D>::Q;
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n_src) return;
-    const int64_t j = src_ai[r];
-    double f[Q], feq[Q], rho, u[3];
-    for (int q = 0; q < Q; ++q) f[q] = double(aux_f[q * naux + j]) + (SHIFT ? Lattice<D>::w[q] : 0.);
-    macro_feq_double<D>(f, compressible, rho, u, feq);
-    for (int q = 0; q < Q; ++q)
-        src_f[r * Q + q] = T(feq[q] + tau_ratio * (f[q] - feq[q]) - (SHIFT ? Lattice<D>::w[q] : 0.));
+    if (r < n_src) {
+        const int64_t j = src_ai[r];
+        double f[Q], feq[Q], rho, u[3];
+        for (int q = 0; q < Q; ++q) f[q] = double(aux_c[q * naux_c + j]) + (SHIFT ? Lattice<D>::w[q] : 0.);
+        macro_feq_double<D>(f, compressible, rho, u, feq);
+        for (int q = 0; q < Q; ++q)
+            src_f[r * Q + q] = T(feq[q] + tau_ratio * (f[q] - feq[q]) - (SHIFT ? Lattice<D>::w[q] : 0.));
+    } else if (r < n_src + n_fb) {
+        const int64_t fi = fb_ai[r - n_src];
+        T v[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) v[q] = aux_f[q * naux_f + fi];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) src_f[r * Q + q] = v[q];
+    }
 }
 
 // InterpolationLagrangian{2D,3D} (grid/grid_interpolation.cpp:224-304, 433-525): each ghost node
-// is the weighted sum of its contributors, accumulated in stencil order.  A workgroup takes a
-// group of consecutive (Morton-ordered, so neighbouring) ghost nodes: it first copies the union
-// of their coarse source rows into LDS with coalesced loads (neighbours share most of their
-// 4^d stencils), then each ghost node is summed from LDS by four threads of 5 (D2Q9: 5 and 4)
-// directions.  A contributor is one 8-byte word {LDS row or fine fallback, FP32 weight}: the
-// tensor-product Lagrange weights are dyadic rationals, exact in FP32.
+// is the weighted sum of its contributors, accumulated in stencil order.  Ghost nodes are listed
+// by contributor count (the parity of the node's coordinates fixes it: 4 Lagrange weights per odd
+// coordinate, 1 per even one), stably, so the threads of a wavefront loop equally often; inside
+// one count they keep Morton order, so the nodes of a workgroup are neighbours.  A
+// workgroup takes a group of at most 256 / PARTS ghost nodes, as many as keep its staged data
+// within the LDS budget: it first copies the union of their rows of src_f and their contributor
+// words into LDS, then each ghost node is summed from LDS by PARTS threads of 5 directions (D2Q9:
+// 5 and 4).  A contributor is one 32-bit word {LDS row offset, FP16 weight}: the tensor-product
+// Lagrange weights are products of -1/16, 9/16, 1/2 and 1, dyadic rationals exact in FP16 (the
+// host checks every one).
 constexpr int kGatherDirs = 5;
+constexpr size_t kGatherLdsBudget = 24 * 1024;
+constexpr int kGatherMaxRows = 768;   // staged rows of one group (entry table in static LDS)
+constexpr int64_t kGatherChunk = 2048;   // ghost nodes sorted by contributor count together
 template <int D, typename T, bool SHIFT>
-__global__ __launch_bounds__(256) void k_c2f_gather(int64_t n_tgt, int group, int max_rows,
+__global__ __launch_bounds__(256) void k_c2f_gather(int64_t n_groups, const GatherGroup *__restrict__ groups,
                                                     const int32_t *__restrict__ tgt_ai,
                                                     const int32_t *__restrict__ begin,
-                                                    const int2 *__restrict__ contrib,
-                                                    const int32_t *__restrict__ rows_begin,
+                                                    const uint32_t *__restrict__ contrib,
                                                     const int32_t *__restrict__ rows, const T *__restrict__ src_f,
                                                     T *__restrict__ aux_f, int64_t naux) {
     constexpr int Q = Lattice<D>::Q;
     constexpr int NQ = kGatherDirs;
     constexpr int PARTS = (Q + NQ - 1) / NQ;
+    constexpr int QP = PARTS * NQ;   // LDS row stride: every thread reads NQ entries, pads included
     extern __shared__ unsigned char lds_raw[];
-    T *rowbuf = reinterpret_cast<T *>(lds_raw);                                  // [max_rows][Q]
-    int2 *cbuf = reinterpret_cast<int2 *>(lds_raw + (((size_t)max_rows * Q * sizeof(T) + 15) & ~(size_t)15));
-    const int64_t gi = blockIdx.x;
-    const int64_t t_first = gi * group;
-    const int64_t t_end = t_first + group < n_tgt ? t_first + group : n_tgt;
-    const int32_t c0 = begin[t_first], nc = begin[t_end] - c0;
-    const int32_t r0 = rows_begin[gi], nr = rows_begin[gi + 1] - r0;
-    // staging: every load of a thread is issued before its first LDS write (a load followed by
-    // its store, repeated, would wait once per element)
-    for (int rb = 0; rb < nr; rb += 256) {
-        const int r = rb + (int)threadIdx.x;
-        if (r < nr) {
-            const T *row = src_f + (int64_t)rows[r0 + r] * Q;
-            T v[Q];
-#pragma unroll
-            for (int q = 0; q < Q; ++q) v[q] = row[q];
-#pragma unroll
-            for (int q = 0; q < Q; ++q) rowbuf[r * Q + q] = v[q];
-        }
+    const int64_t gi = xcd_group(n_groups);   // groups are in spatial order: an XCD takes a range
+    if (gi < 0) return;
+    const GatherGroup G = groups[gi];
+    const int32_t nr = G.r1 - G.r0, nc = G.c1 - G.c0;
+    T *rowbuf = reinterpret_cast<T *>(lds_raw);                                  // [nr][QP]
+    uint32_t *cbuf = reinterpret_cast<uint32_t *>(lds_raw + (((size_t)nr * QP * sizeof(T) + 15) & ~(size_t)15));
+    const int local = threadIdx.x / PARTS;
+    const int q0 = (threadIdx.x - local * PARTS) * NQ;
+    const int32_t t = G.t0 + local;
+    const bool active = t < G.t1;
+    // every load below is issued before the first LDS write that waits on it; none depends on
+    // an earlier one except a row's data on its entry
+    int32_t kb = 0, ke = 0, ai = 0;
+    if (active) {
+        kb = begin[t] - G.c0;
+        ke = begin[t + 1] - G.c0;
+        ai = tgt_ai[t];
     }
     for (int cb = 0; cb < nc; cb += 4 * 256) {
-        int2 w[4];
+        uint32_t w[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int e = cb + j * 256 + (int)threadIdx.x;
-            w[j] = contrib[c0 + (e < nc ? e : nc - 1)];
+            const int k = cb + j * 256 + (int)threadIdx.x;
+            w[j] = k < nc ? contrib[G.c0 + k] : 0u;
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int e = cb + j * 256 + (int)threadIdx.x;
-            if (e < nc) cbuf[e] = w[j];
+            const int k = cb + j * 256 + (int)threadIdx.x;
+            if (k < nc) cbuf[k] = w[j];
+        }
+    }
+    // the rows as one flat run of nr * Q elements, consecutive threads on consecutive elements,
+    // so a wavefront's load covers a few cache lines of src_f instead of 64
+    constexpr int kBatch = 16;
+    const int ne = nr * Q;
+    for (int eb = 0; eb < ne; eb += kBatch * 256) {
+        int32_t g[kBatch];
+        T v[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const int e = eb + j * 256 + (int)threadIdx.x;
+            g[j] = rows[G.r0 + (e < ne ? e : ne - 1) / Q];
+        }
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const int e = eb + j * 256 + (int)threadIdx.x;
+            const int ec = e < ne ? e : ne - 1;
+            v[j] = src_f[(int64_t)g[j] * Q + (ec - (ec / Q) * Q)];
+        }
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const int e = eb + j * 256 + (int)threadIdx.x;
+            if (e < ne) {
+                const int r = e / Q;
+                rowbuf[r * QP + (e - r * Q)] = v[j];
+            }
         }
     }
     __syncthreads();
-    const int local = threadIdx.x / PARTS;
-    const int q0 = (threadIdx.x - local * PARTS) * NQ;
-    const int64_t t = t_first + local;
-    if (local >= group || t >= n_tgt) return;
+    if (!active) return;
+    // a contributor word is {row * QP (low 16 bits), FP16 weight (high 16 bits)}; the pad entry
+    // of the last part is summed like the others and never stored
+    auto weight = [](uint32_t cw) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(cw >> 16)); };
     double acc[NQ];
 #pragma unroll
     for (int i = 0; i < NQ; ++i) acc[i] = 0.;
-    const int32_t k1 = begin[t + 1] - c0;
-    for (int32_t k = begin[t] - c0; k < k1; ++k) {
-        const int2 cw = cbuf[k];
-        const double c = (double)__int_as_float(cw.y);
+    if constexpr (SHIFT) {
+        // FP32 storage holds f - w.  sum c (f~ + w) - w = sum c f~ + w (S - 1) with S = sum c: the
+        // weights are multiples of 2^-12 of magnitude <= 1, so S is exact in FP32.  Runs of four
+        // products are summed in FP32 (rounding error a few 2^-24 of the run), the runs in FP64.
+        float S = 0.f;
+        int32_t k = kb;
+        for (; k + 4 <= ke; k += 4) {
+            float c[4];
+            const float *row[4];
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) {
-            if (q0 + i >= Q) break;
-            const int q = q0 + i;
-            const double v = cw.x >= 0 ? double(rowbuf[cw.x * Q + q]) : double(aux_f[q * naux + (-(int64_t)cw.x - 1)]);
-            acc[i] = acc[i] + c * (v + (SHIFT ? Lattice<D>::w[q] : 0.));
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t cw = cbuf[k + j];
+                c[j] = weight(cw);
+                row[j] = reinterpret_cast<const float *>(rowbuf) + (cw & 0xffffu) + q0;
+            }
+            S += (c[0] + c[1]) + (c[2] + c[3]);
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                float run = c[0] * row[0][i];
+                run = __builtin_fmaf(c[1], row[1][i], run);
+                run = __builtin_fmaf(c[2], row[2][i], run);
+                run = __builtin_fmaf(c[3], row[3][i], run);
+                acc[i] += (double)run;
+            }
         }
-    }
-    const int64_t ai = tgt_ai[t];
+        for (; k < ke; ++k) {
+            const uint32_t cw = cbuf[k];
+            const float c = weight(cw);
+            const float *row = reinterpret_cast<const float *>(rowbuf) + (cw & 0xffffu) + q0;
+            S += c;
 #pragma unroll
-    for (int i = 0; i < NQ; ++i)
-        if (q0 + i < Q) aux_f[(q0 + i) * naux + ai] = T(acc[i] - (SHIFT ? Lattice<D>::w[q0 + i] : 0.));
+            for (int i = 0; i < NQ; ++i) acc[i] += (double)(c * row[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < NQ; ++i)
+            if (q0 + i < Q) aux_f[(q0 + i) * naux + ai] = T(acc[i] + Lattice<D>::w[q0 + i] * ((double)S - 1.));
+    } else {
+        // FP64 storage: every product and sum in FP64, in stencil order
+#pragma unroll 4
+        for (int32_t k = kb; k < ke; ++k) {
+            const uint32_t cw = cbuf[k];
+            const double c = (double)weight(cw);
+            const T *row = rowbuf + (cw & 0xffffu) + q0;
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) acc[i] = acc[i] + c * double(row[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < NQ; ++i)
+            if (q0 + i < Q) aux_f[(q0 + i) * naux + ai] = T(acc[i]);
+    }
 }
 
 // NodeInfoFine2Coarse (lbm_grid_interface.cpp:594-610) + PostStreamFine2Coarse
@@ -421,25 +498,54 @@ inline zlh_status plan_ib(zlh_ctx *ctx, int level) {
 // ----------------------------------------------------------------------------------------------
 // device tables
 // ----------------------------------------------------------------------------------------------
+// IEEE binary16 bits of w if w is exactly representable, else false
+inline bool exact_half_bits(double w, uint16_t *bits) {
+    const uint16_t sign = std::signbit(w) ? 0x8000u : 0u;
+    const double a = std::fabs(w);
+    if (a == 0.) { *bits = sign; return true; }
+    int e;
+    (void)std::frexp(a, &e);                      // a = f 2^e, f in [0.5, 1)
+    const int E = e - 1;                          // a = 1.m 2^E
+    if (E > 15) return false;
+    if (E < -14) {                                // subnormal: a = m 2^-24
+        const double m = std::ldexp(a, 24);
+        if (m != std::floor(m) || m >= 1024.) return false;
+        *bits = (uint16_t)(sign | (uint16_t)m);
+        return true;
+    }
+    const double m = (std::ldexp(a, -E) - 1.) * 1024.;
+    if (m != std::floor(m)) return false;
+    *bits = (uint16_t)(sign | (uint16_t)((E + 15) << 10) | (uint16_t)m);
+    return true;
+}
+
 inline zlh_status upload_transfer(zlh_ctx *ctx, int level) {
     LevelState &F = ctx->lv[level], &C = ctx->lv[level - 1];
     TransferPlan &P = F.xplan;
     TransferTables &X = F.xfer;
     zlh_status s;
-    std::vector<int32_t> tgt, begin, src, src_ai, f2c_c, f2c_f;
-    std::map<int64_t, int32_t> rows;
+    std::vector<int32_t> tgt, begin, src, src_ai, fb_ai, f2c_c, f2c_f;
     for (int64_t t : P.tgt) tgt.push_back(F.node_aux_h[t]);
     for (int64_t b : P.begin) begin.push_back((int32_t)b);
+    // rows of src_f: the coarse sources by coarse aux index, then the fine fallback nodes by fine
+    // aux index
+    for (int64_t c : P.contrib) {
+        if (c >= 0) src_ai.push_back(C.node_aux_h[c]);
+        else fb_ai.push_back(F.node_aux_h[-c - 1]);
+    }
+    for (int32_t v : src_ai) if (v < 0) return ctx->fail(ZLH_EDEVICE, "transfer source not deferred");
+    for (int32_t v : fb_ai) if (v < 0) return ctx->fail(ZLH_EDEVICE, "transfer fallback not deferred");
+    for (auto *v : {&src_ai, &fb_ai}) {
+        std::sort(v->begin(), v->end());
+        v->erase(std::unique(v->begin(), v->end()), v->end());
+    }
+    X.n_fallback = 0;
     for (int64_t c : P.contrib) {
         if (c >= 0) {
-            auto it = rows.find(c);
-            if (it == rows.end()) {
-                it = rows.emplace(c, (int32_t)src_ai.size()).first;
-                src_ai.push_back(C.node_aux_h[c]);
-            }
-            src.push_back(it->second);
+            src.push_back((int32_t)(std::lower_bound(src_ai.begin(), src_ai.end(), C.node_aux_h[c]) - src_ai.begin()));
         } else {
-            src.push_back(-F.node_aux_h[-c - 1] - 1);
+            src.push_back((int32_t)(src_ai.size() + (std::lower_bound(fb_ai.begin(), fb_ai.end(), F.node_aux_h[-c - 1]) - fb_ai.begin())));
+            ++X.n_fallback;
         }
     }
     for (size_t k = 0; k < P.f2c_coarse.size(); ++k) {
@@ -447,60 +553,92 @@ inline zlh_status upload_transfer(zlh_ctx *ctx, int level) {
         f2c_f.push_back(F.node_aux_h[P.f2c_fine[k]]);
     }
     for (int32_t v : tgt) if (v < 0) return ctx->fail(ZLH_EDEVICE, "transfer target not deferred");
-    for (int32_t v : src_ai) if (v < 0) return ctx->fail(ZLH_EDEVICE, "transfer source not deferred");
     X.n_c2f = (int64_t)tgt.size();
     X.n_src = (int64_t)src_ai.size();
+    X.n_fb = (int64_t)fb_ai.size();
     X.n_f2c = (int64_t)f2c_c.size();
-    if ((s = upload(ctx, X.c2f_target, tgt))) return s;
-    if ((s = upload(ctx, X.c2f_begin, begin))) return s;
-    // gather groups: consecutive ghost nodes, their source rows staged in LDS; the group is
-    // halved until the largest row set fits the LDS budget
+    // gather order: ghost nodes by aux index (brick-major Morton order), cut into chunks of
+    // kGatherChunk; inside a chunk by descending contributor count, then aux index.  A group
+    // closes at 256 / PARTS nodes or when its rows and contributor words would pass the budget
     const int Qn = ctx->q;
-    const size_t lds_budget = 48 * 1024;
-    std::vector<int2> packed(src.size());
-    std::vector<int32_t> group_begin, group_rows;
-    int group = 256 / ((Qn + kGatherDirs - 1) / kGatherDirs);
-    for (;; group /= 2) {
-        group_begin.assign(1, 0);
-        group_rows.clear();
-        int max_rows = 0;
-        int64_t max_contrib = 0;
-        const int64_t nt = (int64_t)tgt.size();
-        for (int64_t g0 = 0; g0 < nt; g0 += group) {
-            std::map<int32_t, int32_t> local;
-            for (int64_t t = g0; t < std::min<int64_t>(nt, g0 + group); ++t)
-                for (int32_t k = begin[t]; k < begin[t + 1]; ++k) {
-                    const float cf = (float)P.coeff[k];
-                    if ((double)cf != P.coeff[k]) return ctx->fail(ZLH_EDEVICE, "interpolation weight not exact in FP32");
-                    int32_t bits;
-                    std::memcpy(&bits, &cf, 4);
-                    int32_t sx = src[k];
-                    if (sx >= 0) {
-                        auto it = local.find(sx);
-                        if (it == local.end()) {
-                            it = local.emplace(sx, (int32_t)local.size()).first;
-                            group_rows.push_back(sx);
-                        }
-                        sx = it->second;
-                    }
-                    packed[k] = make_int2(sx, bits);
-                }
-            max_rows = std::max<int>(max_rows, (int)local.size());
-            max_contrib = std::max<int64_t>(max_contrib, begin[std::min<int64_t>(nt, g0 + group)] - begin[g0]);
-            group_begin.push_back((int32_t)group_rows.size());
+    const int parts = (Qn + kGatherDirs - 1) / kGatherDirs, row_stride = parts * kGatherDirs;
+    const size_t lds_budget = kGatherLdsBudget, row_bytes = (size_t)row_stride * ctx->tsize();
+    const int group_max = 256 / parts;
+    const int64_t nt = (int64_t)tgt.size();
+    std::vector<int64_t> order(nt), chunk(nt);
+    for (int64_t t = 0; t < nt; ++t) order[t] = t;
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return tgt[a] < tgt[b]; });
+    for (int64_t k = 0; k < nt; ++k) chunk[order[k]] = k / kGatherChunk;
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        if (chunk[a] != chunk[b]) return chunk[a] < chunk[b];
+        const int32_t na = begin[a + 1] - begin[a], nb = begin[b + 1] - begin[b];
+        return na != nb ? na > nb : tgt[a] < tgt[b];
+    });
+    auto lds_of = [&](size_t nrows, int64_t ncontrib) { return ((nrows * row_bytes + 15) & ~(size_t)15) + (size_t)ncontrib * 4; };
+    std::vector<int32_t> tgt2, begin2(1, 0), group_tgt(1, 0), group_begin(1, 0), group_rows;
+    std::vector<uint32_t> packed;
+    packed.reserve(src.size());
+    std::unordered_map<int32_t, int32_t> local;
+    std::vector<int32_t> distinct;
+    int64_t g_nc = 0;
+    int g_nt = 0;
+    size_t lds_max = 0;
+    auto close_group = [&]() {
+        lds_max = std::max(lds_max, lds_of(local.size(), g_nc));
+        group_tgt.push_back((int32_t)tgt2.size());
+        group_begin.push_back((int32_t)group_rows.size());
+        local.clear();
+        g_nc = 0;
+        g_nt = 0;
+    };
+    for (int64_t t : order) {
+        const int32_t k0 = begin[t], k1 = begin[t + 1];
+        distinct.assign(src.begin() + k0, src.begin() + k1);
+        std::sort(distinct.begin(), distinct.end());
+        distinct.erase(std::unique(distinct.begin(), distinct.end()), distinct.end());
+        size_t n_new = 0;
+        for (int32_t r : distinct) n_new += local.count(r) ? 0 : 1;
+        if (g_nt > 0 && (g_nt == group_max || local.size() + n_new > (size_t)kGatherMaxRows ||
+                         lds_of(local.size() + n_new, g_nc + (k1 - k0)) > lds_budget))
+            close_group();
+        if (lds_of(distinct.size(), k1 - k0) > lds_budget || distinct.size() > (size_t)kGatherMaxRows)
+            return ctx->fail(ZLH_EDEVICE, "interpolation stencil too large for LDS");
+        for (int32_t r : distinct)
+            if (local.emplace(r, (int32_t)local.size()).second) group_rows.push_back(r);
+        for (int32_t k = k0; k < k1; ++k) {
+            uint16_t bits;
+            if (!exact_half_bits(P.coeff[k], &bits)) return ctx->fail(ZLH_EDEVICE, "interpolation weight not exact in FP16");
+            packed.push_back((uint32_t)(local[src[k]] * row_stride) | ((uint32_t)bits << 16));
         }
-        X.max_rows = max_rows;
-        X.lds_bytes = (((size_t)max_rows * Qn * ctx->tsize() + 15) & ~(size_t)15) + (size_t)max_contrib * 8;
-        if (X.lds_bytes <= lds_budget) break;
-        if (group == 1) return ctx->fail(ZLH_EDEVICE, "interpolation stencil too large for LDS");
+        tgt2.push_back(tgt[t]);
+        begin2.push_back((int32_t)packed.size());
+        g_nc += k1 - k0;
+        ++g_nt;
     }
-    X.group = group;
-    X.n_groups = (int64_t)group_begin.size() - 1;
-    if ((s = upload(ctx, X.group_rows_begin, group_begin))) return s;
+    if (g_nt > 0) close_group();
+    X.n_groups = (int64_t)group_tgt.size() - 1;
+    X.lds_bytes = lds_max;
+    X.staged_rows = (int64_t)group_rows.size();
+    X.n_contrib = (int64_t)packed.size();
+    std::vector<GatherGroup> groups(X.n_groups);
+    for (int64_t g = 0; g < X.n_groups; ++g) {
+        GatherGroup &G = groups[g];
+        G.t0 = group_tgt[g];
+        G.t1 = group_tgt[g + 1];
+        G.c0 = begin2[G.t0];
+        G.c1 = begin2[G.t1];
+        G.r0 = group_begin[g];
+        G.r1 = group_begin[g + 1];
+        G.pad0 = G.pad1 = 0;
+    }
+    if ((s = upload(ctx, X.c2f_target, tgt2))) return s;
+    if ((s = upload(ctx, X.c2f_begin, begin2))) return s;
+    if ((s = upload(ctx, X.groups, groups))) return s;
     if ((s = upload(ctx, X.group_rows, group_rows))) return s;
     if ((s = upload(ctx, X.c2f_contrib, packed))) return s;
     if ((s = upload(ctx, X.src_coarse_ai, src_ai))) return s;
-    if ((s = dalloc(ctx, X.src_f, ctx->tsize() * ctx->q * std::max<int64_t>(X.n_src, 1)))) return s;
+    if ((s = upload(ctx, X.fb_fine_ai, fb_ai))) return s;
+    if ((s = dalloc(ctx, X.src_f, ctx->tsize() * ctx->q * std::max<int64_t>(X.n_src + X.n_fb, 1)))) return s;
     if ((s = upload(ctx, X.f2c_coarse_ai, f2c_c))) return s;
     if ((s = upload(ctx, X.f2c_fine_ai, f2c_f))) return s;
     return ZLH_OK;
@@ -549,14 +687,14 @@ zlh_status launch_transfers_t(zlh_ctx *ctx, int level) {
     const double c2f = 0.5 * (nu * 3. / dtc * 2. + 0.5) / (nu * 3. / dtc + 0.5);
     const double f2c = 2. * (nu * 3. / dtf / 2 + 0.5) / (nu * 3. / dtf + 0.5);
     if (X.n_c2f > 0) {
-        if (X.n_src > 0)
-            hipLaunchKernelGGL((k_c2f_sources<D, T, SHIFT>), dim3(blocks(X.n_src)), dim3(256), 0, ctx->stream, X.n_src,
-                               X.src_coarse_ai.as<int32_t>(), C.aux_f_d.as<T>(), C.naux, c2f, comp, X.src_f.as<T>());
-        hipLaunchKernelGGL((k_c2f_gather<D, T, SHIFT>), dim3((unsigned)X.n_groups), dim3(256), X.lds_bytes,
-                           ctx->stream, X.n_c2f, X.group, X.max_rows,
-                           X.c2f_target.as<int32_t>(), X.c2f_begin.as<int32_t>(), X.c2f_contrib.as<int2>(),
-                           X.group_rows_begin.as<int32_t>(), X.group_rows.as<int32_t>(), X.src_f.as<T>(),
-                           F.aux_f_d.as<T>(), F.naux);
+        if (X.n_src + X.n_fb > 0)
+            hipLaunchKernelGGL((k_c2f_sources<D, T, SHIFT>), dim3(blocks(X.n_src + X.n_fb)), dim3(256), 0, ctx->stream,
+                               X.n_src, X.n_fb, X.src_coarse_ai.as<int32_t>(), C.aux_f_d.as<T>(), C.naux,
+                               X.fb_fine_ai.as<int32_t>(), F.aux_f_d.as<T>(), F.naux, c2f, comp, X.src_f.as<T>());
+        hipLaunchKernelGGL((k_c2f_gather<D, T, SHIFT>), dim3((unsigned)((X.n_groups + 7) / 8 * 8)), dim3(256),
+                           X.lds_bytes, ctx->stream, X.n_groups, X.groups.as<GatherGroup>(), X.c2f_target.as<int32_t>(),
+                           X.c2f_begin.as<int32_t>(), X.c2f_contrib.as<uint32_t>(), X.group_rows.as<int32_t>(),
+                           X.src_f.as<T>(), F.aux_f_d.as<T>(), F.naux);
     }
     if (X.n_f2c > 0)
         hipLaunchKernelGGL((k_f2c<D, T, SHIFT>), dim3(blocks(X.n_f2c)), dim3(256), 0, ctx->stream, X.n_f2c,

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <map>
+#include <unordered_map>
 #include <vector>
 #include "kernels.hpp"
 
@@ -31,22 +32,31 @@ struct DevBuf {
 // Coarse-to-fine interpolation and fine-to-coarse copy of one refined level; the tables live on
 // the fine level's state (TransferInfoFromCoarseGrid / TransferInfoToCoarseGrid,
 // lbm_grid_interface.cpp:408-564).
+// one interpolation workgroup: its ghost nodes [t0, t1) of c2f_target, their contributor words
+// [c0, c1) and the rows [r0, r1) of group_rows it stages
+struct GatherGroup {
+    int32_t t0, t1, c0, c1, r0, r1, pad0, pad1;
+};
+
 struct TransferTables {
     int64_t n_c2f = 0;          // fine ghost nodes interpolated from the coarse level
     int64_t n_src = 0;          // coarse nodes feeding the interpolation
     int64_t n_f2c = 0;          // coarse nodes copied from the fine level
-    DevBuf c2f_target;          // [n_c2f] fine aux index
+    DevBuf c2f_target;          // [n_c2f] fine aux index, gather order
     DevBuf c2f_begin;           // [n_c2f + 1] offsets into the contributor list
-    DevBuf c2f_contrib;         // contributor {source, FP32 weight bits}: source >= 0 row of the
-                                // group's staged rows, < 0: -(fine aux index) - 1
-    int group = 0;              // ghost nodes per gather workgroup (consecutive, Morton order)
-    int64_t n_groups = 0;
-    int max_rows = 0;           // most source rows one group stages in LDS
+    DevBuf c2f_contrib;         // contributor word {LDS offset of the row, FP16 weight bits}
+    int64_t n_groups = 0;       // gather workgroups
+    int64_t staged_rows = 0;    // rows staged by all workgroups
+    int64_t n_contrib = 0;      // contributors of all ghost nodes
+    int64_t n_fallback = 0;     // contributors read from a fine fallback node
     size_t lds_bytes = 0;       // rows plus contributor words of the largest group
-    DevBuf group_rows_begin;    // [n_groups + 1] offsets into group_rows
-    DevBuf group_rows;          // source rows (into src_f) of each group, first use order
-    DevBuf src_coarse_ai;       // [n_src] coarse aux index of each source row
-    DevBuf src_f;               // [n_src][Q] rescaled coarse populations (storage form)
+    DevBuf groups;              // [n_groups] GatherGroup
+    DevBuf group_rows;          // rows of src_f each group stages, first use order
+    int64_t n_fb = 0;           // fine nodes standing in for missing coarse sources
+    DevBuf src_coarse_ai;       // [n_src] coarse aux index of each source row (ascending)
+    DevBuf fb_fine_ai;          // [n_fb] fine aux index of each fallback row (ascending)
+    DevBuf src_f;               // [n_src + n_fb][Q] rescaled coarse rows, then fallback copies
+                                // (storage form)
     DevBuf f2c_coarse_ai;       // [n_f2c]
     DevBuf f2c_fine_ai;         // [n_f2c] fine aux index of the coincident node
 };

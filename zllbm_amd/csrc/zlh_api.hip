@@ -1236,6 +1236,16 @@ zlh_status zlh_level_info(zlh_ctx *ctx, int level, int64_t *n_nodes, int64_t *n_
     return ZLH_OK;
 }
 
+zlh_status zlh_transfer_info(zlh_ctx *ctx, int level, int64_t *out) {
+    if (!ctx || !out) return ZLH_EINVAL;
+    if (level < 0 || level > ctx->d.max_level) return ctx->fail(ZLH_EINVAL, "bad level");
+    const TransferTables &X = ctx->lv[level].xfer;
+    const int64_t v[ZLH_XFER_INFO_N] = {X.n_c2f, X.n_src, X.n_f2c, X.n_groups, X.staged_rows, X.n_contrib,
+                                        X.n_fallback, (int64_t)X.lds_bytes};
+    for (int i = 0; i < ZLH_XFER_INFO_N; ++i) out[i] = level > 0 ? v[i] : 0;
+    return ZLH_OK;
+}
+
 zlh_status zlh_synchronize(zlh_ctx *ctx) {
     if (!ctx) return ZLH_EINVAL;
     ZLH_TRY(hipStreamSynchronize(ctx->stream));

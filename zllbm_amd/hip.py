@@ -41,7 +41,7 @@ EXPORTED = (
     "zlh_create", "zlh_destroy", "zlh_last_error", "zlh_set_level_nodes", "zlh_set_interface_layer",
     "zlh_ib_set_markers", "zlh_build", "zlh_init_feq", "zlh_init_macros", "zlh_upload_populations",
     "zlh_step_level", "zlh_transfer_levels", "zlh_time_marching", "zlh_download_populations", "zlh_download_macros",
-    "zlh_download_source_table", "zlh_ib_download_forces", "zlh_conservation", "zlh_level_info",
+    "zlh_download_source_table", "zlh_ib_download_forces", "zlh_conservation", "zlh_level_info", "zlh_transfer_info",
     "zlh_synchronize", "zlh_timing_enable", "zlh_timing_read", "zlh_mlp_set", "zlh_mlp_gradient",
     "zlh_mlp_train", "zlh_mlp_eval", "zlh_closure_features", "zlh_graphs_enable",
 )
@@ -78,6 +78,7 @@ def lib():
         L.zlh_ib_download_forces.argtypes = [P, C.c_int, D(np.float64)]
         L.zlh_conservation.argtypes = [P, C.c_int, D(np.float64)]
         L.zlh_level_info.argtypes = [P, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.zlh_transfer_info.argtypes = [P, C.c_int, C.POINTER(C.c_int64)]
         L.zlh_synchronize.argtypes = [P]
         L.zlh_timing_enable.argtypes = [P, C.c_int]
         L.zlh_timing_read.argtypes = [P, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
@@ -201,6 +202,14 @@ class Solver:
         n, a, b = C.c_int64(), C.c_int64(), C.c_int64()
         self._chk(lib().zlh_level_info(self.h, level, C.byref(n), C.byref(a), C.byref(b)))
         return n.value, a.value, b.value
+
+    def transfer_info(self, level) -> dict:
+        """Interpolation tables of a refined level: sizes of the gather's staged data."""
+        out = (C.c_int64 * 8)()
+        self._chk(lib().zlh_transfer_info(self.h, level, out))
+        keys = ("ghost_nodes", "source_rows", "f2c_nodes", "groups", "staged_rows", "contributors",
+                "fallback_contributors", "lds_bytes")
+        return dict(zip(keys, list(out)))
 
     def f(self, level=0) -> np.ndarray:
         out = np.zeros(self.n(level) * self.q)
