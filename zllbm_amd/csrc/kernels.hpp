@@ -488,58 +488,89 @@ struct BrickTables {
     const uint32_t *__restrict__ aux_missing;
 };
 
+// Per-lane constants of the pull, fixed by the brick layout and computed once per pull: the
+// byte offset 4 lane_of() of the source coordinate along each axis for c = +1, 0, -1, and
+// whether the lane sits on the low / high face of each axis.  lane_of is a sum of one term per
+// axis, so a source slot is the OR of three of these.
+template <int D>
+struct PullLane {
+    uint32_t s4[3][3];   // [axis][1 - c]
+    bool lo[3], hi[3];
+};
+
+template <int D>
+__device__ __forceinline__ PullLane<D> pull_lane(int lane) {
+    using B = Brick<D>;
+    constexpr int E = B::edge;
+    PullLane<D> P;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int l = a < D ? B::lane_coord(lane, a) : 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int sc = (l - (1 - k)) & (E - 1);
+            const int lo = a == 0 ? B::lane_of(sc, 0, 0) : (a == 1 ? B::lane_of(0, sc, 0) : B::lane_of(0, 0, sc));
+            P.s4[a][k] = a < D ? 4u * (uint32_t)lo : 0u;
+        }
+        P.lo[a] = a < D && l == 0;
+        P.hi[a] = a < D && l == E - 1;
+    }
+    return P;
+}
+
 // Pull of brick b (every lane, every direction) into a per-wave LDS buffer [Q][64] with one
 // direct-to-LDS dword load per direction (gfx950 global_load_lds_dword: lane i lands at
-// buf + 4 i).  Absent sources of deferred nodes load from the aux array, lanes without a node
-// load their own slot (a valid address).
+// buf + 4 i).  Every lane loads from a valid address of fin: an absent neighbour brick is
+// replaced by b, a deferred node's missing source loads whatever slot the stencil names, and a
+// lane without a node loads its own slot; the consumer drops those values (a deferred node's
+// missing directions keep the boundary / transfer values already in the aux array).  Addresses are 32-bit byte offsets
+// from the direction's base fin + 64 q.  Returns the lane's missing-source mask (deferred nodes).
 template <int D, typename T>
-__device__ __forceinline__ void pull_to_lds(const BrickTables &t, int64_t naux, const T *__restrict__ fin,
-                                            const T *__restrict__ aux_f, int64_t b, int lane, T *buf) {
+__device__ __forceinline__ uint32_t pull_to_lds(const BrickTables &t, const T *__restrict__ fin, int64_t b, int lane,
+                                                T *buf) {
     using L = Lattice<D>;
     using B = Brick<D>;
     constexpr int Q = L::Q;
-    constexpr int E = B::edge;
     constexpr int NN = B::nnbr;
     static_assert(sizeof(T) == 4, "dword staging");
-    const bool has_node = (t.occ[b] >> lane) & 1ull;
-    const uint64_t am = t.auxmask[b];
-    uint32_t missing = 0;
-    int64_t ai = 0;
-    if ((am >> lane) & 1ull) {
-        ai = t.auxbase[b] + __popcll(am & ((1ull << lane) - 1ull));
-        missing = t.aux_missing[ai];
-    }
-    // consume the mask here: its wait then precedes the direct-to-LDS loads instead of landing
-    // between them (where the in-order counter would make each wait for the ones before)
-    asm volatile("" : "+v"(missing));
+    const PullLane<D> P = pull_lane<D>(lane);   // per pull: live only while the addresses form
     int32_t nb[NN];
 #pragma unroll
-    for (int k = 0; k < NN; ++k) nb[k] = t.nbr[b * NN + k];
-    const int lx = B::lane_coord(lane, 0), ly = B::lane_coord(lane, 1);
-    const int lz = D == 3 ? B::lane_coord(lane, 2) : 0;
+    for (int k = 0; k < NN; ++k) {
+        const int32_t v = t.nbr[b * NN + k];
+        nb[k] = v < 0 ? (int32_t)b : v;
+    }
+    // lanes without a node load their own slot (a line the brick's own pull fetches anyway)
+    const bool has_node = (t.occ[b] >> lane) & 1ull;
+    const uint32_t own = __umul24((uint32_t)b, (uint32_t)(Q * 64 * sizeof(T))) + 4u * (uint32_t)lane;
     // addresses by selection, no branches: a branch between two of these loads makes the
     // compiler wait on the vector memory counter there
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        const T *src = fin + (b * Q + q) * 64 + lane;
-        if (q > 0) {
-            const int sx = lx - L::cx[q], sy = ly - L::cy[q], sz = lz - L::cz[q];
-            const bool ex = L::cx[q] != 0 && (sx < 0 || sx >= E);
-            const bool ey = L::cy[q] != 0 && (sy < 0 || sy >= E);
-            const bool ez = D == 3 && L::cz[q] != 0 && (sz < 0 || sz >= E);
-            constexpr int base = D == 3 ? 13 : 4;
-            const int dx = -L::cx[q], dy = -3 * L::cy[q], dz = D == 3 ? -9 * L::cz[q] : 0;
-            const int sb = ez ? (ey ? (ex ? nb[base + dx + dy + dz] : nb[base + dy + dz])
-                                    : (ex ? nb[base + dx + dz] : nb[base + dz]))
-                              : (ey ? (ex ? nb[base + dx + dy] : nb[base + dy])
-                                    : (ex ? nb[base + dx] : nb[base]));
-            const T *pulled = fin + ((int64_t)sb * Q + q) * 64 + B::lane_of(sx & (E - 1), sy & (E - 1), sz & (E - 1));
-            const T *deferred = aux_f + q * naux + ai;
-            src = has_node ? (((missing >> q) & 1u) ? deferred : pulled) : src;
-        }
+        constexpr int base = D == 3 ? 13 : 4;
+        const int cx = L::cx[q], cy = L::cy[q], cz = D == 3 ? L::cz[q] : 0;
+        const bool ex = cx == 1 ? P.lo[0] : (cx == -1 ? P.hi[0] : false);
+        const bool ey = cy == 1 ? P.lo[1] : (cy == -1 ? P.hi[1] : false);
+        const bool ez = cz == 1 ? P.lo[2] : (cz == -1 ? P.hi[2] : false);
+        const int dx = -cx, dy = -3 * cy, dz = -9 * cz;
+        const int sb = ez ? (ey ? (ex ? nb[base + dx + dy + dz] : nb[base + dy + dz])
+                                : (ex ? nb[base + dx + dz] : nb[base + dz]))
+                          : (ey ? (ex ? nb[base + dx + dy] : nb[base + dy])
+                                : (ex ? nb[base + dx] : nb[base]));
+        const uint32_t off = has_node ? __umul24((uint32_t)sb, (uint32_t)(Q * 64 * sizeof(T))) +
+                                            (P.s4[0][1 - cx] | P.s4[1][1 - cy] | P.s4[2][1 - cz])
+                                      : own;
+        // an opaque scalar base, so that the load takes the scalar-base + 32-bit-offset form
+        const char *qbase = reinterpret_cast<const char *>(fin + q * 64);
+        asm volatile("" : "+s"(qbase));
+        const char *src = qbase + off;
         __builtin_amdgcn_global_load_lds((const void *)src, (__attribute__((address_space(3))) void *)(buf + q * 64),
                                          4, 0, 0);
     }
+    uint32_t missing = 0;
+    const uint64_t am = t.auxmask[b];
+    if ((am >> lane) & 1ull) missing = t.aux_missing[t.auxbase[b] + __popcll(am & ((1ull << lane) - 1ull))];
+    return missing;
 }
 
 // The same with the FP32 learned closure.  Each workgroup stages the network's operand images
@@ -571,7 +602,7 @@ void k_stream_collide_mlp(int64_t nbrick, int64_t naux, const uint64_t *__restri
     T *buf = stage[wave];
     int64_t b = g * per_group + wave;
     if (b >= nbrick) return;                 // uniform per wave; no barrier follows
-    pull_to_lds<D, T>(t, naux, fin, aux_f, b, threadIdx.x & 63, buf);
+    uint32_t miss = pull_to_lds<D, T>(t, fin, b, threadIdx.x & 63, buf);
 #pragma unroll 1
     for (int r = 0; r < ROUNDS; ++r) {
         int lane = threadIdx.x & 63;
@@ -583,16 +614,19 @@ void k_stream_collide_mlp(int64_t nbrick, int64_t naux, const uint64_t *__restri
         const bool has_node = (t.occ[b] >> lane) & 1ull;
         const uint64_t am = t.auxmask[b];
         if ((am >> lane) & 1ull) {
-            // deferred node: post-stream values to the aux array
+            // deferred node: post-stream values to the aux array, except the missing sources
+            // (their boundary / transfer values are there already)
             const int64_t ai = t.auxbase[b] + __popcll(am & ((1ull << lane) - 1ull));
 #pragma unroll
-            for (int q = 0; q < Q; ++q) aux_f[q * naux + ai] = T(f[q]);
+            for (int q = 0; q < Q; ++q)
+                if (!((miss >> q) & 1u)) aux_f[q * naux + ai] = T(f[q]);
         }
         // the buffer's reads have returned before it is refilled
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
         const int64_t bn = b + 4;
         const bool more = r + 1 < ROUNDS && bn < nbrick;
-        if (more) pull_to_lds<D, T>(t, naux, fin, aux_f, bn, lane, buf);
+        uint32_t miss_next = 0;
+        if (more) miss_next = pull_to_lds<D, T>(t, fin, bn, lane, buf);
         if (!has_node) {
 #pragma unroll
             for (int q = 0; q < Q; ++q) f[q] = C(0);          // rest state (shifted storage)
@@ -607,6 +641,7 @@ void k_stream_collide_mlp(int64_t nbrick, int64_t naux, const uint64_t *__restri
         for (int q = 0; q < Q; ++q) __builtin_nontemporal_store(T(f[q]), dst + q * 64);
         if (!more) break;
         b = bn;
+        miss = miss_next;
     }
 }
 
