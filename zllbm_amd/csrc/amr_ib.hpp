@@ -641,6 +641,9 @@ inline zlh_status upload_transfer(zlh_ctx *ctx, int level) {
     if ((s = dalloc(ctx, X.src_f, ctx->tsize() * ctx->q * std::max<int64_t>(X.n_src + X.n_fb, 1)))) return s;
     if ((s = upload(ctx, X.f2c_coarse_ai, f2c_c))) return s;
     if ((s = upload(ctx, X.f2c_fine_ai, f2c_f))) return s;
+    std::vector<uint8_t> macro_flag(std::max<int64_t>(F.naux, 1), 0);
+    for (int32_t i : f2c_f) macro_flag[i] = 1;
+    if ((s = upload(ctx, F.aux_macro_flag_d, macro_flag))) return s;
     return ZLH_OK;
 }
 

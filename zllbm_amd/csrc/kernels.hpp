@@ -574,10 +574,10 @@ __device__ __forceinline__ uint32_t pull_to_lds(const BrickTables &t, const T *_
 }
 
 // The same with the FP32 learned closure.  Each workgroup stages the network's operand images
-// in LDS once; its four waves then take ROUNDS rounds of four adjacent bricks (4 on large
-// levels; 1 on small ones, where the grid would not fill the chip otherwise), and each wave
-// pulls its next brick into LDS while it collides the current one, so the pull's
-// latency hides behind the network's arithmetic at the occupancy its registers allow.  Within
+// in LDS once (after its waves' first pulls are out); its four waves then take ROUNDS rounds of
+// four adjacent bricks (the host picks 1 or 2), and each wave pulls its next brick into LDS
+// while it collides the current one, so the pull's latency hides behind the network's
+// arithmetic at the occupancy its registers allow.  Within
 // a round the order is: read the staged brick, write deferred nodes' post-stream values, issue
 // the next pull, collide, store; the store group is unconditional so that the next round's wait
 // (in-order vector memory counter) covers the pull and not these stores.
@@ -597,12 +597,14 @@ void k_stream_collide_mlp(int64_t nbrick, int64_t naux, const uint64_t *__restri
     constexpr int64_t per_group = 4 * ROUNDS;
     const int64_t g = xcd_group((nbrick + per_group - 1) / per_group);
     if (g < 0) return;                       // uniform per workgroup: before the barrier
-    stage_mlp_ops(ops, p.mlp->ops);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     T *buf = stage[wave];
     int64_t b = g * per_group + wave;
+    // the first pull goes out before the network is staged, so that their latencies overlap
+    uint32_t miss = 0;
+    if (b < nbrick) miss = pull_to_lds<D, T>(t, fin, b, threadIdx.x & 63, buf);
+    stage_mlp_ops(ops, p.mlp->ops);
     if (b >= nbrick) return;                 // uniform per wave; no barrier follows
-    uint32_t miss = pull_to_lds<D, T>(t, fin, b, threadIdx.x & 63, buf);
 #pragma unroll 1
     for (int r = 0; r < ROUNDS; ++r) {
         int lane = threadIdx.x & 63;
@@ -646,22 +648,17 @@ void k_stream_collide_mlp(int64_t nbrick, int64_t naux, const uint64_t *__restri
 }
 
 // Collision of the aux nodes (deferred from the previous substep), writing post-collision
-// values into the buffer the next fused kernel reads, plus rho/u (GridNodeLbm::rho_/velocity_).
+// values into the buffer the next fused kernel reads, plus rho/u (GridNodeLbm::rho_/velocity_) of
+// the nodes flagged in macro_flag (the fine-to-coarse sources, the only readers).
 // VAR as in the fused kernel; with the FP32 learned closure the workgroup stages the network in
 // LDS first.
 template <int D, typename T, typename C, bool SHIFT, int COLL, int VAR>
 __global__ __launch_bounds__(256) void k_aux_collide(BrickLevel<T> lv, const T *__restrict__ aux_f,
                                                      const double *__restrict__ aux_force,
                                                      T *__restrict__ fa, double *__restrict__ aux_macro,
-                                                     StepParams p) {
+                                                     const uint8_t *__restrict__ macro_flag, StepParams p) {
     constexpr int Q = Lattice<D>::Q;
     constexpr bool kWave = mlp_on_mfma<C, VAR>();
-    const MlpOps *ops = nullptr;
-    if constexpr (kWave) {
-        __shared__ MlpOps s_ops;
-        stage_mlp_ops(s_ops, p.mlp->ops);
-        ops = &s_ops;
-    }
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     // lanes past the end run the collision on the rest state (FP32 learned closure is
     // wavefront-wide) and store nothing
@@ -670,6 +667,13 @@ __global__ __launch_bounds__(256) void k_aux_collide(BrickLevel<T> lv, const T *
     C f[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) f[q] = active ? C(aux_f[q * lv.naux + i]) : C(SHIFT ? 0. : Lattice<D>::w[q]);
+    // the network is staged after the node's loads are out, so that their latencies overlap
+    const MlpOps *ops = nullptr;
+    if constexpr (kWave) {
+        __shared__ MlpOps s_ops;
+        stage_mlp_ops(s_ops, p.mlp->ops);
+        ops = &s_ops;
+    }
     C F[3] = {C(p.const_force[0]), C(p.const_force[1]), C(p.const_force[2])};
     if (active && (VAR & 1) && aux_force) {
         F[0] += C(aux_force[i * 3]); F[1] += C(aux_force[i * 3 + 1]); F[2] += C(aux_force[i * 3 + 2]);
@@ -680,10 +684,12 @@ __global__ __launch_bounds__(256) void k_aux_collide(BrickLevel<T> lv, const T *
     const int64_t slot = lv.aux_slot[i];
 #pragma unroll
     for (int q = 0; q < Q; ++q) fa[pop_at<Q>(q, slot)] = T(f[q]);
-    aux_macro[i * 4] = double(rho);
-    aux_macro[i * 4 + 1] = double(u[0]);
-    aux_macro[i * 4 + 2] = double(u[1]);
-    aux_macro[i * 4 + 3] = double(u[2]);
+    if (macro_flag[i]) {
+        aux_macro[i * 4] = double(rho);
+        aux_macro[i * 4 + 1] = double(u[0]);
+        aux_macro[i * 4 + 2] = double(u[1]);
+        aux_macro[i * 4 + 3] = double(u[2]);
+    }
 }
 
 }  // namespace zlh

@@ -116,7 +116,8 @@ struct LevelState {
     std::vector<uint8_t> aux_mark;   // nodes that must take the deferred path
     // device
     DevBuf keys_d, bkeys_d, node_slot_d, slot_node_d, occ_d, nbr_d, auxmask_d, auxbase_d, aux_missing_d,
-        aux_slot_d, node_aux_d, f_d[2], aux_f_d, aux_macro_d, aux_force_d;
+        aux_slot_d, node_aux_d, f_d[2], aux_f_d, aux_macro_d, aux_force_d,
+        aux_macro_flag_d;   // [naux] 1: the aux collision records rho, u (fine-to-coarse sources)
     // boundary tables per direction: aux index of node, partner aux index, slot
     DevBuf bc_ai_d[7], bc_partner_d[7], bc_slot_d[7];
     int64_t bc_n[7] = {0, 0, 0, 0, 0, 0, 0};

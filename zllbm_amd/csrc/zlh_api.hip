@@ -522,6 +522,32 @@ int64_t find_key(const std::vector<uint64_t> &keys, uint64_t k) {
 // ==============================================================================================
 namespace {
 
+// learned-closure kernel with R rounds of four bricks per workgroup
+template <int D, typename T, typename C, bool SHIFT, int COLL, int V, int R>
+void launch_mlp_rounds(zlh_ctx *ctx, const BrickLevel<T> &v, T *fa, T *fb, T *aux_f, const StepParams &sp) {
+    const int64_t ng = (v.nbrick + 4 * R - 1) / (4 * R);
+    hipLaunchKernelGGL((k_stream_collide_mlp<D, T, C, SHIFT, COLL, V, R>), dim3((unsigned)((ng + 7) / 8 * 8)), dim3(256),
+                       0, ctx->stream, v.nbrick, v.naux, v.occ, v.auxmask, v.auxbase, v.nbr, v.aux_missing, fa, fb,
+                       aux_f, sp);
+}
+
+// one round per workgroup while the grid is at most two waves of resident workgroups (the
+// latency of a short grid is one brick's), two on larger levels (each wave's second pull then
+// overlaps its first brick's network; more rounds measured slower on a 3M-node level)
+template <int D, typename T, typename C, bool SHIFT, int COLL, int V>
+void launch_mlp(zlh_ctx *ctx, const BrickLevel<T> &v, T *fa, T *fb, T *aux_f, const StepParams &sp) {
+    static int per_cu = [] {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stream_collide_mlp<D, T, C, SHIFT, COLL, V, 1>, 256, 0) !=
+                hipSuccess || n < 1)
+            n = 1;
+        return n;
+    }();
+    const int64_t resident = (int64_t)per_cu * ctx->n_cu;
+    if ((v.nbrick + 3) / 4 <= 2 * resident) launch_mlp_rounds<D, T, C, SHIFT, COLL, V, 1>(ctx, v, fa, fb, aux_f, sp);
+    else launch_mlp_rounds<D, T, C, SHIFT, COLL, V, 2>(ctx, v, fa, fb, aux_f, sp);
+}
+
 // bulk kernel variant: collision operator x (sub-grid model, forcing)
 template <int D, typename T, typename C, bool SHIFT, int COLL>
 void launch_stream_collide_v(zlh_ctx *ctx, int var, unsigned grid, const BrickLevel<T> &v, T *fa, T *fb, T *aux_f,
@@ -529,18 +555,7 @@ void launch_stream_collide_v(zlh_ctx *ctx, int var, unsigned grid, const BrickLe
 #define ZLH_SC(V)                                                                                             \
     case V:                                                                                                   \
         if constexpr (mlp_on_mfma<C, V>()) {                                                                  \
-            /* 4 rounds once the grid still gives every CU 4 workgroups */                                \
-            if (v.nbrick >= (int64_t)16 * 4 * ctx->n_cu) {                                                   \
-                const int64_t ng = (v.nbrick + 15) / 16;                                                      \
-                hipLaunchKernelGGL((k_stream_collide_mlp<D, T, C, SHIFT, COLL, V, 4>), dim3((unsigned)((ng + 7) / 8 * 8)), \
-                                   dim3(256), 0, ctx->stream, v.nbrick, v.naux, v.occ, v.auxmask, v.auxbase, v.nbr, \
-                                   v.aux_missing, fa, fb, aux_f, sp);                                         \
-            } else {                                                                                          \
-                const int64_t ng = (v.nbrick + 3) / 4;                                                        \
-                hipLaunchKernelGGL((k_stream_collide_mlp<D, T, C, SHIFT, COLL, V, 1>), dim3((unsigned)((ng + 7) / 8 * 8)), \
-                                   dim3(256), 0, ctx->stream, v.nbrick, v.naux, v.occ, v.auxmask, v.auxbase, v.nbr, \
-                                   v.aux_missing, fa, fb, aux_f, sp);                                         \
-            }                                                                                                 \
+            launch_mlp<D, T, C, SHIFT, COLL, V>(ctx, v, fa, fb, aux_f, sp);                                   \
         } else {                                                                                              \
             hipLaunchKernelGGL((k_stream_collide<D, T, C, SHIFT, COLL, V>), dim3(grid), dim3(256), 0, ctx->stream, \
                                v, fa, fb, aux_f, sp);                                                         \
@@ -559,7 +574,8 @@ void launch_aux_collide_v(zlh_ctx *ctx, int var, LevelState &L, const BrickLevel
 #define ZLH_AC(V)                                                                                             \
     case V:                                                                                                   \
         hipLaunchKernelGGL((k_aux_collide<D, T, C, SHIFT, COLL, V>), dim3(blocks(L.naux)), dim3(256), 0,       \
-                           ctx->stream, v, aux_f, aux_force, fa, L.aux_macro_d.as<double>(), L.sp);           \
+                           ctx->stream, v, aux_f, aux_force, fa, L.aux_macro_d.as<double>(),                  \
+                           L.aux_macro_flag_d.as<uint8_t>(), L.sp);                                           \
         break;
     switch (var) {
         ZLH_AC(0) ZLH_AC(1) ZLH_AC(2) ZLH_AC(3) ZLH_AC(4) ZLH_AC(5) ZLH_AC(6) ZLH_AC(7) ZLH_AC(8) ZLH_AC(9)
@@ -847,6 +863,7 @@ zlh_status build_level_aux(zlh_ctx *ctx, int level) {
     if ((s = dalloc(ctx, L.f_d[1], ts * Q * nslot))) return s;
     if ((s = dalloc(ctx, L.aux_f_d, ts * Q * std::max<int64_t>(L.naux, 1)))) return s;
     if ((s = dalloc(ctx, L.aux_macro_d, sizeof(double) * 4 * std::max<int64_t>(L.naux, 1)))) return s;
+    if ((s = dalloc(ctx, L.aux_macro_flag_d, std::max<int64_t>(L.naux, 1)))) return s;
     if ((s = dalloc(ctx, L.aux_force_d, sizeof(double) * 3 * std::max<int64_t>(L.naux, 1)))) return s;
     L.sp = level_params(ctx, level);
     ZLH_TRY(hipStreamSynchronize(ctx->stream));
