@@ -157,6 +157,60 @@ def test_fp32_learned_closure_tracks_fp64(operands):
     assert l2_rho < 1e-5, l2_rho
 
 
+def test_fp32_learned_closure_two_rounds_per_workgroup():
+    """A level large enough (97^3 nodes in 15,625 bricks: more than two waves of resident
+    workgroups on a 256-CU device) that the learned-closure kernel takes two rounds of bricks per
+    workgroup, each wave pulling its second brick into LDS while it collides the first. FP32
+    against the FP64 solver with the same network, in lockstep: a round that read its brick before
+    the pull had landed would leave the FP32 run far from the FP64 one."""
+    nu = 0.002
+    grid, kw = sc.periodic_tgv(3, 96, nu=nu)
+    m = _random_model(3 * nu + 0.5)
+    init = sc.tgv_fields(grid, 0.05, noise=1e-3)
+    out = []
+    for prec in (hip.FP64, hip.FP32_SHIFTED):
+        s = hip.Solver(grid, precision=prec, collision=(hip.MRT,), les=hip.LES_MLP, cs=0.1, **kw).build()
+        assert s.info(0)[2] == 25 ** 3
+        s.mlp_set(m)
+        s.init_macro(0, *init)
+        s.time_marching(50)
+        out.append(s.macros(0))
+        del s
+    (rho_d, u_d), (rho_s, u_s) = out
+    l2_u = np.sqrt(np.sum((u_s - u_d) ** 2) / np.sum(u_d ** 2))
+    l2_rho = np.sqrt(np.mean((rho_s - rho_d) ** 2))
+    print("97^3, FP32 + closure vs FP64 + closure after 50 steps: L2(u) %.2e, L2(rho) %.2e" % (l2_u, l2_rho))
+    assert l2_u < 2e-5, l2_u
+    assert l2_rho < 1e-6, l2_rho
+
+
+def test_four_level_turbulence_fp32_tracks_fp64():
+    """The 4-level decaying-turbulence workload (3M nodes on the finest level, whose
+    learned-closure kernel takes two rounds per workgroup) with the trained closure: FP32
+    against FP64 in lockstep over 100 background steps. FP32 rounding alone gives 1.5e-5 here; a
+    round reading its brick before the pull had landed gave 9e-4."""
+    import os
+    from zllbm_amd import workloads
+    W = workloads.decaying_turbulence(32, 4)
+    model = closure.ClosureModel.load(os.path.join(os.path.dirname(closure.__file__), "data", "closure_turbulence.npz"))
+    out = []
+    for prec in (hip.FP64, hip.FP32_SHIFTED):
+        kw = dict(W.solver)
+        kw.update(les=hip.LES_MLP)
+        s = hip.Solver(W.grid, precision=prec, **kw).build()
+        s.mlp_set(model)
+        for lvl, (rho, u) in enumerate(W.init):
+            s.init_macro(lvl, rho, u)
+        s.time_marching(100)
+        out.append([s.macros(l) for l in range(W.grid.max_level + 1)])
+        del s
+    u_d = np.concatenate([m[1] for m in out[0]])
+    u_s = np.concatenate([m[1] for m in out[1]])
+    l2_u = np.sqrt(np.sum((u_s - u_d) ** 2) / np.sum(u_d ** 2))
+    print("4-level turbulence, FP32 + learned vs FP64 + learned after 100 steps: L2(u) %.2e" % l2_u)
+    assert l2_u < 1e-4, l2_u
+
+
 def test_closure_features_and_fallback():
     nu, cs = 0.002, 0.1
     tau0 = 3 * nu + 0.5

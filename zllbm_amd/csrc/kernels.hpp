@@ -605,8 +605,13 @@ void k_stream_collide_mlp(int64_t nbrick, int64_t naux, const uint64_t *__restri
     if (b < nbrick) miss = pull_to_lds<D, T>(t, fin, b, threadIdx.x & 63, buf);
     stage_mlp_ops(ops, p.mlp->ops);
     if (b >= nbrick) return;                 // uniform per wave; no barrier follows
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // the first pull has landed
 #pragma unroll 1
     for (int r = 0; r < ROUNDS; ++r) {
+        // the pull into buf has landed: the compiler does not order LDS reads after direct-to-LDS
+        // loads of an earlier iteration.  In-order counter: the last Q vector memory operations
+        // before this point are the previous round's stores, which may stay in flight.
+        asm volatile("s_waitcnt vmcnt(%0)" ::"n"(Q) : "memory");
         int lane = threadIdx.x & 63;
         // opaque per brick: keeps the lane's stencil offsets from being hoisted out of the loop
         asm volatile("" : "+v"(lane));
