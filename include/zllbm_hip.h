@@ -116,6 +116,11 @@ zlh_status zlh_time_marching(zlh_ctx *ctx, int nsteps);
 /* zlh_time_marching replays pairs of background steps as captured HIP graphs (default on);
  * off launches every kernel directly (no reference counterpart) */
 zlh_status zlh_graphs_enable(zlh_ctx *ctx, int enable);
+/* run each level's substeps of zlh_time_marching on a stream of its own, ordered by events
+ * around the transfers, so that coarse levels overlap finer ones (default 1; no reference
+ * counterpart).  Results are identical either way; timed runs (zlh_timing_enable) use one
+ * stream. */
+zlh_status zlh_level_streams_enable(zlh_ctx *ctx, int enable);
 /* post-stream populations GridNodeLbm::f_ in key order [n][q] (synchronising) */
 zlh_status zlh_download_populations(zlh_ctx *ctx, int level, double *f);
 /* density and velocity of the post-stream populations, key order; u is [n][dims] (synchronising) */
@@ -133,8 +138,10 @@ zlh_status zlh_level_info(zlh_ctx *ctx, int level, int64_t *n_nodes, int64_t *n_
 /* interpolation tables of a refined level (no reference counterpart): out[ZLH_XFER_INFO_N] =
  * {ghost nodes, coarse source rows, fine-to-coarse nodes, gather workgroups, rows staged by all
  * workgroups, contributors, contributors taken from a fine fallback node, LDS bytes of the largest
- * workgroup}; zeros for level 0 */
-#define ZLH_XFER_INFO_N 8
+ * workgroup, 1 if the transfer into the next finer level writes a node of this level that this
+ * level's own transfer touches, or reads one it writes (time marching then orders the two)};
+ * zeros for level 0 */
+#define ZLH_XFER_INFO_N 9
 zlh_status zlh_transfer_info(zlh_ctx *ctx, int level, int64_t *out);
 zlh_status zlh_synchronize(zlh_ctx *ctx);
 /* device time of the fused stream-collide kernel: enable recording, then read the total in

@@ -140,25 +140,29 @@ def test_vorticity_refined_turbulence_fp32_velocity_l2():
 
 
 def test_graph_replay_matches_direct_launches():
-    """Replayed two-step graphs against direct launches, over calls of odd and even length with
-    manual substeps and downloads (synchronisations) in between: every replay bit-identical."""
+    """Replayed two-step graphs and direct launches, each with the levels on streams of their own,
+    against direct launches on one stream, over calls of odd and even length with manual substeps
+    and downloads (synchronisations) in between: every replay bit-identical."""
     from zllbm_amd import workloads
     W = workloads.decaying_turbulence(24, 3, buffer=2)
     runs = []
-    for graphs in (True, False):
+    for graphs, streams in ((True, True), (False, True), (False, False)):
         s = hip.Solver(W.grid, precision=hip.FP32_SHIFTED, **W.solver).build()
         s.graphs_enable(graphs)
+        s.level_streams_enable(streams)
         for l, (rho, u) in enumerate(W.init):
             s.init_macro(l, rho, u)
         runs.append(s)
-    a, b = runs
+    ref = runs[-1]
     for n in (2, 3, 4, 1, 2, 2):
-        a.time_marching(n)
-        b.time_marching(n)
+        for s in runs:
+            s.time_marching(n)
         for l in range(3):
-            np.testing.assert_array_equal(a.f(l), b.f(l))
+            f = ref.f(l)
+            for s in runs[:-1]:
+                np.testing.assert_array_equal(s.f(l), f)
         if n == 1:          # a finest-level substep outside the graph flips that level's buffers
-            for s in (a, b):
+            for s in runs:
                 s.step_level(2, 1)
 
 

@@ -641,6 +641,19 @@ inline zlh_status upload_transfer(zlh_ctx *ctx, int level) {
     if ((s = dalloc(ctx, X.src_f, ctx->tsize() * ctx->q * std::max<int64_t>(X.n_src + X.n_fb, 1)))) return s;
     if ((s = upload(ctx, X.f2c_coarse_ai, f2c_c))) return s;
     if ((s = upload(ctx, X.f2c_fine_ai, f2c_f))) return s;
+    F.xfer_writes = tgt;
+    F.xfer_reads = fb_ai;
+    F.xfer_reads.insert(F.xfer_reads.end(), f2c_f.begin(), f2c_f.end());
+    if (level - 1 >= 1) {
+        // the coarse level's own transfer (from level - 2) against this one's coarse side: this
+        // one reads src_ai and writes f2c_c there
+        std::vector<uint8_t> mark(std::max<int64_t>(C.naux, 1), 0);   // 1 read, 2 written
+        for (int32_t i : C.xfer_reads) mark[i] |= 1;
+        for (int32_t i : C.xfer_writes) mark[i] |= 2;
+        C.xfer_overlap = false;
+        for (int32_t i : src_ai) C.xfer_overlap |= (mark[i] & 2) != 0;
+        for (int32_t i : f2c_c) C.xfer_overlap |= mark[i] != 0;
+    }
     std::vector<uint8_t> macro_flag(std::max<int64_t>(F.naux, 1), 0);
     for (int32_t i : f2c_f) macro_flag[i] = 1;
     if ((s = upload(ctx, F.aux_macro_flag_d, macro_flag))) return s;

@@ -114,6 +114,11 @@ struct LevelState {
     std::vector<int32_t> node_slot_h, node_aux_h;
     std::vector<uint32_t> missing_h;
     std::vector<uint8_t> aux_mark;   // nodes that must take the deferred path
+    // aux nodes of this level that the transfer from the coarser level writes (ghost targets) and
+    // reads (fallback rows, fine-to-coarse sources), and whether the transfer into the next finer
+    // level conflicts with them (writes a node the first touches, or reads a node it writes)
+    std::vector<int32_t> xfer_writes, xfer_reads;
+    bool xfer_overlap = false;
     // device
     DevBuf keys_d, bkeys_d, node_slot_d, slot_node_d, occ_d, nbr_d, auxmask_d, auxbase_d, aux_missing_d,
         aux_slot_d, node_aux_d, f_d[2], aux_f_d, aux_macro_d, aux_force_d,

@@ -383,6 +383,12 @@ struct zlh_ctx {
     // indices (two steps flip every level's buffers an even number of times, so replays chain)
     bool use_graphs = true;
     std::map<uint32_t, hipGraphExec_t> graph_exec;
+    // time marching runs each level on a stream of its own; events order a transfer after the
+    // coarse step it pairs with, and the coarse level's next step after the transfer
+    bool level_streams = true;
+    std::vector<hipStream_t> lstream;
+    std::vector<hipEvent_t> ev_step, ev_xfer, ev_join;
+    hipEvent_t ev_fork = nullptr;
     void drop_graphs() {
         for (auto &g : graph_exec) (void)hipGraphExecDestroy(g.second);
         graph_exec.clear();
@@ -898,6 +904,18 @@ zlh_status zlh_create(const zlh_solver_desc *desc, zlh_ctx **out) {
     ctx->q = desc->dims == 2 ? 9 : 19;
     ctx->lv.resize(desc->max_level + 1);
     if (hipStreamCreate(&ctx->stream) != hipSuccess) return ZLH_EDEVICE;
+    const int nl = desc->max_level + 1;
+    ctx->lstream.assign(nl, nullptr);
+    ctx->ev_step.assign(nl, nullptr);
+    ctx->ev_xfer.assign(nl, nullptr);
+    ctx->ev_join.assign(nl, nullptr);
+    if (hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess) return ZLH_EDEVICE;
+    for (int l = 0; l < nl; ++l)
+        if (hipStreamCreate(&ctx->lstream[l]) != hipSuccess ||
+            hipEventCreateWithFlags(&ctx->ev_step[l], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&ctx->ev_xfer[l], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&ctx->ev_join[l], hipEventDisableTiming) != hipSuccess)
+            return ZLH_EDEVICE;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ctx->n_cu < 1)
@@ -913,6 +931,12 @@ void zlh_destroy(zlh_ctx *ctx) {
     for (auto &e : ctx->events) { (void)hipEventDestroy(e.start); (void)hipEventDestroy(e.stop); }
     ctx->drop_graphs();
     ctx->lv.clear();
+    for (hipStream_t s : ctx->lstream)
+        if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); }
+    for (auto *v : {&ctx->ev_step, &ctx->ev_xfer, &ctx->ev_join})
+        for (hipEvent_t e : *v)
+            if (e) (void)hipEventDestroy(e);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1071,17 +1095,61 @@ zlh_status march_launches(zlh_ctx *ctx, int nsteps) {
         if (l < ctx->d.max_level) { rec(l + 1); rec(l + 1); }
     };
     rec(0);
-    for (int t = 0; t < nsteps; ++t) {
-        std::vector<int> tsl(ctx->d.max_level + 1, 0);
+    const int nl = ctx->d.max_level + 1;
+    // InformationFromGridOfDifferentLevel (lbm_solver_interface.cpp:224-238): refined levels
+    // exchange with their parent when their substep count is even.  With level streams, level l
+    // runs on lstream[l]; the transfer into level l (on lstream[l]) waits for the latest step of
+    // level l - 1 (and for the latest transfer into level l + 1 if the two share nodes of level
+    // l), and level l - 1's next step waits for it.  Nothing else is shared between levels, so a
+    // coarse level's step overlaps the finer levels' substeps.  Timed runs keep one stream, so
+    // that per-launch events time a kernel alone.
+    const bool split = ctx->level_streams && !ctx->timing && nl > 1;
+    hipStream_t main = ctx->stream;
+    if (split) {
+        ZLH_TRY(hipEventRecord(ctx->ev_fork, main));
+        for (int l = 0; l < nl; ++l) ZLH_TRY(hipStreamWaitEvent(ctx->lstream[l], ctx->ev_fork, 0));
+    }
+    std::vector<uint8_t> after_xfer(nl, 0);   // level l's next step must wait for ev_xfer[l + 1]
+    zlh_status s = ZLH_OK;
+    hipError_t e = hipSuccess;
+    for (int t = 0; t < nsteps && !s && e == hipSuccess; ++t) {
+        std::vector<int> tsl(nl, 0);
         for (int l : order) {
             ++tsl[l];
-            zlh_status s = zlh_step_level(ctx, l, tsl[l]);
-            if (s) return s;
-            // InformationFromGridOfDifferentLevel (lbm_solver_interface.cpp:224-238): refined
-            // levels exchange with their parent when their substep count is even
-            if (l > 0 && tsl[l] % 2 == 0 && (s = zlh_transfer_levels(ctx, l))) return s;
+            if (split) {
+                ctx->stream = ctx->lstream[l];
+                if (after_xfer[l] && (e = hipStreamWaitEvent(ctx->stream, ctx->ev_xfer[l + 1], 0)) != hipSuccess) break;
+                after_xfer[l] = 0;
+            }
+            if ((s = zlh_step_level(ctx, l, tsl[l]))) break;
+            if (split && (e = hipEventRecord(ctx->ev_step[l], ctx->stream)) != hipSuccess) break;
+            if (l > 0 && tsl[l] % 2 == 0) {
+                if (split && (e = hipStreamWaitEvent(ctx->stream, ctx->ev_step[l - 1], 0)) != hipSuccess) break;
+                // and after the latest transfer into level l + 1, as in the serial order, when the
+                // two touch common nodes of level l (interfaces that come within a stencil)
+                if (split && l < nl - 1 && ctx->lv[l].xfer_overlap &&
+                    (e = hipStreamWaitEvent(ctx->stream, ctx->ev_xfer[l + 1], 0)) != hipSuccess)
+                    break;
+                if ((s = zlh_transfer_levels(ctx, l))) break;
+                if (split) {
+                    if ((e = hipEventRecord(ctx->ev_xfer[l], ctx->stream)) != hipSuccess) break;
+                    after_xfer[l - 1] = 1;
+                }
+            }
         }
     }
+    if (split) {
+        // join every level stream back into the caller's stream (also on failure, so that a
+        // capture can end)
+        ctx->stream = main;
+        for (int l = 0; l < nl; ++l) {
+            const hipError_t ej = hipEventRecord(ctx->ev_join[l], ctx->lstream[l]);
+            const hipError_t ew = ej == hipSuccess ? hipStreamWaitEvent(main, ctx->ev_join[l], 0) : ej;
+            if (e == hipSuccess) e = ew;
+        }
+    }
+    if (s) return s;
+    if (e != hipSuccess) return ctx->fail(ZLH_EDEVICE, "level streams: %s", hipGetErrorString(e));
     return ZLH_OK;
 }
 
@@ -1139,6 +1207,13 @@ zlh_status zlh_graphs_enable(zlh_ctx *ctx, int enable) {
     if (!ctx) return ZLH_EINVAL;
     ctx->use_graphs = enable != 0;
     if (!ctx->use_graphs) ctx->drop_graphs();
+    return ZLH_OK;
+}
+
+zlh_status zlh_level_streams_enable(zlh_ctx *ctx, int enable) {
+    if (!ctx) return ZLH_EINVAL;
+    ctx->level_streams = enable != 0;
+    ctx->drop_graphs();   // captured graphs hold the other ordering
     return ZLH_OK;
 }
 
@@ -1258,7 +1333,7 @@ zlh_status zlh_transfer_info(zlh_ctx *ctx, int level, int64_t *out) {
     if (level < 0 || level > ctx->d.max_level) return ctx->fail(ZLH_EINVAL, "bad level");
     const TransferTables &X = ctx->lv[level].xfer;
     const int64_t v[ZLH_XFER_INFO_N] = {X.n_c2f, X.n_src, X.n_f2c, X.n_groups, X.staged_rows, X.n_contrib,
-                                        X.n_fallback, (int64_t)X.lds_bytes};
+                                        X.n_fallback, (int64_t)X.lds_bytes, ctx->lv[level].xfer_overlap ? 1 : 0};
     for (int i = 0; i < ZLH_XFER_INFO_N; ++i) out[i] = level > 0 ? v[i] : 0;
     return ZLH_OK;
 }

@@ -41,7 +41,7 @@ EXPORTED = (
     "zlh_create", "zlh_destroy", "zlh_last_error", "zlh_set_level_nodes", "zlh_set_interface_layer",
     "zlh_ib_set_markers", "zlh_build", "zlh_init_feq", "zlh_init_macros", "zlh_upload_populations",
     "zlh_step_level", "zlh_transfer_levels", "zlh_time_marching", "zlh_download_populations", "zlh_download_macros",
-    "zlh_download_source_table", "zlh_ib_download_forces", "zlh_conservation", "zlh_level_info", "zlh_transfer_info",
+    "zlh_download_source_table", "zlh_ib_download_forces", "zlh_conservation", "zlh_level_info", "zlh_transfer_info", "zlh_level_streams_enable",
     "zlh_synchronize", "zlh_timing_enable", "zlh_timing_read", "zlh_mlp_set", "zlh_mlp_gradient",
     "zlh_mlp_train", "zlh_mlp_eval", "zlh_closure_features", "zlh_graphs_enable",
 )
@@ -89,6 +89,7 @@ def lib():
         L.zlh_mlp_eval.argtypes = [P, F32, C.c_int64, C.c_int, F32]
         L.zlh_closure_features.argtypes = [P, C.c_int, F32, F32, D(np.int32)]
         L.zlh_graphs_enable.argtypes = [P, C.c_int]
+        L.zlh_level_streams_enable.argtypes = [P, C.c_int]
         _lib = L
     return _lib
 
@@ -205,10 +206,10 @@ class Solver:
 
     def transfer_info(self, level) -> dict:
         """Interpolation tables of a refined level: sizes of the gather's staged data."""
-        out = (C.c_int64 * 8)()
+        out = (C.c_int64 * 9)()
         self._chk(lib().zlh_transfer_info(self.h, level, out))
         keys = ("ghost_nodes", "source_rows", "f2c_nodes", "groups", "staged_rows", "contributors",
-                "fallback_contributors", "lds_bytes")
+                "fallback_contributors", "lds_bytes", "overlaps_finer_transfer")
         return dict(zip(keys, list(out)))
 
     def f(self, level=0) -> np.ndarray:
@@ -274,6 +275,10 @@ class Solver:
 
     def graphs_enable(self, on=True):
         self._chk(lib().zlh_graphs_enable(self.h, 1 if on else 0))
+
+    def level_streams_enable(self, on=True):
+        """Run each level's substeps on a stream of its own (default on); results are identical."""
+        self._chk(lib().zlh_level_streams_enable(self.h, 1 if on else 0))
 
     def synchronize(self):
         self._chk(lib().zlh_synchronize(self.h))
