@@ -658,7 +658,8 @@ void k_stream_collide_mlp(int64_t nbrick, int64_t naux, const uint64_t *__restri
 // VAR as in the fused kernel; with the FP32 learned closure the workgroup stages the network in
 // LDS first.
 template <int D, typename T, typename C, bool SHIFT, int COLL, int VAR>
-__global__ __launch_bounds__(256) void k_aux_collide(BrickLevel<T> lv, const T *__restrict__ aux_f,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mlp_on_mfma<C, VAR>() ? ((VAR & 1) ? 3 : 4) : 1)))
+void k_aux_collide(BrickLevel<T> lv, const T *__restrict__ aux_f,
                                                      const double *__restrict__ aux_force,
                                                      T *__restrict__ fa, double *__restrict__ aux_macro,
                                                      const uint8_t *__restrict__ macro_flag, StepParams p) {
