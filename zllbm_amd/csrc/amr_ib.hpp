@@ -71,8 +71,8 @@ __global__ void k_c2f_sources(int64_t n_src, int64_t n_fb, const int32_t *src_ai
 // Lagrange weights are products of -1/16, 9/16, 1/2 and 1, dyadic rationals exact in FP16 (the
 // host checks every one).
 constexpr int kGatherDirs = 5;
-constexpr size_t kGatherLdsBudget = 24 * 1024;
-constexpr int kGatherMaxRows = 768;   // staged rows of one group (entry table in static LDS)
+constexpr size_t kGatherLdsBudget = 20 * 1024;
+constexpr int kGatherMaxRows = 768;   // staged rows of one group (row offsets fit the word's 16 bits)
 constexpr int64_t kGatherChunk = 2048;   // ghost nodes sorted by contributor count together
 template <int D, typename T, bool SHIFT>
 __global__ __launch_bounds__(256) void k_c2f_gather(int64_t n_groups, const GatherGroup *__restrict__ groups,
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void k_c2f_gather(int64_t n_groups, const Gath
     }
     // the rows as one flat run of nr * Q elements, consecutive threads on consecutive elements,
     // so a wavefront's load covers a few cache lines of src_f instead of 64
-    constexpr int kBatch = 16;
+    constexpr int kBatch = 8;
     const int ne = nr * Q;
     for (int eb = 0; eb < ne; eb += kBatch * 256) {
         int32_t g[kBatch];
