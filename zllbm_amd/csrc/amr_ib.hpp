@@ -35,28 +35,38 @@ __device__ __forceinline__ void macro_feq_double(const double *f, int compressib
 // and a plain copy of every fine node that stands in for a missing coarse one.  The rows are
 // stored node-major, [n_src + n_fb][Q], in the populations' storage form (FP64, or FP32 shifted by
 // w), so that each stencil contributor is one contiguous row; both row sets are sorted by aux
-// index, so consecutive threads read neighbouring entries of each direction's plane.
+// index, so consecutive threads read neighbouring entries of each direction's plane.  A
+// workgroup's 256 rows go out through LDS as one contiguous, coalesced run.
 template <int D, typename T, bool SHIFT>
-__global__ void k_c2f_sources(int64_t n_src, int64_t n_fb, const int32_t *src_ai, const T *aux_c, int64_t naux_c,
-                              const int32_t *fb_ai, const T *aux_f, int64_t naux_f, double tau_ratio,
-                              int compressible, T *src_f) {
+__global__ __launch_bounds__(256) void k_c2f_sources(int64_t n_src, int64_t n_fb, const int32_t *src_ai,
+                                                     const T *aux_c, int64_t naux_c, const int32_t *fb_ai,
+                                                     const T *aux_f, int64_t naux_f, double tau_ratio,
+                                                     int compressible, T *src_f) {
     constexpr int Q = Lattice<D>::Q;
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ T tile[256 * Q];   // row-major; the odd stride Q keeps the row writes conflict-free
+    const int64_t r0 = (int64_t)blockIdx.x * 256;
+    const int64_t r = r0 + threadIdx.x;
+    const int64_t n = n_src + n_fb;
     if (r < n_src) {
         const int64_t j = src_ai[r];
         double f[Q], feq[Q], rho, u[3];
+#pragma unroll
         for (int q = 0; q < Q; ++q) f[q] = double(aux_c[q * naux_c + j]) + (SHIFT ? Lattice<D>::w[q] : 0.);
         macro_feq_double<D>(f, compressible, rho, u, feq);
+#pragma unroll
         for (int q = 0; q < Q; ++q)
-            src_f[r * Q + q] = T(feq[q] + tau_ratio * (f[q] - feq[q]) - (SHIFT ? Lattice<D>::w[q] : 0.));
-    } else if (r < n_src + n_fb) {
+            tile[threadIdx.x * Q + q] = T(feq[q] + tau_ratio * (f[q] - feq[q]) - (SHIFT ? Lattice<D>::w[q] : 0.));
+    } else if (r < n) {
         const int64_t fi = fb_ai[r - n_src];
         T v[Q];
 #pragma unroll
         for (int q = 0; q < Q; ++q) v[q] = aux_f[q * naux_f + fi];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) src_f[r * Q + q] = v[q];
+        for (int q = 0; q < Q; ++q) tile[threadIdx.x * Q + q] = v[q];
     }
+    __syncthreads();
+    const int rows = (int)(n - r0 < 256 ? n - r0 : 256);
+    for (int e = threadIdx.x; e < rows * Q; e += 256) src_f[r0 * Q + e] = tile[e];
 }
 
 // InterpolationLagrangian{2D,3D} (grid/grid_interpolation.cpp:224-304, 433-525): each ghost node
