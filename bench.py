@@ -129,6 +129,8 @@ def main():
     ap.add_argument("--l2-steps", type=int, default=1000)
     ap.add_argument("--l2-every", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--one-stream", action="store_true",
+                    help="time marching on one stream (levels do not overlap; for per-kernel traces)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (for counter calibration)")
     args = ap.parse_args()
@@ -160,6 +162,8 @@ def main():
         les = hip.LES_SMAGORINSKY
     log("workload:", W.name, "nodes per level", W.nodes)
     s = make_solver(hip, W, hip.FP32_SHIFTED, les, model)
+    if args.one_stream:
+        s.level_streams_enable(False)
     log("aux nodes per level", [s.info(l)[1] for l in range(len(W.nodes))])
     wall, k_ms, k_n, _ = timed(s, W, args.warmup, args.steps)
     mlups = W.updates_per_step * args.steps / wall / 1e6
