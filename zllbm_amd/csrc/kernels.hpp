@@ -104,7 +104,7 @@ __device__ __forceinline__ void macro(const C (&f)[Lattice<D>::Q], const C *F, C
         if (D == 3) jz += C(0.5) * F[2] * dt;
     }
     if (compressible) {
-        C inv = C(1) / rho;
+        C inv = recip(rho);
         u[0] = jx * inv; u[1] = jy * inv; u[2] = D == 3 ? jz * inv : C(0);
     } else {
         u[0] = jx; u[1] = jy; u[2] = D == 3 ? jz : C(0);
@@ -228,7 +228,7 @@ __device__ __forceinline__ void collide_mrt(C (&f)[Lattice<D>::Q], const C *F, c
     }
     const C rhoc = p.compressible ? rho : C(1);
     {
-        const C inv = C(1) / rhoc;
+        const C inv = recip(rhoc);
         u[0] = p.compressible ? j[0] * inv : j[0];
         u[1] = p.compressible ? j[1] * inv : j[1];
         u[2] = D == 3 ? (p.compressible ? j[2] * inv : j[2]) : C(0);
@@ -300,7 +300,7 @@ __device__ __forceinline__ void collide_mrt(C (&f)[Lattice<D>::Q], const C *F, c
     for (int k = 0; k < Q; ++k)
         if (!L::viscous_row[k]) post(k, C(L::s_base[k]), L::s_base[k] != 0.);
     const C tau = les ? relaxation_time<C>(les, pi, rho, drho, u, p, ops) : C(p.tau0);
-    const C s_nu = C(1) / tau;
+    const C s_nu = recip(tau);
 #pragma unroll
     for (int k = 0; k < Q; ++k)
         if (L::viscous_row[k]) post(k, s_nu, true);
@@ -371,7 +371,7 @@ __device__ __forceinline__ void collide(C (&f)[Lattice<D>::Q], const C *F, const
         if (forced) stress_force_terms<D, C>(pi, F, u, dt);
         tau = relaxation_time<C>(les, pi, rho, drho, u, p, ops);
     }
-    const C inv_tau = C(1) / tau;
+    const C inv_tau = recip(tau);
     const C fcoef = (C(1) - C(0.5) * inv_tau) * dt;
     C uF = C(0);
     if (forced) uF = u[0] * F[0] + u[1] * F[1] + (D == 3 ? u[2] * F[2] : C(0));

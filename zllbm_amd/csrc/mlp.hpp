@@ -57,12 +57,20 @@ struct MlpDev {
 __device__ __forceinline__ float sigmoidf_(float z) { return 1.f / (1.f + __expf(-z)); }
 __device__ __forceinline__ float softplusf_(float z) { return z > 20.f ? z : log1pf(__expf(z)); }
 
+// 1 / x: exact division in FP64; in FP32 the hardware reciprocal (about 1 ulp, one instruction
+// where the IEEE-rounded division takes about ten)
+template <typename C>
+__device__ __forceinline__ C recip(C x) {
+    if constexpr (std::is_same<C, float>::value) return __builtin_amdgcn_rcpf(x);
+    else return C(1) / x;
+}
+
 // raw closure inputs from the non-equilibrium stress Pi (xx, yy, zz, xy, xz, yz), density and
 // velocity: P = Pi / rho, a = sqrt(2 P:P), I1 = tr P, I2 = (I1^2 - P:P) / 2, I3 = det P, |u|^2,
 // rho - 1, tau0, a tau0, a^2
 template <typename C>
 __device__ __forceinline__ void closure_features(const C *pi, C rho, C drho, const C *u, C tau0, float *x) {
-    const C ir = C(1) / rho;
+    const C ir = recip(rho);
     const C pxx = pi[0] * ir, pyy = pi[1] * ir, pzz = pi[2] * ir, pxy = pi[3] * ir, pxz = pi[4] * ir, pyz = pi[5] * ir;
     const C pp = pxx * pxx + pyy * pyy + pzz * pzz + C(2) * (pxy * pxy + pxz * pxz + pyz * pyz);
     const C a = sqrt(C(2) * pp);
