@@ -131,15 +131,27 @@ def f16_round(x):
     return np.asarray(x, dtype=np.float64).astype(np.float16).astype(np.float64)
 
 
+SILU_SCALE = -np.log2(np.e)
+
+
 def forward_lowp(p, X, rnd):
     """The network with low-precision operands and wide accumulation, as the matrix-core form
-    evaluates it: inputs, W1, b1, W2, b2 and the layer-1 activations rounded by `rnd` (the biases
-    enter the products as weights of a constant input 1); w3, b3 and the accumulations exact."""
+    evaluates it. W1, b1 and b2 are first scaled by c = -log2(e) (in FP64, stored as FP32), so
+    both layers' pre-activations are t = c z and an activation is t / (1 + 2^t) = c silu(z).
+    Inputs, the scaled W1 / b1 / b2, W2 and the layer-1 activations are rounded by `rnd` (the
+    biases enter the products as weights of a constant input 1); w3 / c, b3 and the
+    accumulations are exact. Up to those roundings this is forward()."""
     W1, b1, W2, b2, w3, b3 = unpack(p)
-    Z1 = rnd(X) @ rnd(W1).T + rnd(b1)
-    H1 = rnd(silu(Z1))
-    Z2 = H1 @ rnd(W2).T + rnd(b2)
-    return softplus(silu(Z2) @ w3 + b3)
+    c = SILU_SCALE
+
+    def sc(a):
+        return (c * np.asarray(a, np.float64)).astype(np.float32).astype(np.float64)
+
+    with np.errstate(over="ignore"):          # 2^t -> inf gives the activation 0, as on the device
+        T1 = rnd(X) @ rnd(sc(W1)).T + rnd(sc(b1))
+        H1 = rnd(T1 / (1.0 + np.exp2(T1)))
+        T2 = H1 @ rnd(W2).T + rnd(sc(b2))
+        return softplus((T2 / (1.0 + np.exp2(T2))) @ (w3 / c) + b3)
 
 
 def forward_bf16(p, X):

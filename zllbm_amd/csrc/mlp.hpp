@@ -176,10 +176,14 @@ struct OperandsF16 {
     }
 };
 
-// SiLU for BF16 activations: hardware exp2 and reciprocal (about 1 ulp each), far below BF16's
-// rounding; large negative z gives exp2 -> inf, rcp -> 0
-__device__ __forceinline__ float siluf_(float z) {
-    return z * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+// SiLU of the matrix-core form, on pre-activations scaled by c = -log2(e) (the host scales W1, b1
+// and b2 in the operand images): for t = c z, t / (1 + 2^t) = c silu(z).  Hardware exp2 and
+// reciprocal, about 1 ulp each, far below the operands' rounding; a large t gives 2^t -> inf,
+// rcp -> 0.  The next layer takes the scaled activations as they are (its weights are unscaled;
+// the output weights carry 1/c).
+constexpr double kSiluScale = -1.4426950408889634;
+__device__ __forceinline__ float silu_scaled(float t) {
+    return t * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(t));
 }
 
 // value held by lane l ^ 32 (v_permlane32_swap exchanges the halves of a register pair; unlike
@@ -222,8 +226,8 @@ __device__ __forceinline__ float mlp_forward_wave(const MlpOps &o, const float *
     vec g0[3], g1[3];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-        g0[r >> 3][r & 7] = (elem)siluf_(z0[r]);
-        g1[r >> 3][r & 7] = (elem)siluf_(z1[r]);
+        g0[r >> 3][r & 7] = (elem)silu_scaled(z0[r]);
+        g1[r >> 3][r & 7] = (elem)silu_scaled(z1[r]);
     }
 #pragma unroll
     for (int e = 0; e < 8; ++e) g0[2][e] = g1[2][e] = (elem)(h == 0 && e == 0 ? 1.f : 0.f);
@@ -238,8 +242,8 @@ __device__ __forceinline__ float mlp_forward_wave(const MlpOps &o, const float *
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const float ww = o.w3r[h][r];
-        p0 += ww * siluf_(y0[r]);
-        p1 += ww * siluf_(y1[r]);
+        p0 += ww * silu_scaled(y0[r]);
+        p1 += ww * silu_scaled(y1[r]);
     }
     // node l % 32 + 32 h: own half of its block plus the other half's, held by lane l ^ 32
     const float recv = swap_halves(h ? p0 : p1);

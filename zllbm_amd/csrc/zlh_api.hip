@@ -1390,14 +1390,17 @@ zlh_status zlh_mlp_set(zlh_ctx *ctx, const float *params, const float *mean, con
         m.fmax[i] = fmax[i];
     }
     m.nu_scale = (float)nu_scale;
-    // operand images of the wavefront MFMA network (layout in mlp.hpp)
+    // operand images of the wavefront MFMA network (layout in mlp.hpp).  W1, b1 and b2 are scaled
+    // by c = -log2(e) (in FP64, stored as FP32, then rounded to the operand format), so that both
+    // hidden layers' pre-activations arrive as c z; the output weights carry 1/c (silu_scaled)
     const float *W1 = m.p, *b1 = W1 + kMlpH * kMlpF, *W2 = b1 + kMlpH, *b2 = W2 + kMlpH * kMlpH, *w3 = b2 + kMlpH;
+    auto scaled = [](float v) { return (float)(kSiluScale * (double)v); };
     MlpOps &o = m.ops;
     for (int l = 0; l < 64; ++l) {
         const int i = l % 32, h = l / 32;
         for (int e = 0; e < 8; ++e) {
             const int k = 8 * h + e;
-            const float a1 = k < kMlpF ? W1[i * kMlpF + k] : (k == kMlpF ? b1[i] : 0.f);
+            const float a1 = scaled(k < kMlpF ? W1[i * kMlpF + k] : (k == kMlpF ? b1[i] : 0.f));
             o.w1a[l][e] = bf16_bits(a1);
             o.h1a[l][e] = f16_bits(a1);
             for (int st = 0; st < 2; ++st) {
@@ -1405,12 +1408,12 @@ zlh_status zlh_mlp_set(zlh_ctx *ctx, const float *params, const float *mean, con
                 o.w2a[st][l][e] = bf16_bits(a2);
                 o.h2a[st][l][e] = f16_bits(a2);
             }
-            o.w2a[2][l][e] = bf16_bits(k == 0 ? b2[i] : 0.f);
-            o.h2a[2][l][e] = f16_bits(k == 0 ? b2[i] : 0.f);
+            o.w2a[2][l][e] = bf16_bits(k == 0 ? scaled(b2[i]) : 0.f);
+            o.h2a[2][l][e] = f16_bits(k == 0 ? scaled(b2[i]) : 0.f);
         }
     }
     for (int h = 0; h < 2; ++h)
-        for (int r = 0; r < 16; ++r) o.w3r[h][r] = w3[mfma_row(h, r)];
+        for (int r = 0; r < 16; ++r) o.w3r[h][r] = (float)((double)w3[mfma_row(h, r)] / kSiluScale);
     for (int i = 0; i < kMlpF; ++i) {
         o.mean[i] = m.mean[i];
         o.inv_std[i] = m.inv_std[i];
