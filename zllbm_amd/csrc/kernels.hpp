@@ -534,6 +534,9 @@ __device__ __forceinline__ uint32_t pull_to_lds(const BrickTables &t, const T *_
     constexpr int NN = B::nnbr;
     static_assert(sizeof(T) == 4, "dword staging");
     const PullLane<D> P = pull_lane<D>(lane);   // per pull: live only while the addresses form
+    // opaque per pull: the direction's LDS destinations are formed here, not hoisted out of the
+    // caller's loop into 19 long-lived scalars
+    asm volatile("" : "+s"(buf));
     int32_t nb[NN];
 #pragma unroll
     for (int k = 0; k < NN; ++k) {
@@ -622,11 +625,14 @@ void k_stream_collide_mlp(int64_t nbrick, int64_t naux, const uint64_t *__restri
         const uint64_t am = t.auxmask[b];
         if ((am >> lane) & 1ull) {
             // deferred node: post-stream values to the aux array, except the missing sources
-            // (their boundary / transfer values are there already)
-            const int64_t ai = t.auxbase[b] + __popcll(am & ((1ull << lane) - 1ull));
+            // (their boundary / transfer values are there already): those go to the spare slot
+            // at index Q naux, so the stores need no branch.  32-bit element indices (the host
+            // keeps Q naux + 1 below 2^31).
+            const uint32_t ai = (uint32_t)(t.auxbase[b] + __popcll(am & ((1ull << lane) - 1ull)));
+            const uint32_t spare = (uint32_t)(Q * naux);
 #pragma unroll
             for (int q = 0; q < Q; ++q)
-                if (!((miss >> q) & 1u)) aux_f[q * naux + ai] = T(f[q]);
+                aux_f[((miss >> q) & 1u) ? spare : (uint32_t)(q * naux) + ai] = T(f[q]);
         }
         // the buffer's reads have returned before it is refilled
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");

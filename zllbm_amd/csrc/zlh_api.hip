@@ -867,7 +867,10 @@ zlh_status build_level_aux(zlh_ctx *ctx, int level) {
     const size_t ts = ctx->tsize();
     if ((s = dalloc(ctx, L.f_d[0], ts * Q * nslot))) return s;
     if ((s = dalloc(ctx, L.f_d[1], ts * Q * nslot))) return s;
-    if ((s = dalloc(ctx, L.aux_f_d, ts * Q * std::max<int64_t>(L.naux, 1)))) return s;
+    // one spare element past [Q][naux]: the learned-closure kernel parks a deferred node's
+    // missing directions there instead of branching around their stores
+    if ((int64_t)Q * L.naux + 1 >= ((int64_t)1 << 31)) return ctx->fail(ZLH_ENOMEM, "level %d: aux array too large", level);
+    if ((s = dalloc(ctx, L.aux_f_d, ts * (Q * std::max<int64_t>(L.naux, 1) + 1)))) return s;
     if ((s = dalloc(ctx, L.aux_macro_d, sizeof(double) * 4 * std::max<int64_t>(L.naux, 1)))) return s;
     if ((s = dalloc(ctx, L.aux_macro_flag_d, std::max<int64_t>(L.naux, 1)))) return s;
     if ((s = dalloc(ctx, L.aux_force_d, sizeof(double) * 3 * std::max<int64_t>(L.naux, 1)))) return s;
