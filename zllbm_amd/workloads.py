@@ -7,6 +7,15 @@
   speed 0.05/sqrt(3) per component); each level is refined where |vorticity| at its cell centres
   exceeds the 80th percentile of |vorticity| over the box (percentile from a 64^3 sample).
   base 32, levels 4 is the 256^3-equivalent configuration.
+* ``cavity(n)``: D2Q9 SRT lid-driven cavity, n x n cells, Re 100 on the lid speed 0.1, FP64 at rest.
+* ``cylinder_amr()``: D2Q9 MRT channel of 512 x 256 background cells with two nested refinement
+  boxes around a cylinder of 32 finest cells diameter (101 direct-forcing markers on the finest
+  level), Re 100 on the diameter; uniform inflow 0.05 (non-equilibrium extrapolation in and out,
+  bounce-back walls).
+* ``sphere_amr()``: D3Q19 MRT-LES channel of 32 x 16 x 16 background cells (512 x 256 x 256 on
+  the finest of 5 levels), each level refined within a band around a sphere of 48 finest cells
+  diameter (7,239 Fibonacci markers on the finest level), Re 300, inflow 0.05 with a seeded 1 %
+  velocity perturbation.
 """
 from __future__ import annotations
 
@@ -18,7 +27,7 @@ import numpy as np
 from zllbm_amd import grid as g
 from zllbm_amd.fields import FourierModes, fourier_modes
 
-PER = g.BC_PERIODIC
+PER, BB, NEE = g.BC_PERIODIC, g.BC_BOUNCE_BACK, g.BC_NEE
 
 
 def periodic_bc(dims):
@@ -87,3 +96,78 @@ def decaying_turbulence(base=32, levels=4, nu=None, seed=20261202, percentile=80
     name = "decaying turbulence, %d levels over a %d^3 periodic box (%d^3-equivalent), 64 Fourier modes" % (
         levels, base, base << (levels - 1))
     return Workload(name, grid, solver, init, modes)
+
+
+@dataclass
+class IbSet:
+    level: int
+    xyz: np.ndarray
+    area: np.ndarray
+
+
+def _channel_bc(dims, u_in):
+    bc = {g.XMIN: (NEE, (u_in, 0., 0.)), g.XMAX: (NEE, (u_in, 0., 0.))}
+    for d in range(3, 2 * dims + 1):
+        bc[d] = (BB, (0., 0., 0.))
+    return bc
+
+
+def cavity(n=128, lid=0.1, re=100.0):
+    grid = g.uniform_grid(2, (n, n))
+    bc = {d: (BB, (0., 0., 0.)) for d in range(1, 5)}
+    bc[g.YMAX] = (BB, (lid, 0., 0.))
+    nu = lid * (n + 1) / re
+    init = [(np.ones(grid.levels[0].n), np.zeros((grid.levels[0].n, 2)))]
+    return Workload("D2Q9 SRT lid-driven cavity %dx%d, Re %g" % (n, n, re), grid,
+                    dict(nu0=nu, bc=bc, collision=(0,)), init)
+
+
+def cylinder_amr(cells=(512, 256), diameter_fine=32, u_in=0.05, re=100.0):
+    """Background units: node index i of level l sits at i / 2^l, the domain spans [1, cells + 1]."""
+    levels = 3
+    d_bg = diameter_fine / (1 << (levels - 1))
+    cx, cy = cells[0] / 4.0 + 1.0, cells[1] / 2.0 + 1.0
+    c0 = np.array([int(cx), int(cy)])
+    box1 = (tuple(c0 - 24), tuple(c0 + 24))                 # level-0 node indices
+    box2 = (tuple(2 * c0 - 20), tuple(2 * c0 + 20))         # level-1 node indices
+    grid = g.refined_box_grid(2, cells, [box1, box2])
+    nu = u_in * d_bg / re
+    xy, area = _geometry().circle_markers((cx + 0.13, cy + 0.07), d_bg / 2.0, 1.0 / (1 << (levels - 1)))
+    init = [(np.ones(lev.n), np.tile([u_in, 0.0], (lev.n, 1))) for lev in grid.levels]
+    wl = Workload("D2Q9 MRT channel %dx%d, 3 levels, cylinder D=%d finest cells, Re %g" % (cells + (diameter_fine, re)),
+                  grid, dict(nu0=nu, bc=_channel_bc(2, u_in), collision=(1,), has_force=True), init)
+    wl.ib = IbSet(levels - 1, xy, area)
+    return wl
+
+
+def sphere_amr(cells=(32, 16, 16), levels=5, diameter_fine=48, u_in=0.05, re=300.0, perturb=0.01,
+               band=8.0, margin=3, les=2, seed=20261202):
+    """Level k is refined where a level-k cell centre lies within r + band level-k cells of the
+    sphere centre, at least ``margin`` level-k cells inside level k's region (refined_mask_grid;
+    the 16-cell cross-section leaves room for 5 nested levels around the sphere with margin 3,
+    the reference's minimum extension)."""
+    d_bg = diameter_fine / (1 << (levels - 1))
+    r = d_bg / 2.0
+    ctr = np.array([cells[0] / 4.0, cells[1] / 2.0, cells[2] / 2.0])   # from the domain minimum
+
+    def refine(k, c):
+        return np.linalg.norm(c - ctr, axis=1) < r + band / (1 << k)
+
+    grid = g.refined_mask_grid(3, cells, levels, refine, margin=margin)
+    nu = u_in * d_bg / re
+    xyz, area = _geometry().fibonacci_sphere_markers(tuple(ctr + 1.0 + 0.013), r, 1.0 / (1 << (levels - 1)))
+    rng = np.random.default_rng(seed)
+    init = []
+    for lev in grid.levels:
+        u = np.tile([u_in, 0.0, 0.0], (lev.n, 1)) + rng.normal(0.0, perturb * u_in, (lev.n, 3))
+        init.append((np.ones(lev.n), u))
+    wl = Workload("D3Q19 MRT-LES channel %dx%dx%d (%d levels), sphere D=%d finest cells, Re %g" % (
+        cells + (levels, diameter_fine, re)), grid,
+        dict(nu0=nu, bc=_channel_bc(3, u_in), collision=(1,), les=les, les_tau_mode=1, cs=0.1, has_force=True), init)
+    wl.ib = IbSet(levels - 1, xyz, area)
+    return wl
+
+
+def _geometry():
+    from zllbm_amd import geometry
+    return geometry
