@@ -6,9 +6,12 @@ For every configuration:
 * the FP32 shifted-population solver against the FP64 solver, both on the GPU, stepped in lockstep
   from the same initial state at the configuration's full size;
 * the FP64 GPU solver against the FP64 oracle on the CPU where the oracle finishes in a minute or
-  so: configurations 1 and 2 at full size for fewer steps, and configuration 4's closure on a
+  so: configurations 1 and 2 at full size and length, and configuration 4's closure on a
   2-level 24^3 box. (Elsewhere the parity tests pin the FP64 solver to the oracle at smaller
   sizes, to 1e-11.)
+
+Throughput: MLUPS (node updates of all levels per background step) of the FP32 and FP64 forms,
+from graph-replayed background steps with the populations resident on the GPU.
 
 Errors: velocity relative L2, sqrt(sum |u_a - u_b|^2 / sum |u_b|^2) over the nodes of all levels,
 and density RMS difference. One JSON line per configuration on stdout; progress on stderr.
@@ -66,13 +69,18 @@ def oracle_solver(W, model=None):
     return o
 
 
-def gpu_state(s, W):
-    out = [s.macros(l) for l in range(W.grid.max_level + 1)]
-    return np.concatenate([r for r, _ in out]), np.concatenate([u for _, u in out])
+def macros(f, dims):
+    """Density and velocity of post-stream populations [n][Q] (FP64, unshifted), the same
+    moments for both sides."""
+    from oracle.zlbm_oracle import lattice
+    cx, cy, cz, _, _ = (np.asarray(v, np.float64) for v in lattice(dims))
+    rho = f.sum(1)
+    u = np.stack([f @ cx, f @ cy] + ([f @ cz] if dims == 3 else []), axis=1) / rho[:, None]
+    return rho, u
 
 
-def oracle_state(o, W):
-    out = [o.node_macro(l) for l in range(W.grid.max_level + 1)]
+def state(sim, W):
+    out = [macros(sim.f(l), W.grid.dims) for l in range(W.grid.max_level + 1)]
     return np.concatenate([r for r, _ in out]), np.concatenate([u for _, u in out])
 
 
@@ -89,7 +97,7 @@ def fp32_vs_fp64(W, steps, every, model=None):
     for t in range(every, steps + 1, every):
         a.time_marching(every)
         b.time_marching(every)
-        lu, lr = errors(gpu_state(a, W), gpu_state(b, W))
+        lu, lr = errors(state(a, W), state(b, W))
         curve.append((t, lu, lr))
         log("  FP32 vs FP64 after %6d steps: L2(u) %.3e  RMS(rho) %.3e" % (t, lu, lr))
     return curve
@@ -102,9 +110,26 @@ def gpu_vs_oracle(W, steps, model=None):
     o.time_marching(steps)
     t_cpu = time.perf_counter() - t0
     s.time_marching(steps)
-    lu, lr = errors(gpu_state(s, W), oracle_state(o, W))
+    lu, lr = errors(state(s, W), state(o, W))
     log("  FP64 GPU vs FP64 oracle after %d steps: L2(u) %.3e  RMS(rho) %.3e (oracle %.1f s)" % (steps, lu, lr, t_cpu))
     return {"steps": steps, "l2_u": lu, "rms_rho": lr, "oracle_seconds": round(t_cpu, 1)}
+
+
+def throughput(W, steps, model=None):
+    out = {}
+    for name, prec in (("fp32", hip.FP32_SHIFTED), ("fp64", hip.FP64)):
+        s = gpu_solver(W, prec, model)
+        s.time_marching(10)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.time_marching(steps)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        out[name + "_mlups"] = round(W.updates_per_step * steps / wall / 1e6, 1)
+        out[name + "_ms_per_step"] = round(wall / steps * 1e3, 4)
+        del s
+    log("  MLUPS FP32 %.1f, FP64 %.1f" % (out["fp32_mlups"], out["fp64_mlups"]))
+    return out
 
 
 def summary(curve):
@@ -122,19 +147,22 @@ def main():
     if 1 in want:
         W = workloads.cavity(128)
         log("config 1:", W.name, W.nodes)
-        rec = {"config": 1, "workload": W.name, "fp32_vs_fp64": summary(fp32_vs_fp64(W, 10000, 2000)),
-               "fp64_vs_oracle": gpu_vs_oracle(W, 2000)}
+        rec = {"config": 1, "workload": W.name, "throughput": throughput(W, 2000),
+               "fp32_vs_fp64": summary(fp32_vs_fp64(W, 10000, 2000)),
+               "fp64_vs_oracle": gpu_vs_oracle(W, 10000)}
         print(json.dumps(rec), flush=True)
     if 2 in want:
         W = workloads.cylinder_amr()
         log("config 2:", W.name, W.nodes)
-        rec = {"config": 2, "workload": W.name, "fp32_vs_fp64": summary(fp32_vs_fp64(W, 2000, 500)),
-               "fp64_vs_oracle": gpu_vs_oracle(W, 100)}
+        rec = {"config": 2, "workload": W.name, "throughput": throughput(W, 400),
+               "fp32_vs_fp64": summary(fp32_vs_fp64(W, 2000, 500)),
+               "fp64_vs_oracle": gpu_vs_oracle(W, 2000)}
         print(json.dumps(rec), flush=True)
     if 3 in want:
         W = workloads.tgv(256)
         log("config 3:", W.name, W.nodes)
-        rec = {"config": 3, "workload": W.name, "fp32_vs_fp64": summary(fp32_vs_fp64(W, 2000, 500))}
+        rec = {"config": 3, "workload": W.name, "throughput": throughput(W, 200),
+               "fp32_vs_fp64": summary(fp32_vs_fp64(W, 2000, 500))}
         print(json.dumps(rec), flush=True)
     if 4 in want:
         W = workloads.decaying_turbulence(32, 4)
@@ -142,13 +170,15 @@ def main():
         log("config 4:", W.name, W.nodes)
         small = workloads.decaying_turbulence(24, 2)
         small.solver["les"] = hip.LES_MLP
-        rec = {"config": 4, "workload": W.name, "fp32_vs_fp64": summary(fp32_vs_fp64(W, 1000, 250, model)),
+        rec = {"config": 4, "workload": W.name, "throughput": throughput(W, 200, model),
+               "fp32_vs_fp64": summary(fp32_vs_fp64(W, 1000, 250, model)),
                "fp64_vs_oracle": dict(gpu_vs_oracle(small, 20, model), workload=small.name)}
         print(json.dumps(rec), flush=True)
     if 5 in want:
         W = workloads.sphere_amr()
         log("config 5:", W.name, W.nodes)
-        rec = {"config": 5, "workload": W.name, "fp32_vs_fp64": summary(fp32_vs_fp64(W, 400, 100, model))}
+        rec = {"config": 5, "workload": W.name, "throughput": throughput(W, 200, model),
+               "fp32_vs_fp64": summary(fp32_vs_fp64(W, 400, 100, model))}
         print(json.dumps(rec), flush=True)
 
 
