@@ -92,6 +92,13 @@ zlh_status zlh_set_interface_layer(zlh_ctx *ctx, int level, int kind, int layer,
  * coordinate_/velocity_/area_, immersed_boundary.h:23-28); xyz and vel are [n][3] */
 zlh_status zlh_ib_set_markers(zlh_ctx *ctx, int level, const double *xyz, const double *area,
                               const double *vel, int64_t n, int stencil_dis, double stiffness);
+/* velocity profile of a non-equilibrium-extrapolation face, before zlh_build: 0 uniform (the
+ * reference's one velocity per face, BoundaryNonEqExtrapolation::SetValues), 1 parabolic: the face
+ * velocity times the product over the transverse axes of 4 t (1 - t), t the node's position across
+ * the domain (no reference counterpart; a channel inflow) */
+#define ZLH_PROFILE_UNIFORM 0
+#define ZLH_PROFILE_PARABOLIC 1
+zlh_status zlh_set_boundary_profile(zlh_ctx *ctx, int dir, int profile);
 /* device slot and neighbour tables, boundary / interface / IB tables
  * (replaces the per-access unordered_map lookups of lbm_d3q19_solver.cpp:20-135) */
 zlh_status zlh_build(zlh_ctx *ctx);

@@ -66,6 +66,7 @@ struct zo_sim {
     double mlp_p[1409], mlp_mean[9], mlp_istd[9], mlp_min[9], mlp_max[9], mlp_scale;
     int mlp_set;
     int64_t mlp_fallbacks;
+    int bc_profile[7];   /* per face: 0 uniform boundary velocity, 1 parabolic (zo_set_bc_profile) */
 };
 
 static void zo_seterr(zo_sim *s, const char *fmt, ...) {
@@ -700,19 +701,47 @@ static void zo_bc_periodic(zo_sim *s, int level, int dir) {
     }
 }
 
+/* Parabolic inflow (no reference counterpart; the reference's faces take one velocity): the face
+ * velocity times the product over the transverse axes of 4 t (1 - t), t the node's position across
+ * the domain, 0 and 1 on the walls, so the profile peaks at the face velocity on the centre line
+ * and vanishes on the walls. */
+static double zo_profile_factor(const zo_sim *s, int level, int axis, uint64_t key) {
+    uint32_t c[3] = {0, 0, 0};
+    zo_decode(s->p.dims, key, c);
+    double fac = 1.0;
+    for (int a = 0; a < s->p.dims; ++a) {
+        if (a == axis) continue;
+        double lo = (double)((int64_t)s->p.dom_min[a] << level), hi = (double)((int64_t)s->p.dom_max[a] << level);
+        double t = ((double)c[a] - lo) / (hi - lo);
+        fac *= 4.0 * t * (1.0 - t);
+    }
+    return fac;
+}
+
+int zo_set_bc_profile(zo_sim *s, int dir, int profile) {
+    if (dir < 1 || dir > 2 * s->p.dims || profile < 0 || profile > 1) {
+        zo_seterr(s, "bad boundary profile %d on face %d", profile, dir);
+        return -1;
+    }
+    s->bc_profile[dir] = profile;
+    return 0;
+}
+
 static void zo_bc_nee(zo_sim *s, int level, int dir) {
     zo_level *L = &s->lv[level];
     int q = s->q, dims = s->p.dims;
     int axis = (dir - 1) / 2;
     int step = (dir % 2 == 1) ? +1 : -1;
     double feq_b[ZO_MAXQ], feq_in[ZO_MAXQ], rho_in, u_in[3];
-    double ub[3] = {s->p.bc_value[dir][0], s->p.bc_value[dir][1], s->p.bc_value[dir][2]};
+    const double *bv = s->p.bc_value[dir];
     /* BoundaryNonEqExtrapolation{2D,3D}::CalBoundaryCondition (lbm_noneq_extrapolation.cpp:29-144) */
     for (int64_t b = 0; b < L->nb[dir]; ++b) {
         int64_t i = L->bnd[dir][b];
         uint64_t ik = zo_find_neighbor(dims, L->key[i], axis, step);
         int64_t j = zo_find(L, ik);
         if (j < 0) continue;
+        double fac = s->bc_profile[dir] ? zo_profile_factor(s, level, axis, L->key[i]) : 1.0;
+        double ub[3] = {bv[0] * fac, bv[1] * fac, bv[2] * fac};
         if (s->p.has_force) {
             double F[3];
             zo_all_forces(s, L->force + j * 3, F);

@@ -89,6 +89,9 @@ int zo_time_marching(zo_sim *s, int nsteps);
 /* level order of one background step (grid/grid_manager.cpp:997-1024); returns its length */
 int zo_time_stepping_order(int max_level, int32_t *order, int cap);
 
+/* velocity profile of a non-equilibrium-extrapolation face: 0 uniform (the reference), 1 parabolic
+ * across the transverse axes with the face velocity on the centre line */
+int zo_set_bc_profile(zo_sim *s, int dir, int profile);
 /* immersed boundary markers for one level (immersed_boundary.cpp:35-384) */
 int zo_set_ib(zo_sim *s, int level, int64_t n, const double *xyz, const double *area,
               const double *vel, int stencil_dis, double stiffness, double dx_background);

@@ -63,6 +63,7 @@ def lib():
         L.zo_advance_level.argtypes = [P, C.c_int, C.c_int]
         L.zo_time_marching.argtypes = [P, C.c_int]
         L.zo_time_stepping_order.argtypes = [C.c_int, D(np.int32), C.c_int]
+        L.zo_set_bc_profile.argtypes = [P, C.c_int, C.c_int]
         L.zo_set_ib.argtypes = [P, C.c_int, C.c_int64, D(np.float64), D(np.float64), D(np.float64),
                                 C.c_int, C.c_double, C.c_double]
         L.zo_get_ib_force.argtypes = [P, C.c_int, D(np.float64)]
@@ -111,7 +112,8 @@ class Oracle:
     def __init__(self, grid, *, nu0: float, rho0: float = 1.0, u0=(0., 0., 0.),
                  collision: Sequence[int] = (0,), les: int = 0, les_tau_mode: int = 1,
                  cs: float = 0.1, has_force: bool = False, const_force=(0., 0., 0.),
-                 bc: Optional[Dict[int, tuple]] = None, compressible: bool = True):
+                 bc: Optional[Dict[int, tuple]] = None, compressible: bool = True,
+                 bc_profile: Optional[Dict[int, int]] = None):
         self.grid = grid
         self.dims = grid.dims
         self.q = 9 if grid.dims == 2 else 19
@@ -147,6 +149,8 @@ class Oracle:
             for (kind, layer), keys in lev.layers.items():
                 self._chk(L.zo_set_layer(self.h, l, kind, layer, keys.size,
                                          np.ascontiguousarray(keys, np.uint64)))
+        for d, prof in (bc_profile or {}).items():
+            self._chk(L.zo_set_bc_profile(self.h, d, prof))
         self._chk(L.zo_finalize(self.h))
         self._chk(L.zo_init_feq(self.h))
 

@@ -126,6 +126,18 @@ def test_channel_nee_fp64():
     assert_close(s.f(0), o.f(0))
 
 
+def test_channel_parabolic_inflow_fp64():
+    """Parabolic inflow on the non-equilibrium-extrapolation inlet (per-node boundary velocity),
+    FP64 against the oracle, and the FP32 form close to it."""
+    grid, kw = sc.channel_nee(u_in=0.06)
+    prof = {g.XMIN: hip.PROFILE_PARABOLIC}
+    o, s = run_pair(grid, 150, collision=(hip.MRT,), bc_profile=prof, **kw)
+    assert_close(s.f(0), o.f(0))
+    _, s32 = run_pair(grid, 150, precision=hip.FP32_SHIFTED, collision=(hip.MRT,), bc_profile=prof, **kw)
+    d = s32.f(0) - o.f(0)
+    assert np.sqrt(np.mean(d ** 2)) < 1e-6
+
+
 def test_const_force_fp64():
     grid = g.uniform_grid(3, (6, 14, 6))
     bc = sc.all_bc(3, sc.PER)

@@ -395,3 +395,23 @@ def test_time_stepping_order():
     assert list(zo.time_stepping_order(2)) == [0, 1, 2, 2, 1, 2, 2]      # survey section 3.2
     for ml in range(2, 7):
         assert list(zo.time_stepping_order(ml)) == _reference_order_loop(ml)
+
+
+def test_parabolic_inflow_profile():
+    """A channel at rest: after one step the inlet nodes carry feq(rho0, u_b) + f_in - feq(f_in) with
+    the inner node still at rest, so their velocity is the profile itself, u_in 4 t (1 - t) across
+    the channel (t = 0, 1 on the walls), peaking at u_in on the centre line."""
+    u_in = 0.06
+    grid, kw = sc.channel_nee(nx=24, ny=16, u_in=u_in)
+    o = Oracle(grid, collision=(0,), bc_profile={g.XMIN: 1}, **kw)
+    o.advance_level(0, 1)
+    f = o.f(0)
+    cx, cy, _, _, _ = zo.lattice(2)
+    rho = f.sum(1)
+    ux = (f @ np.asarray(cx, np.float64)) / rho
+    xy = grid.coords(0).astype(np.int64)
+    lo, hi = grid.dom_min, grid.dom_max
+    inlet = (xy[:, 0] == lo[0]) & (xy[:, 1] > lo[1]) & (xy[:, 1] < hi[1])
+    t = (xy[inlet, 1] - lo[1]) / float(hi[1] - lo[1])
+    np.testing.assert_allclose(ux[inlet], u_in * 4 * t * (1 - t), rtol=1e-12, atol=1e-15)
+    assert abs(ux[inlet].max() - u_in) < 1e-12        # even node count across: a node on the centre line

@@ -125,6 +125,7 @@ struct LevelState {
         aux_macro_flag_d;   // [naux] 1: the aux collision records rho, u (fine-to-coarse sources)
     // boundary tables per direction: aux index of node, partner aux index, slot
     DevBuf bc_ai_d[7], bc_partner_d[7], bc_slot_d[7];
+    DevBuf bc_u_d[7];   // [n][3] per-node boundary velocity of a profiled face (empty: uniform)
     int64_t bc_n[7] = {0, 0, 0, 0, 0, 0, 0};
     std::vector<int64_t> bnode[7], bpart[7];
     TransferTables xfer;

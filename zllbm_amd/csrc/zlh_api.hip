@@ -286,7 +286,8 @@ __global__ void k_bc_periodic(int64_t nb, const int32_t *node_ai, const int32_t 
 // f = feq(rho0, u_b) + f(inner) - feq(rho, u of inner)
 template <int D, typename T, typename C, bool SHIFT>
 __global__ void k_bc_nee(int64_t nb, const int32_t *node_ai, const int32_t *inner_ai, T *aux_f, int64_t naux,
-                         const double *aux_force, StepParams p, double rho0, double ux, double uy, double uz) {
+                         const double *aux_force, StepParams p, double rho0, double ux, double uy, double uz,
+                         const double *ub_node) {
     constexpr int Q = Lattice<D>::Q;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nb) return;
@@ -302,7 +303,9 @@ __global__ void k_bc_nee(int64_t nb, const int32_t *node_ai, const int32_t *inne
     macro<D, SHIFT, C>(f, F, C(p.dt), p.has_force, p.compressible, rho, drho, u);
     const C usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
     const C rb = C(rho0), drb = C(rho0 - 1.);
-    const C ubx = C(ux), uby = C(uy), ubz = C(D == 3 ? uz : 0.);
+    // a profiled face carries one velocity per boundary node
+    const C ubx = C(ub_node ? ub_node[t * 3] : ux), uby = C(ub_node ? ub_node[t * 3 + 1] : uy);
+    const C ubz = C(D == 3 ? (ub_node ? ub_node[t * 3 + 2] : uz) : 0.);
     const C ubsq = ubx * ubx + uby * uby + ubz * ubz;
     for (int q = 0; q < Q; ++q) {
         const C fb = feq_stored<D, SHIFT, C>(q, rb, drb, ubx, uby, ubz, ubsq, p.compressible);
@@ -385,6 +388,7 @@ struct zlh_ctx {
     std::map<uint32_t, hipGraphExec_t> graph_exec;
     // time marching runs each level on a stream of its own; events order a transfer after the
     // coarse step it pairs with, and the coarse level's next step after the transfer
+    int bc_profile[7] = {};   // per face: ZLH_PROFILE_*
     bool level_streams = true;
     std::vector<hipStream_t> lstream;
     std::vector<hipEvent_t> ev_step, ev_xfer, ev_join;
@@ -646,7 +650,8 @@ zlh_status launch_step_kernels(zlh_ctx *ctx, int level, int coll) {
             case ZLH_BC_NEE:
                 hipLaunchKernelGGL((k_bc_nee<D, T, C, SHIFT>), dim3(blocks(L.bc_n[dir])), dim3(256), 0, ctx->stream,
                                    L.bc_n[dir], L.bc_ai_d[dir].as<int32_t>(), L.bc_partner_d[dir].as<int32_t>(),
-                                   aux_f, L.naux, aux_force, L.sp, ctx->d.rho0, bv[0], bv[1], bv[2]);
+                                   aux_f, L.naux, aux_force, L.sp, ctx->d.rho0, bv[0], bv[1], bv[2],
+                                   L.bc_u_d[dir].p ? L.bc_u_d[dir].as<double>() : nullptr);
                 break;
             default:
                 break;
@@ -860,6 +865,23 @@ zlh_status build_level_aux(zlh_ctx *ctx, int level) {
             slot.push_back(L.node_slot_h[L.bnode[dir][k]]);
         }
         L.bc_n[dir] = (int64_t)ai.size();
+        if (ctx->bc_profile[dir] == ZLH_PROFILE_PARABOLIC && ctx->d.bc_scheme[dir] == ZLH_BC_NEE) {
+            // the face velocity times prod over the transverse axes of 4 t (1 - t)
+            const int axis = (dir - 1) / 2;
+            std::vector<double> ub;
+            for (int64_t i : L.bnode[dir]) {
+                double fac = 1.0;
+                for (int a = 0; a < dims; ++a) {
+                    if (a == axis) continue;
+                    const double lo = (double)((int64_t)ctx->d.dom_min[a] << level);
+                    const double hi = (double)((int64_t)ctx->d.dom_max[a] << level);
+                    const double t = ((double)axis_coord(dims, L.keys[i], a) - lo) / (hi - lo);
+                    fac *= 4.0 * t * (1.0 - t);
+                }
+                for (int c = 0; c < 3; ++c) ub.push_back(ctx->d.bc_value[dir][c] * fac);
+            }
+            if ((s = upload(ctx, L.bc_u_d[dir], ub))) return s;
+        }
         if ((s = upload(ctx, L.bc_ai_d[dir], ai))) return s;
         if ((s = upload(ctx, L.bc_partner_d[dir], part))) return s;
         if ((s = upload(ctx, L.bc_slot_d[dir], slot))) return s;
@@ -1210,6 +1232,17 @@ zlh_status zlh_graphs_enable(zlh_ctx *ctx, int enable) {
     if (!ctx) return ZLH_EINVAL;
     ctx->use_graphs = enable != 0;
     if (!ctx->use_graphs) ctx->drop_graphs();
+    return ZLH_OK;
+}
+
+zlh_status zlh_set_boundary_profile(zlh_ctx *ctx, int dir, int profile) {
+    if (!ctx) return ZLH_EINVAL;
+    if (ctx->built) return ctx->fail(ZLH_ESTATE, "boundary profile after build");
+    if (dir < 1 || dir > 2 * ctx->d.dims) return ctx->fail(ZLH_EINVAL, "bad face %d", dir);
+    if (profile != ZLH_PROFILE_UNIFORM && profile != ZLH_PROFILE_PARABOLIC) return ctx->fail(ZLH_EINVAL, "bad profile %d", profile);
+    if (profile == ZLH_PROFILE_PARABOLIC && ctx->d.bc_scheme[dir] != ZLH_BC_NEE)
+        return ctx->fail(ZLH_EINVAL, "face %d: profiles apply to non-equilibrium extrapolation faces", dir);
+    ctx->bc_profile[dir] = profile;
     return ZLH_OK;
 }
 

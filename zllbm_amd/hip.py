@@ -21,6 +21,7 @@ FP64, FP32_SHIFTED = 0, 1
 SRT, MRT = 0, 1
 LES_NONE, LES_SMAGORINSKY, LES_MLP = 0, 1, 2
 LES_TAU_REFERENCE, LES_TAU_PHYSICAL = 0, 1
+PROFILE_UNIFORM, PROFILE_PARABOLIC = 0, 1
 MLP_FP32, MLP_BF16_MFMA, MLP_FP16_MFMA = 0, 1, 2
 MLP_OPERANDS_BF16, MLP_OPERANDS_FP16 = 0, 1
 
@@ -41,7 +42,7 @@ EXPORTED = (
     "zlh_create", "zlh_destroy", "zlh_last_error", "zlh_set_level_nodes", "zlh_set_interface_layer",
     "zlh_ib_set_markers", "zlh_build", "zlh_init_feq", "zlh_init_macros", "zlh_upload_populations",
     "zlh_step_level", "zlh_transfer_levels", "zlh_time_marching", "zlh_download_populations", "zlh_download_macros",
-    "zlh_download_source_table", "zlh_ib_download_forces", "zlh_conservation", "zlh_level_info", "zlh_transfer_info", "zlh_level_streams_enable",
+    "zlh_download_source_table", "zlh_ib_download_forces", "zlh_conservation", "zlh_level_info", "zlh_transfer_info", "zlh_level_streams_enable", "zlh_set_boundary_profile",
     "zlh_synchronize", "zlh_timing_enable", "zlh_timing_read", "zlh_mlp_set", "zlh_mlp_gradient",
     "zlh_mlp_train", "zlh_mlp_eval", "zlh_closure_features", "zlh_graphs_enable",
 )
@@ -90,6 +91,7 @@ def lib():
         L.zlh_closure_features.argtypes = [P, C.c_int, F32, F32, D(np.int32)]
         L.zlh_graphs_enable.argtypes = [P, C.c_int]
         L.zlh_level_streams_enable.argtypes = [P, C.c_int]
+        L.zlh_set_boundary_profile.argtypes = [P, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -127,7 +129,8 @@ class Solver:
     """One device solver over a :class:`zllbm_amd.grid.Grid` (the SolverLbmInterface +
     GridInfoLbmInteface pair of the reference, lbm_interface.h:140-474)."""
 
-    def __init__(self, grid: "_grid.Grid", **kw):
+    def __init__(self, grid: "_grid.Grid", bc_profile=None, **kw):
+        """bc_profile: {face: PROFILE_PARABOLIC} for non-equilibrium-extrapolation faces."""
         self.grid = grid
         self.dims = grid.dims
         self.q = 9 if grid.dims == 2 else 19
@@ -138,6 +141,8 @@ class Solver:
         if rc != 0:
             raise RuntimeError("zlh_create failed with status %d" % rc)
         self.h = h
+        for face, prof in (bc_profile or {}).items():
+            self._chk(L.zlh_set_boundary_profile(self.h, face, prof))
         for l, lev in enumerate(grid.levels):
             self._chk(L.zlh_set_level_nodes(self.h, l, np.ascontiguousarray(lev.keys, np.uint64),
                                             np.ascontiguousarray(lev.flags, np.int32), lev.n))

@@ -10,8 +10,9 @@
 * ``cavity(n)``: D2Q9 SRT lid-driven cavity, n x n cells, Re 100 on the lid speed 0.1, FP64 at rest.
 * ``cylinder_amr()``: D2Q9 MRT channel of 512 x 256 background cells with two nested refinement
   boxes around a cylinder of 32 finest cells diameter (101 direct-forcing markers on the finest
-  level), Re 100 on the diameter; uniform inflow 0.05 (non-equilibrium extrapolation in and out,
-  bounce-back walls).
+  level), Re 100 on the diameter and the mean inflow speed 0.05; parabolic inflow and outflow
+  (peak 0.075, non-equilibrium extrapolation), bounce-back walls, started from the parabolic
+  profile everywhere.
 * ``sphere_amr()``: D3Q19 MRT-LES channel of 32 x 16 x 16 background cells (512 x 256 x 256 on
   the finest of 5 levels), each level refined within a band around a sphere of 48 finest cells
   diameter (7,239 Fibonacci markers on the finest level), Re 300, inflow 0.05 with a seeded 1 %
@@ -123,8 +124,10 @@ def cavity(n=128, lid=0.1, re=100.0):
 
 
 def cylinder_amr(cells=(512, 256), diameter_fine=32, u_in=0.05, re=100.0):
-    """Background units: node index i of level l sits at i / 2^l, the domain spans [1, cells + 1]."""
+    """Background units: node index i of level l sits at i / 2^l, the domain spans [1, cells + 1].
+    u_in is the mean inflow speed; the parabolic profile peaks at 1.5 u_in."""
     levels = 3
+    u_peak = 1.5 * u_in
     d_bg = diameter_fine / (1 << (levels - 1))
     cx, cy = cells[0] / 4.0 + 1.0, cells[1] / 2.0 + 1.0
     c0 = np.array([int(cx), int(cy)])
@@ -133,9 +136,17 @@ def cylinder_amr(cells=(512, 256), diameter_fine=32, u_in=0.05, re=100.0):
     grid = g.refined_box_grid(2, cells, [box1, box2])
     nu = u_in * d_bg / re
     xy, area = _geometry().circle_markers((cx + 0.13, cy + 0.07), d_bg / 2.0, 1.0 / (1 << (levels - 1)))
-    init = [(np.ones(lev.n), np.tile([u_in, 0.0], (lev.n, 1))) for lev in grid.levels]
-    wl = Workload("D2Q9 MRT channel %dx%d, 3 levels, cylinder D=%d finest cells, Re %g" % (cells + (diameter_fine, re)),
-                  grid, dict(nu0=nu, bc=_channel_bc(2, u_in), collision=(1,), has_force=True), init)
+    init = []
+    for l in range(levels):
+        y = grid.coords(l)[:, 1].astype(np.float64) / (1 << l)
+        t = (y - 1.0) / cells[1]
+        u = np.zeros((len(y), 2))
+        u[:, 0] = u_peak * 4.0 * t * (1.0 - t)
+        init.append((np.ones(len(y)), u))
+    prof = {g.XMIN: 1, g.XMAX: 1}
+    wl = Workload("D2Q9 MRT channel %dx%d, 3 levels, cylinder D=%d finest cells, Re %g, parabolic inflow" % (
+        cells + (diameter_fine, re)), grid,
+        dict(nu0=nu, bc=_channel_bc(2, u_peak), bc_profile=prof, collision=(1,), has_force=True), init)
     wl.ib = IbSet(levels - 1, xy, area)
     return wl
 
