@@ -177,6 +177,10 @@ zlh_status zlh_mlp_train(zlh_ctx *ctx, const float *x, const float *t, int64_t n
 /* network forms: FP32 on the vector units (the FP64 solver's collision) and BF16 operands with
  * FP32 accumulation on the matrix cores, 64 nodes per wavefront (the FP32 solver's collision) */
 enum { ZLH_MLP_FP32 = 0, ZLH_MLP_BF16_MFMA = 1, ZLH_MLP_FP16_MFMA = 2 };
+/* kernel of zlh_mlp_gradient and zlh_mlp_train: ZLH_MLP_FP32 (default; FP32 on the vector units,
+ * one thread per sample) or ZLH_MLP_BF16_MFMA (forward and backward of 64 samples per wavefront
+ * on the matrix cores, BF16 operands and FP32 accumulation); ZLH_EINVAL for any other form */
+zlh_status zlh_mlp_training_form(zlh_ctx *ctx, int form);
 /* network output (nu_t / nu_scale) of normalised inputs [n][9] with the parameters set by
  * zlh_mlp_set, in the given form */
 zlh_status zlh_mlp_eval(zlh_ctx *ctx, const float *x, int64_t n, int form, float *y);

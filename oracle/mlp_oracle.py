@@ -160,3 +160,37 @@ def forward_bf16(p, X):
 
 def forward_f16(p, X):
     return forward_lowp(p, X, f16_round)
+
+
+def loss_and_grad_lowp(p, X, T, rnd):
+    """loss_and_grad with the roundings of the matrix-core training kernel: X, W1, b1, W2, b2
+    and the layer-1 activations are rounded by `rnd` as operands of the forward products; the
+    layer-2 delta is rounded as operand of both dW2 and the layer-1 delta, and the layer-1 delta
+    as operand of dW1 and db1 (db1 is a column of the same product, against inputs of 1). The
+    SiLU arguments, the output layer, db2, dw3, db3 and the loss use unrounded values. With rnd
+    the identity this is loss_and_grad (a test checks it)."""
+    W1, b1, W2, b2, w3, b3 = unpack(p)
+    n = X.shape[0]
+    Xr = rnd(X)
+    Z1 = Xr @ rnd(W1).T + rnd(b1)
+    H1 = rnd(silu(Z1))
+    Z2 = H1 @ rnd(W2).T + rnd(b2)
+    H2 = silu(Z2)
+    z3 = H2 @ w3 + b3
+    y = softplus(z3)
+    loss = np.mean((y - T) ** 2)
+    dz3 = 2.0 * (y - T) / n * sigmoid(z3)
+    dw3 = H2.T @ dz3
+    db3 = dz3.sum()
+    s2 = sigmoid(Z2)
+    dZ2 = np.outer(dz3, w3) * s2 * (1.0 + Z2 * (1.0 - s2))
+    db2 = dZ2.sum(0)
+    dZ2r = rnd(dZ2)
+    dW2 = dZ2r.T @ H1
+    s1 = sigmoid(Z1)
+    dZ1 = rnd((dZ2r @ rnd(W2)) * s1 * (1.0 + Z1 * (1.0 - s1)))
+    dW1 = dZ1.T @ Xr
+    db1 = dZ1.sum(0)
+    g = np.concatenate([dW1.ravel(), db1, dW2.ravel(), db2, dw3, [db3]])
+    return loss, g
+

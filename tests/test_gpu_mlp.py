@@ -1,5 +1,7 @@
 """Learned closure on the device: gradients and training against the FP64 reference, the
 closure fused into the collision against the oracle, and the gain over Smagorinsky."""
+import functools
+
 import numpy as np
 import pytest
 
@@ -49,6 +51,58 @@ def test_training_reduces_loss_and_adam_step():
     l0 = mo.loss_and_grad(p0.astype(np.float64), X, T)[0]
     l2 = mo.loss_and_grad(p2.astype(np.float64), X, T)[0]
     assert l2 < 0.2 * l0, (l0, l2)
+
+
+@pytest.mark.parametrize("n", [3000, 70000])
+def test_gradient_bf16_mfma_matches_emulation(n):
+    """Matrix-core training kernel (BF16 operands, FP32 accumulation) against an FP64 evaluation
+    of the same roundings, and within BF16 accuracy of the exact gradient. 3,000 samples end in a
+    partial wavefront; 70,000 take two trips of the 256-workgroup grid, the second partial."""
+    s = _ctx()
+    X, T = _data(n, seed=1)
+    p = closure.init_params(3)
+    loss, gd = s.mlp_gradient(p, X, T, form=hip.MLP_BF16_MFMA)
+    args = (p.astype(np.float64), X.astype(np.float64), T.astype(np.float64))
+    lemu, gemu = mo.loss_and_grad_lowp(*args, mo.bf16_round)
+    lref, gref = mo.loss_and_grad(*args)
+    err_emu = np.max(np.abs(gd - gemu)) / np.max(np.abs(gemu))
+    err_ref = np.linalg.norm(gd - gref) / np.linalg.norm(gref)
+    print("n %d: max |g - g_emulated| / max |g| %.2e, |g - g_exact| / |g| %.2e, loss %.6e (emulated %.6e)"
+          % (n, err_emu, err_ref, loss, lemu))
+    assert abs(loss - lemu) <= 1e-4 * lemu
+    assert err_emu < 5e-4
+    assert err_ref < 2e-2
+    # the emulation accounts for nearly all of the difference from the exact gradient
+    assert np.linalg.norm(gd - gemu) < 0.1 * np.linalg.norm(gd - gref)
+
+
+def test_training_bf16_mfma_adam_step_and_loss():
+    """Adam with the matrix-core gradient: one full-batch step is the reference update of that
+    gradient, and minibatch training lowers the loss as the FP32 form does."""
+    s = _ctx()
+    X, T = _data()
+    p0 = closure.init_params(5)
+    p1, _ = s.mlp_train(p0, X, T, 1, len(T), 1e-3, form=hip.MLP_BF16_MFMA)
+    _, g0 = s.mlp_gradient(p0, X, T, form=hip.MLP_BF16_MFMA)
+    pr, _, _ = mo.adam(p0.astype(np.float64), g0.astype(np.float64), np.zeros(mo.NPARAM), np.zeros(mo.NPARAM), 1, 1e-3)
+    # atomics add in varying order: the two gradients agree to FP32 accumulation, and the first
+    # Adam step is about lr sign(g) wherever |g| >> eps
+    np.testing.assert_allclose(p1, pr, rtol=1e-5, atol=2e-5)
+    l0 = mo.loss_and_grad(p0.astype(np.float64), X, T)[0]
+    losses = {}
+    for form in (hip.MLP_FP32, hip.MLP_BF16_MFMA):
+        p2, _ = s.mlp_train(p0, X, T, 400, 512, 3e-3, form=form)
+        losses[form] = mo.loss_and_grad(p2.astype(np.float64), X, T)[0]
+    print("loss %.4e -> FP32 training %.4e, BF16 matrix-core training %.4e" % (l0, losses[0], losses[1]))
+    assert losses[hip.MLP_BF16_MFMA] < 0.2 * l0
+    assert losses[hip.MLP_BF16_MFMA] < 2.0 * losses[hip.MLP_FP32]
+
+
+def test_training_form_rejects_fp16():
+    s = _ctx()
+    X, T = _data(100)
+    with pytest.raises(RuntimeError, match="training form"):
+        s.mlp_gradient(closure.init_params(1), X, T, form=hip.MLP_FP16_MFMA)
 
 
 def test_eval_bf16_mfma_matches_emulation():
@@ -239,6 +293,7 @@ def test_closure_features_and_fallback():
     np.testing.assert_allclose(nu_t[out], nu_smag[out], rtol=2e-6)
 
 
+@functools.lru_cache(maxsize=2)
 def _fine_snapshots(n_fine, steps, snaps, nu=0.004, seed=20261202):
     grid = g.uniform_grid(3, (n_fine,) * 3)
     u = closure.random_fourier_velocity(grid, nmodes=64, kmax=8, urms=0.05 / np.sqrt(3.0), seed=seed)
@@ -253,10 +308,11 @@ def _fine_snapshots(n_fine, steps, snaps, nu=0.004, seed=20261202):
     return out
 
 
-def test_learned_closure_beats_smagorinsky_a_priori():
-    """Trained on box-filtered fine runs, the network predicts the filtered sub-grid viscosity of
-    a held-out snapshot with a smaller error than Smagorinsky at Cs 0.1 and at the best constant
-    of a sweep."""
+@pytest.mark.parametrize("form", [hip.MLP_FP32, hip.MLP_BF16_MFMA])
+def test_learned_closure_beats_smagorinsky_a_priori(form):
+    """Trained on box-filtered fine runs (by the vector-unit or the matrix-core gradient kernel),
+    the network predicts the filtered sub-grid viscosity of a held-out snapshot with a smaller
+    error than Smagorinsky at Cs 0.1 and at the best constant of a sweep."""
     ratio, nu_f = 4, 0.004
     tau0_c = 3 * nu_f / ratio + 0.5
     snaps = _fine_snapshots(96, 150, 4, nu=nu_f)
@@ -264,7 +320,7 @@ def test_learned_closure_beats_smagorinsky_a_priori():
     xtr = np.concatenate([d[0] for d in data[:3]])
     ttr = np.concatenate([d[1] for d in data[:3]])
     s = _ctx()
-    model, hist = closure.fit_model(s, xtr, ttr, steps=1500, batch=2048, lr=3e-3)
+    model, hist = closure.fit_model(s, xtr, ttr, steps=1500, batch=2048, lr=3e-3, form=form)
     assert hist[-50:].mean() < hist[:50].mean()
     xte, tte, pte = data[3]
     s.mlp_set(model)
@@ -273,8 +329,8 @@ def test_learned_closure_beats_smagorinsky_a_priori():
     rms = lambda a: float(np.sqrt(np.mean((a[inside] - tte[inside]) ** 2)))
     err_mlp = rms(pred)
     err_smag = {cs: rms(closure.smagorinsky_nu_t(pte, tau0_c, cs)) for cs in (0.05, 0.07, 0.08, 0.1, 0.12, 0.15)}
-    print("a-priori rms error of nu_t: mlp %.3e, smagorinsky %s (%.0f%% of samples in range)"
-          % (err_mlp, ", ".join("Cs %.2f %.3e" % kv for kv in err_smag.items()), 100 * inside.mean()))
+    print("a-priori rms error of nu_t (training form %d): mlp %.3e, smagorinsky %s (%.0f%% of samples in range)"
+          % (form, err_mlp, ", ".join("Cs %.2f %.3e" % kv for kv in err_smag.items()), 100 * inside.mean()))
     assert inside.mean() > 0.5
     assert err_mlp < 0.9 * err_smag[0.1]
     assert err_mlp < min(err_smag.values())

@@ -122,3 +122,20 @@ def test_forward_bf16_close_to_exact():
     yb = mo.forward_bf16(p, X)
     rel = np.abs(yb - y) / np.abs(y)
     assert 0 < rel.max() < 3e-2 and np.median(rel) < 5e-3
+
+
+def test_lowp_gradient_with_exact_rounding_is_the_gradient():
+    """The emulation of the matrix-core training kernel reduces to the finite-difference-pinned
+    gradient when its rounding is the identity, and stays within BF16 accuracy of it."""
+    rng = np.random.default_rng(4)
+    p = closure.init_params(2).astype(np.float64) + rng.normal(0, 0.05, mo.NPARAM)
+    X = rng.normal(size=(300, mo.NF))
+    T = rng.uniform(0.1, 2.0, 300)
+    loss, g = mo.loss_and_grad(p, X, T)
+    l_id, g_id = mo.loss_and_grad_lowp(p, X, T, lambda a: np.asarray(a, np.float64))
+    assert abs(l_id - loss) <= 1e-13 * loss
+    np.testing.assert_allclose(g_id, g, rtol=1e-11, atol=1e-14 * np.abs(g).max())
+    l_b, g_b = mo.loss_and_grad_lowp(p, X, T, mo.bf16_round)
+    rel = np.linalg.norm(g_b - g) / np.linalg.norm(g)
+    assert 1e-4 < rel < 2e-2, rel
+    assert abs(l_b - loss) < 1e-2 * loss

@@ -137,9 +137,10 @@ def smagorinsky_nu_t(pi6, tau0, cs):
     return 0.5 * (np.sqrt(tau0 ** 2 + 18 * cs * cs * q) - tau0) / 3.0
 
 
-def fit_model(solver, x_raw, nu_t, steps=3000, batch=4096, lr=2e-3, seed=20261202, margin=0.0):
-    """Normalise, train on the device (solver: any zllbm_amd.hip.Solver context) and return the
-    model with the training range of the raw inputs."""
+def fit_model(solver, x_raw, nu_t, steps=3000, batch=4096, lr=2e-3, seed=20261202, margin=0.0, form=None):
+    """Normalise, train on the device (solver: any zllbm_amd.hip.Solver context; form: gradient
+    kernel, hip.MLP_FP32 by default or hip.MLP_BF16_MFMA) and return the model with the training
+    range of the raw inputs."""
     mean = x_raw.mean(0)
     std = x_raw.std(0)
     const = std == 0
@@ -147,7 +148,8 @@ def fit_model(solver, x_raw, nu_t, steps=3000, batch=4096, lr=2e-3, seed=2026120
     nu_scale = float(np.mean(nu_t)) if np.mean(nu_t) > 0 else 1.0
     xn = ((x_raw - mean) / std).astype(np.float32)
     t = (nu_t / nu_scale).astype(np.float32)
-    params, hist = solver.mlp_train(init_params(seed), xn, t, steps, min(batch, len(t)), lr, seed)
+    kw = {} if form is None else {"form": form}
+    params, hist = solver.mlp_train(init_params(seed), xn, t, steps, min(batch, len(t)), lr, seed, **kw)
     span = x_raw.max(0) - x_raw.min(0)
     # a feature without spread in the data does not bound the range
     fmin = np.where(const, -np.inf, x_raw.min(0) - margin * span).astype(np.float32)

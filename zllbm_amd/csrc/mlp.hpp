@@ -376,6 +376,191 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float *__restrict__ para
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// training on the matrix cores: forward and backward of 64 samples per wavefront as products
+// with BF16 operands and FP32 accumulation (v_mfma_f32_32x32x16_bf16).  Same gradient, batch
+// selection and loss as k_mlp_grad.  The forward has mlp_forward_wave's layout without the SiLU
+// scaling (the parameters change every step, so the operands are rounded here from FP32).  The
+// weight gradients contract over samples, which sit on lanes, so the backward turns the layer
+// deltas and activations around through LDS:
+//   dW2 [32 x 32]        = D2 [32 x 64 samples] . H1^T     4 K steps of 16 samples
+//   D1^T [32 x 64]       = W2^T . D2                       D2 kept in accumulator order, as the
+//                                                          forward keeps H1 for layer 2
+//   [dW1 | db1] [32 x 10] = D1 [32 x 64] . [X | 1]          4 K steps; column 9 is the bias
+// Rounded to BF16: X, W1, b1, W2, b2, H1, D2 (as operand of both products) and D1.  db2, dw3,
+// db3 and the loss accumulate per lane in FP32 from unrounded values.  A wavefront keeps its
+// sums over all the batches it takes in its accumulators; the workgroup adds its four
+// wavefronts' sums in LDS and then into grad, one atomic per parameter.
+// ---------------------------------------------------------------------------------------------
+constexpr int kGradTileStride = 72;   // LDS row stride of the transposed tiles, BF16 elements (144 B)
+constexpr int kGradMaxBlocks = 256;
+
+struct alignas(16) GradTiles {
+    __bf16 d[kMlpH][kGradTileStride];   // layer delta [hidden][sample]: D2, then D1
+    __bf16 h[kMlpH][kGradTileStride];   // layer-1 activations [hidden][sample]
+    __bf16 x[16][kGradTileStride];      // inputs [feature][sample]; row 9 = 1, rows 10..15 = 0
+};
+
+__device__ __forceinline__ float dsilu(float z, float sg) { return sg * (1.f + z * (1.f - sg)); }
+
+__global__ __launch_bounds__(256) void k_mlp_grad_mfma(const float *__restrict__ params, const float *__restrict__ X,
+                                                       const float *__restrict__ T, int64_t n, int batch,
+                                                       uint32_t seed, int random, float *__restrict__ grad,
+                                                       float *__restrict__ loss) {
+    __shared__ float sp[kMlpParams];
+    __shared__ float acc[kMlpParams + 1];   // workgroup sums: gradient, then the loss
+    __shared__ GradTiles tiles[4];
+    const int lane = __lane_id(), h = lane >> 5, col = lane & 31;
+    GradTiles &tl = tiles[threadIdx.x >> 6];
+    for (int i = threadIdx.x; i < kMlpParams + 1; i += blockDim.x) {
+        if (i < kMlpParams) sp[i] = params[i];
+        acc[i] = 0.f;
+    }
+    for (int r = kMlpF; r < 16; ++r) tl.x[r][lane] = (__bf16)(r == kMlpF ? 1.f : 0.f);
+    __syncthreads();
+    const float *W1 = sp, *b1 = W1 + kMlpH * kMlpF, *W2 = b1 + kMlpH, *b2 = W2 + kMlpH * kMlpH;
+    const float *w3 = b2 + kMlpH;
+    const float b3 = w3[kMlpH];
+    // A operands: row `col` of the weight matrix, K slots 8 h + e (the layout of mlp_forward_wave's operand images)
+    bf16x8_t a1, a2[3], at[2];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int k = 8 * h + e;
+        a1[e] = (__bf16)(k < kMlpF ? W1[col * kMlpF + k] : k == kMlpF ? b1[col] : 0.f);
+#pragma unroll
+        for (int st = 0; st < 2; ++st) {
+            a2[st][e] = (__bf16)W2[col * kMlpH + mfma_row(h, 8 * st + e)];
+            at[st][e] = (__bf16)W2[mfma_row(h, 8 * st + e) * kMlpH + col];   // W2^T
+        }
+        a2[2][e] = (__bf16)(k == 0 ? b2[col] : 0.f);
+    }
+    float w3r[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) w3r[r] = w3[mfma_row(h, r)];
+    const float inv_b = 1.f / (float)batch;
+    const f32x16_t zero = {};
+    f32x16_t gw2 = zero, gw1 = zero;
+    float gw3[16], gb2[16], gb3 = 0.f, lsum = 0.f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) gw3[r] = gb2[r] = 0.f;
+    // every wavefront of the workgroup runs the same trips (the barriers need it); lanes past the
+    // batch carry zero inputs and a zero output delta, so they add nothing
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < batch; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = base + threadIdx.x;
+        const bool active = k < batch;
+        int64_t s = 0;
+        if (active) s = random ? (int64_t)(hash32(seed * 2654435761u + (uint32_t)k) % (uint64_t)n) : k;
+        float xn[kMlpF], px[kMlpF];
+#pragma unroll
+        for (int i = 0; i < kMlpF; ++i) xn[i] = active ? X[s * kMlpF + i] : 0.f;
+        const float t = active ? T[s] : 0.f;
+#pragma unroll
+        for (int i = 0; i < kMlpF; ++i) px[i] = swap_halves(xn[i]);
+        // layer 1: column block c holds samples 32 c .. 32 c + 31 of the wavefront
+        bf16x8_t xb[2];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float hi = e == 1 ? 1.f : 0.f;
+            xb[0][e] = (__bf16)(e == 0 ? (h ? px[8] : xn[0]) : (h ? hi : xn[e]));
+            xb[1][e] = (__bf16)(e == 0 ? (h ? xn[8] : px[0]) : (h ? hi : px[e]));
+        }
+        f32x16_t z1[2], z2[2];
+        bf16x8_t g[2][3];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            z1[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, xb[c], zero, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) g[c][r >> 3][r & 7] = (__bf16)(z1[c][r] * sigmoidf_(z1[c][r]));
+#pragma unroll
+            for (int e = 0; e < 8; ++e) g[c][2][e] = (__bf16)(h == 0 && e == 0 ? 1.f : 0.f);
+            z2[c] = zero;
+#pragma unroll
+            for (int st = 0; st < 3; ++st) z2[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2[st], g[c][st], z2[c], 0, 0, 0);
+        }
+        // output, loss and its derivative for this lane's sample (lane l holds sample l)
+        float p[2] = {0.f, 0.f}, s2[2][16];
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                s2[c][r] = sigmoidf_(z2[c][r]);
+                p[c] += w3r[r] * z2[c][r] * s2[c][r];
+            }
+        const float z3 = (h ? p[1] : p[0]) + swap_halves(h ? p[0] : p[1]) + b3;
+        const float y = softplusf_(z3);
+        const float dz3 = active ? 2.f * (y - t) * inv_b * sigmoidf_(z3) : 0.f;
+        lsum += active ? (y - t) * (y - t) * inv_b : 0.f;
+        gb3 += dz3;
+        // output delta of the samples of this lane's two columns
+        const float other = swap_halves(dz3);
+        const float d3[2] = {h ? other : dz3, h ? dz3 : other};
+        bf16x8_t bd[2][2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = mfma_row(h, r);
+                gw3[r] += d3[c] * z2[c][r] * s2[c][r];
+                const float dz = d3[c] * w3r[r] * dsilu(z2[c][r], s2[c][r]);
+                gb2[r] += dz;
+                const __bf16 db = (__bf16)dz;
+                bd[c][r >> 3][r & 7] = db;
+                tl.d[row][32 * c + col] = db;
+                tl.h[row][32 * c + col] = g[c][r >> 3][r & 7];
+            }
+#pragma unroll
+        for (int i = 0; i < kMlpF; ++i) tl.x[i][lane] = (__bf16)xn[i];
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            const bf16x8_t A = *reinterpret_cast<const bf16x8_t *>(&tl.d[col][16 * ks + 8 * h]);
+            const bf16x8_t B = *reinterpret_cast<const bf16x8_t *>(&tl.h[col][16 * ks + 8 * h]);
+            gw2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, B, gw2, 0, 0, 0);
+        }
+        f32x16_t e1[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            e1[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(at[0], bd[c][0], zero, 0, 0, 0);
+            e1[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(at[1], bd[c][1], e1[c], 0, 0, 0);
+        }
+        __syncthreads();   // D2 read by every lane before D1 takes its place
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                tl.d[mfma_row(h, r)][32 * c + col] = (__bf16)(e1[c][r] * dsilu(z1[c][r], sigmoidf_(z1[c][r])));
+        __syncthreads();
+        const int xr = col < 16 ? col : 15;   // columns 16..31 of [X | 1] are zero (row 15)
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            const bf16x8_t A = *reinterpret_cast<const bf16x8_t *>(&tl.d[col][16 * ks + 8 * h]);
+            const bf16x8_t B = *reinterpret_cast<const bf16x8_t *>(&tl.x[xr][16 * ks + 8 * h]);
+            gw1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, B, gw1, 0, 0, 0);
+        }
+        __syncthreads();   // tiles read before the next batch overwrites them
+    }
+    float *aW1 = acc, *ab1 = aW1 + kMlpH * kMlpF, *aW2 = ab1 + kMlpH, *ab2 = aW2 + kMlpH * kMlpH;
+    float *aw3 = ab2 + kMlpH;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = mfma_row(h, r);
+        atomicAdd(&aW2[row * kMlpH + col], gw2[r]);
+        if (col < kMlpF) atomicAdd(&aW1[row * kMlpF + col], gw1[r]);
+        else if (col == kMlpF) atomicAdd(&ab1[row], gw1[r]);
+        atomicAdd(&aw3[row], gw3[r]);
+        atomicAdd(&ab2[row], gb2[r]);
+    }
+    gb3 = wave_sum_f(gb3);
+    lsum = wave_sum_f(lsum);
+    if (lane == 0) {
+        atomicAdd(&aw3[kMlpH], gb3);
+        atomicAdd(&acc[kMlpParams], lsum);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) atomicAdd(&grad[i], acc[i]);
+    if (threadIdx.x == 0) atomicAdd(loss, acc[kMlpParams]);
+}
+
 __global__ void k_adam(float *params, const float *grad, float *m, float *v, int n, float lr, float b1, float b2,
                        float eps, int t) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -381,6 +381,7 @@ struct zlh_ctx {
     DevBuf lattice_d;
     DevBuf mlp_d;           // MlpDev of the learned closure
     bool mlp_set = false;
+    int mlp_train_form = ZLH_MLP_FP32;   // kernel of zlh_mlp_gradient / zlh_mlp_train
     int n_cu = 0;           // compute units of the context's device
     // two background steps captured as one graph, per combination of the levels' buffer
     // indices (two steps flip every level's buffers an even number of times, so replays chain)
@@ -1465,6 +1466,27 @@ zlh_status zlh_mlp_set(zlh_ctx *ctx, const float *params, const float *mean, con
     return ZLH_OK;
 }
 
+// gradient kernel of the selected training form: one thread per sample on the vector units, or
+// 64 samples per wavefront on the matrix cores (a grid of at most kGradMaxBlocks workgroups that
+// loop over the batch)
+static void launch_mlp_grad(zlh_ctx *ctx, const float *p, const float *x, const float *t, int64_t n, int nb,
+                            uint32_t seed, int random, float *grad, float *loss) {
+    if (ctx->mlp_train_form == ZLH_MLP_BF16_MFMA)
+        hipLaunchKernelGGL(k_mlp_grad_mfma, dim3(std::min<unsigned>(blocks(nb), kGradMaxBlocks)), dim3(256), 0,
+                           ctx->stream, p, x, t, n, nb, seed, random, grad, loss);
+    else
+        hipLaunchKernelGGL(k_mlp_grad, dim3(blocks(nb)), dim3(256), 0, ctx->stream, p, x, t, n, nb, seed, random, grad,
+                           loss);
+}
+
+zlh_status zlh_mlp_training_form(zlh_ctx *ctx, int form) {
+    if (!ctx) return ZLH_EINVAL;
+    if (form != ZLH_MLP_FP32 && form != ZLH_MLP_BF16_MFMA)
+        return ctx->fail(ZLH_EINVAL, "training form must be ZLH_MLP_FP32 or ZLH_MLP_BF16_MFMA (FP16 gradients underflow)");
+    ctx->mlp_train_form = form;
+    return ZLH_OK;
+}
+
 zlh_status zlh_mlp_gradient(zlh_ctx *ctx, const float *params, const float *x, const float *t, int64_t n, float *grad,
                             float *loss) {
     if (!ctx) return ZLH_EINVAL;
@@ -1479,8 +1501,8 @@ zlh_status zlh_mlp_gradient(zlh_ctx *ctx, const float *params, const float *x, c
     ZLH_TRY(hipMemcpyAsync(dp.p, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, ctx->stream));
     ZLH_TRY(hipMemcpyAsync(dx.p, x, sizeof(float) * kMlpF * n, hipMemcpyHostToDevice, ctx->stream));
     ZLH_TRY(hipMemcpyAsync(dt.p, t, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_mlp_grad, dim3(blocks(n)), dim3(256), 0, ctx->stream, dp.as<float>(), dx.as<float>(),
-                       dt.as<float>(), n, (int)n, 0u, 0, dg.as<float>(), dl.as<float>());
+    launch_mlp_grad(ctx, dp.as<float>(), dx.as<float>(), dt.as<float>(), n, (int)n, 0u, 0, dg.as<float>(),
+                    dl.as<float>());
     ZLH_TRY(hipGetLastError());
     ZLH_TRY(hipMemcpyAsync(grad, dg.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, ctx->stream));
     if (loss) ZLH_TRY(hipMemcpyAsync(loss, dl.p, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
@@ -1511,9 +1533,9 @@ zlh_status zlh_mlp_train(zlh_ctx *ctx, const float *x, const float *t, int64_t n
     const int nb = full ? (int)n : batch;
     for (int it = 1; it <= steps; ++it) {
         ZLH_TRY(hipMemsetAsync(dg.p, 0, sizeof(float) * kMlpParams, ctx->stream));
-        hipLaunchKernelGGL(k_mlp_grad, dim3(blocks(nb)), dim3(256), 0, ctx->stream, dp.as<float>(), dx.as<float>(),
-                           dt.as<float>(), n, nb, (uint32_t)(seed * 1000003ull + (uint64_t)it), full ? 0 : 1,
-                           dg.as<float>(), dl.as<float>() + (it - 1));
+        launch_mlp_grad(ctx, dp.as<float>(), dx.as<float>(), dt.as<float>(), n, nb,
+                        (uint32_t)(seed * 1000003ull + (uint64_t)it), full ? 0 : 1, dg.as<float>(),
+                        dl.as<float>() + (it - 1));
         hipLaunchKernelGGL(k_adam, dim3(blocks(kMlpParams)), dim3(256), 0, ctx->stream, dp.as<float>(), dg.as<float>(),
                            dm.as<float>(), dv.as<float>(), kMlpParams, (float)lr, 0.9f, 0.999f, 1e-8f, it);
     }

@@ -44,7 +44,7 @@ EXPORTED = (
     "zlh_step_level", "zlh_transfer_levels", "zlh_time_marching", "zlh_download_populations", "zlh_download_macros",
     "zlh_download_source_table", "zlh_ib_download_forces", "zlh_conservation", "zlh_level_info", "zlh_transfer_info", "zlh_level_streams_enable", "zlh_set_boundary_profile",
     "zlh_synchronize", "zlh_timing_enable", "zlh_timing_read", "zlh_mlp_set", "zlh_mlp_gradient",
-    "zlh_mlp_train", "zlh_mlp_eval", "zlh_closure_features", "zlh_graphs_enable",
+    "zlh_mlp_train", "zlh_mlp_training_form", "zlh_mlp_eval", "zlh_closure_features", "zlh_graphs_enable",
 )
 
 
@@ -87,6 +87,7 @@ def lib():
         L.zlh_mlp_set.argtypes = [P, F32, F32, F32, F32, F32, C.c_double]
         L.zlh_mlp_gradient.argtypes = [P, F32, F32, F32, C.c_int64, F32, F32]
         L.zlh_mlp_train.argtypes = [P, F32, F32, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_uint64, F32, F32]
+        L.zlh_mlp_training_form.argtypes = [P, C.c_int]
         L.zlh_mlp_eval.argtypes = [P, F32, C.c_int64, C.c_int, F32]
         L.zlh_closure_features.argtypes = [P, C.c_int, F32, F32, D(np.int32)]
         L.zlh_graphs_enable.argtypes = [P, C.c_int]
@@ -249,17 +250,22 @@ class Solver:
         self._chk(lib().zlh_mlp_set(self.h, f(model.params), f(model.mean), f(model.std), f(model.fmin),
                                     f(model.fmax), float(model.nu_scale)))
 
-    def mlp_gradient(self, params, x, t):
+    def mlp_gradient(self, params, x, t, form=MLP_FP32):
+        """Mean-squared-error gradient over all samples; form MLP_FP32 (vector units) or
+        MLP_BF16_MFMA (matrix cores, BF16 operands)."""
         f = lambda v: np.ascontiguousarray(v, np.float32)
         g = np.zeros(len(params), np.float32)
         loss = np.zeros(1, np.float32)
+        self._chk(lib().zlh_mlp_training_form(self.h, form))
         self._chk(lib().zlh_mlp_gradient(self.h, f(params), f(x), f(t), len(t), g, loss))
         return float(loss[0]), g
 
-    def mlp_train(self, params, x, t, steps, batch, lr, seed=20261202):
+    def mlp_train(self, params, x, t, steps, batch, lr, seed=20261202, form=MLP_FP32):
+        """Adam on the device with the gradient kernel of `form` (as mlp_gradient)."""
         f = lambda v: np.ascontiguousarray(v, np.float32)
         p = f(params).copy()
         hist = np.zeros(steps, np.float32)
+        self._chk(lib().zlh_mlp_training_form(self.h, form))
         self._chk(lib().zlh_mlp_train(self.h, f(x), f(t), len(t), steps, batch, lr, seed, p, hist))
         return p, hist
 

@@ -35,6 +35,8 @@ def main(argv=None):
     ap.add_argument("--every", type=int, default=256, help="fine steps between snapshots")
     ap.add_argument("--per-ratio", type=int, default=200000, help="samples kept per snapshot and ratio")
     ap.add_argument("--train-steps", type=int, default=4000)
+    ap.add_argument("--form", choices=("fp32", "bf16-mfma"), default="fp32",
+                    help="gradient kernel: vector units (FP32) or matrix cores (BF16 operands)")
     ap.add_argument("--out", required=True)
     args = ap.parse_args(argv)
     t0 = time.time()
@@ -76,7 +78,8 @@ def main(argv=None):
     train = [p for per in data[:-1] for p in per]
     x = np.concatenate([p[2] for p in train])
     t = np.concatenate([p[3] for p in train])
-    model, hist = closure.fit_model(s, x, t, steps=args.train_steps, batch=4096, lr=3e-3)
+    form = hip.MLP_BF16_MFMA if args.form == "bf16-mfma" else hip.MLP_FP32
+    model, hist = closure.fit_model(s, x, t, steps=args.train_steps, batch=4096, lr=3e-3, form=form)
     log("trained on %d samples: loss %.4f -> %.4f, %.0f s" % (len(t), hist[:50].mean(), hist[-50:].mean(),
                                                              time.time() - t0))
     s.mlp_set(model)
