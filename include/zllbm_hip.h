@@ -34,7 +34,9 @@ enum {
 /* storage / arithmetic form of the populations */
 enum {
     ZLH_FP64 = 0,           /* f stored and computed in double */
-    ZLH_FP32_SHIFTED = 1    /* f - w_i stored and computed in float */
+    ZLH_FP32_SHIFTED = 1,   /* f - w_i stored and computed in float */
+    ZLH_BF16_SHIFTED = 2    /* f - w_i stored in bfloat16 (round to nearest even), computed in
+                               float; not with the learned closure (ZLH_EINVAL) */
 };
 
 /* ELbmBoundaryConditionScheme (boundary_conditions/lbm_boundary_conditions.h) */

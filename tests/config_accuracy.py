@@ -4,7 +4,8 @@
 
 For every configuration:
 * the FP32 shifted-population solver against the FP64 solver, both on the GPU, stepped in lockstep
-  from the same initial state at the configuration's full size;
+  from the same initial state at the configuration's full size; likewise the BF16-storage solver
+  for the configurations without the learned closure (1 to 3);
 * the FP64 GPU solver against the FP64 oracle on the CPU where the oracle finishes in a minute or
   so: configurations 1 and 2 at full size and length, and configuration 4's closure on a
   2-level 24^3 box. (Elsewhere the parity tests pin the FP64 solver to the oracle at smaller
@@ -90,16 +91,17 @@ def errors(a, b):
     return float(np.sqrt(np.sum((ua - ub) ** 2) / np.sum(ub ** 2))), float(np.sqrt(np.mean((ra - rb) ** 2)))
 
 
-def fp32_vs_fp64(W, steps, every, model=None):
-    a = gpu_solver(W, hip.FP32_SHIFTED, model)
+def fp32_vs_fp64(W, steps, every, model=None, precision=hip.FP32_SHIFTED):
+    a = gpu_solver(W, precision, model)
     b = gpu_solver(W, hip.FP64, model)
+    name = "BF16" if precision == hip.BF16_SHIFTED else "FP32"
     curve = []
     for t in range(every, steps + 1, every):
         a.time_marching(every)
         b.time_marching(every)
         lu, lr = errors(state(a, W), state(b, W))
         curve.append((t, lu, lr))
-        log("  FP32 vs FP64 after %6d steps: L2(u) %.3e  RMS(rho) %.3e" % (t, lu, lr))
+        log("  %s vs FP64 after %6d steps: L2(u) %.3e  RMS(rho) %.3e" % (name, t, lu, lr))
     return curve
 
 
@@ -117,7 +119,10 @@ def gpu_vs_oracle(W, steps, model=None):
 
 def throughput(W, steps, model=None):
     out = {}
-    for name, prec in (("fp32", hip.FP32_SHIFTED), ("fp64", hip.FP64)):
+    forms = [("fp32", hip.FP32_SHIFTED), ("fp64", hip.FP64)]
+    if W.solver.get("les") != hip.LES_MLP:
+        forms.append(("bf16", hip.BF16_SHIFTED))
+    for name, prec in forms:
         s = gpu_solver(W, prec, model)
         s.time_marching(10)
         s.synchronize()
@@ -128,7 +133,7 @@ def throughput(W, steps, model=None):
         out[name + "_mlups"] = round(W.updates_per_step * steps / wall / 1e6, 1)
         out[name + "_ms_per_step"] = round(wall / steps * 1e3, 4)
         del s
-    log("  MLUPS FP32 %.1f, FP64 %.1f" % (out["fp32_mlups"], out["fp64_mlups"]))
+    log("  MLUPS " + ", ".join("%s %.1f" % (k[:4].upper(), v) for k, v in out.items() if k.endswith("mlups")))
     return out
 
 
@@ -149,6 +154,7 @@ def main():
         log("config 1:", W.name, W.nodes)
         rec = {"config": 1, "workload": W.name, "throughput": throughput(W, 2000),
                "fp32_vs_fp64": summary(fp32_vs_fp64(W, 10000, 2000)),
+               "bf16_vs_fp64": summary(fp32_vs_fp64(W, 10000, 2000, precision=hip.BF16_SHIFTED)),
                "fp64_vs_oracle": gpu_vs_oracle(W, 10000)}
         print(json.dumps(rec), flush=True)
     if 2 in want:
@@ -156,13 +162,15 @@ def main():
         log("config 2:", W.name, W.nodes)
         rec = {"config": 2, "workload": W.name, "throughput": throughput(W, 400),
                "fp32_vs_fp64": summary(fp32_vs_fp64(W, 2000, 500)),
+               "bf16_vs_fp64": summary(fp32_vs_fp64(W, 2000, 500, precision=hip.BF16_SHIFTED)),
                "fp64_vs_oracle": gpu_vs_oracle(W, 2000)}
         print(json.dumps(rec), flush=True)
     if 3 in want:
         W = workloads.tgv(256)
         log("config 3:", W.name, W.nodes)
         rec = {"config": 3, "workload": W.name, "throughput": throughput(W, 200),
-               "fp32_vs_fp64": summary(fp32_vs_fp64(W, 2000, 500))}
+               "fp32_vs_fp64": summary(fp32_vs_fp64(W, 2000, 500)),
+               "bf16_vs_fp64": summary(fp32_vs_fp64(W, 2000, 500, precision=hip.BF16_SHIFTED))}
         print(json.dumps(rec), flush=True)
     if 4 in want:
         W = workloads.decaying_turbulence(32, 4)

@@ -163,37 +163,38 @@ __global__ __launch_bounds__(256) void k_c2f_gather(int64_t n_groups, const Gath
 #pragma unroll
     for (int i = 0; i < NQ; ++i) acc[i] = 0.;
     if constexpr (SHIFT) {
-        // FP32 storage holds f - w.  sum c (f~ + w) - w = sum c f~ + w (S - 1) with S = sum c: the
-        // weights are multiples of 2^-12 of magnitude <= 1, so S is exact in FP32.  Runs of four
-        // products are summed in FP32 (rounding error a few 2^-24 of the run), the runs in FP64.
+        // FP32 / BF16 storage holds f - w.  sum c (f~ + w) - w = sum c f~ + w (S - 1) with S =
+        // sum c: the weights are multiples of 2^-12 of magnitude <= 1, so S is exact in FP32.
+        // Runs of four products are summed in FP32 (rounding error a few 2^-24 of the run), the
+        // runs in FP64.
         float S = 0.f;
         int32_t k = kb;
         for (; k + 4 <= ke; k += 4) {
             float c[4];
-            const float *row[4];
+            const T *row[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const uint32_t cw = cbuf[k + j];
                 c[j] = weight(cw);
-                row[j] = reinterpret_cast<const float *>(rowbuf) + (cw & 0xffffu) + q0;
+                row[j] = rowbuf + (cw & 0xffffu) + q0;
             }
             S += (c[0] + c[1]) + (c[2] + c[3]);
 #pragma unroll
             for (int i = 0; i < NQ; ++i) {
-                float run = c[0] * row[0][i];
-                run = __builtin_fmaf(c[1], row[1][i], run);
-                run = __builtin_fmaf(c[2], row[2][i], run);
-                run = __builtin_fmaf(c[3], row[3][i], run);
+                float run = c[0] * float(row[0][i]);
+                run = __builtin_fmaf(c[1], float(row[1][i]), run);
+                run = __builtin_fmaf(c[2], float(row[2][i]), run);
+                run = __builtin_fmaf(c[3], float(row[3][i]), run);
                 acc[i] += (double)run;
             }
         }
         for (; k < ke; ++k) {
             const uint32_t cw = cbuf[k];
             const float c = weight(cw);
-            const float *row = reinterpret_cast<const float *>(rowbuf) + (cw & 0xffffu) + q0;
+            const T *row = rowbuf + (cw & 0xffffu) + q0;
             S += c;
 #pragma unroll
-            for (int i = 0; i < NQ; ++i) acc[i] += (double)(c * row[i]);
+            for (int i = 0; i < NQ; ++i) acc[i] += (double)(c * float(row[i]));
         }
 #pragma unroll
         for (int i = 0; i < NQ; ++i)
@@ -732,8 +733,12 @@ zlh_status launch_transfers_t(zlh_ctx *ctx, int level) {
 
 inline zlh_status launch_transfers(zlh_ctx *ctx, int level) {
     if (ctx->d.dims == 2)
-        return ctx->fp64() ? launch_transfers_t<2, double, false>(ctx, level) : launch_transfers_t<2, float, true>(ctx, level);
-    return ctx->fp64() ? launch_transfers_t<3, double, false>(ctx, level) : launch_transfers_t<3, float, true>(ctx, level);
+        return ctx->fp64() ? launch_transfers_t<2, double, false>(ctx, level)
+             : ctx->bf16() ? launch_transfers_t<2, bf16_t, true>(ctx, level)
+                           : launch_transfers_t<2, float, true>(ctx, level);
+    return ctx->fp64() ? launch_transfers_t<3, double, false>(ctx, level)
+         : ctx->bf16() ? launch_transfers_t<3, bf16_t, true>(ctx, level)
+                       : launch_transfers_t<3, float, true>(ctx, level);
 }
 
 template <int D, typename T, typename Cc, bool SHIFT>
@@ -754,8 +759,12 @@ zlh_status launch_ib_t(zlh_ctx *ctx, int level) {
 inline zlh_status launch_ib(zlh_ctx *ctx, int level) {
     if (ctx->lv[level].ibt.n_markers == 0) return ZLH_OK;
     if (ctx->d.dims == 2)
-        return ctx->fp64() ? launch_ib_t<2, double, double, false>(ctx, level) : launch_ib_t<2, float, float, true>(ctx, level);
-    return ctx->fp64() ? launch_ib_t<3, double, double, false>(ctx, level) : launch_ib_t<3, float, float, true>(ctx, level);
+        return ctx->fp64() ? launch_ib_t<2, double, double, false>(ctx, level)
+             : ctx->bf16() ? launch_ib_t<2, bf16_t, float, true>(ctx, level)
+                           : launch_ib_t<2, float, float, true>(ctx, level);
+    return ctx->fp64() ? launch_ib_t<3, double, double, false>(ctx, level)
+         : ctx->bf16() ? launch_ib_t<3, bf16_t, float, true>(ctx, level)
+                       : launch_ib_t<3, float, float, true>(ctx, level);
 }
 
 }  // namespace zlh

@@ -47,6 +47,17 @@ __host__ __device__ __forceinline__ int64_t pop_at(int q, int64_t slot) {
     return ((slot >> 6) * Q + q) * 64 + (slot & 63);
 }
 
+// Storage form of ZLH_BF16_SHIFTED: f - w in bfloat16, converted to and from float with
+// round-to-nearest-even (v_cvt_pk_bf16_f32).
+typedef __bf16 bf16_t;
+
+// streaming store of one population (2-byte storage goes out as its bit pattern)
+template <typename T>
+__device__ __forceinline__ void store_nt(T v, T *dst) {
+    if constexpr (sizeof(T) == 2) __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, v), reinterpret_cast<uint16_t *>(dst));
+    else __builtin_nontemporal_store(v, dst);
+}
+
 template <typename T>
 struct BrickLevel {
     int64_t nbrick;
@@ -461,7 +472,7 @@ __device__ __forceinline__ void stream_collide_brick(const BrickLevel<T> &lv, co
     // streaming stores: the next substep reads these values through other CUs' caches
     T *__restrict__ dst = fout + b * Q * 64 + lane;
 #pragma unroll
-    for (int q = 0; q < Q; ++q) __builtin_nontemporal_store(T(f[q]), dst + q * 64);
+    for (int q = 0; q < Q; ++q) store_nt(T(f[q]), dst + q * 64);
 }
 
 // Fused pull-stream + collide over all nodes of a level, one wavefront per brick, four bricks
@@ -651,7 +662,7 @@ void k_stream_collide_mlp(int64_t nbrick, int64_t naux, const uint64_t *__restri
         // next substep's aux collision before the fused kernel reads them
         T *__restrict__ dst = fout + b * Q * 64 + lane;
 #pragma unroll
-        for (int q = 0; q < Q; ++q) __builtin_nontemporal_store(T(f[q]), dst + q * 64);
+        for (int q = 0; q < Q; ++q) store_nt(T(f[q]), dst + q * 64);
         if (!more) break;
         b = bn;
         miss = miss_next;

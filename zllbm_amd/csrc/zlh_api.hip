@@ -409,7 +409,8 @@ struct zlh_ctx {
         return code;
     }
     bool fp64() const { return d.precision == ZLH_FP64; }
-    size_t tsize() const { return fp64() ? 8 : 4; }
+    bool bf16() const { return d.precision == ZLH_BF16_SHIFTED; }
+    size_t tsize() const { return fp64() ? 8 : (bf16() ? 2 : 4); }
 };
 
 namespace {
@@ -565,8 +566,10 @@ void launch_stream_collide_v(zlh_ctx *ctx, int var, unsigned grid, const BrickLe
                              const StepParams &sp) {
 #define ZLH_SC(V)                                                                                             \
     case V:                                                                                                   \
-        if constexpr (mlp_on_mfma<C, V>()) {                                                                  \
+        if constexpr (mlp_on_mfma<C, V>() && sizeof(T) == 4) {                                                \
             launch_mlp<D, T, C, SHIFT, COLL, V>(ctx, v, fa, fb, aux_f, sp);                                   \
+        } else if constexpr (mlp_on_mfma<C, V>()) {                                                           \
+            /* BF16 storage with the learned closure: refused by zlh_create */                                \
         } else {                                                                                              \
             hipLaunchKernelGGL((k_stream_collide<D, T, C, SHIFT, COLL, V>), dim3(grid), dim3(256), 0, ctx->stream, \
                                v, fa, fb, aux_f, sp);                                                         \
@@ -698,25 +701,31 @@ zlh_status do_step_kernels(zlh_ctx *ctx, int level) {
     const int coll = ctx->d.collision[level];
     if (ctx->d.dims == 2) {
         return ctx->fp64() ? launch_step_kernels<2, double, double, false>(ctx, level, coll)
+             : ctx->bf16() ? launch_step_kernels<2, bf16_t, float, true>(ctx, level, coll)
                            : launch_step_kernels<2, float, float, true>(ctx, level, coll);
     }
     return ctx->fp64() ? launch_step_kernels<3, double, double, false>(ctx, level, coll)
+         : ctx->bf16() ? launch_step_kernels<3, bf16_t, float, true>(ctx, level, coll)
                        : launch_step_kernels<3, float, float, true>(ctx, level, coll);
 }
 
 zlh_status do_upload(zlh_ctx *ctx, int level, const double *f_dev) {
     if (ctx->d.dims == 2)
         return ctx->fp64() ? dispatch_upload<2, double, double, false>(ctx, level, f_dev)
+             : ctx->bf16() ? dispatch_upload<2, bf16_t, float, true>(ctx, level, f_dev)
                            : dispatch_upload<2, float, float, true>(ctx, level, f_dev);
     return ctx->fp64() ? dispatch_upload<3, double, double, false>(ctx, level, f_dev)
+         : ctx->bf16() ? dispatch_upload<3, bf16_t, float, true>(ctx, level, f_dev)
                        : dispatch_upload<3, float, float, true>(ctx, level, f_dev);
 }
 
 zlh_status do_materialize(zlh_ctx *ctx, int level, double *out_dev) {
     if (ctx->d.dims == 2)
         return ctx->fp64() ? dispatch_materialize<2, double, false>(ctx, level, out_dev)
+             : ctx->bf16() ? dispatch_materialize<2, bf16_t, true>(ctx, level, out_dev)
                            : dispatch_materialize<2, float, true>(ctx, level, out_dev);
     return ctx->fp64() ? dispatch_materialize<3, double, false>(ctx, level, out_dev)
+         : ctx->bf16() ? dispatch_materialize<3, bf16_t, true>(ctx, level, out_dev)
                        : dispatch_materialize<3, float, true>(ctx, level, out_dev);
 }
 
@@ -913,7 +922,10 @@ zlh_status zlh_create(const zlh_solver_desc *desc, zlh_ctx **out) {
     if (!desc || !out) return ZLH_EINVAL;
     *out = nullptr;
     if ((desc->dims != 2 && desc->dims != 3) || desc->max_level < 0 || desc->max_level > 15) return ZLH_EINVAL;
-    if (desc->precision != ZLH_FP64 && desc->precision != ZLH_FP32_SHIFTED) return ZLH_EINVAL;
+    if (desc->precision != ZLH_FP64 && desc->precision != ZLH_FP32_SHIFTED && desc->precision != ZLH_BF16_SHIFTED)
+        return ZLH_EINVAL;
+    // the learned closure's fused kernel stages 4-byte populations with direct-to-LDS loads
+    if (desc->precision == ZLH_BF16_SHIFTED && desc->les == ZLH_LES_MLP) return ZLH_EINVAL;
     if (!(desc->nu0 > 0.)) return ZLH_EINVAL;
     for (int l = 0; l <= desc->max_level; ++l)
         if (desc->collision[l] != ZLH_SRT && desc->collision[l] != ZLH_MRT) return ZLH_EINVAL;

@@ -17,7 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libzllbm_hip.so")
 _lib = None
 
-FP64, FP32_SHIFTED = 0, 1
+FP64, FP32_SHIFTED, BF16_SHIFTED = 0, 1, 2
 SRT, MRT = 0, 1
 LES_NONE, LES_SMAGORINSKY, LES_MLP = 0, 1, 2
 LES_TAU_REFERENCE, LES_TAU_PHYSICAL = 0, 1
