@@ -6,12 +6,14 @@ Workloads (synthetic data, zllbm_amd.workloads):
           populations with the learned closure (BF16 matrix-core network, Smagorinsky fallback).
           Compared in the same run with the FP64 Smagorinsky form on the same grid (MLUPS), and
           with the FP64 form of the same closure (velocity and density L2).
-  tgv256  D3Q19 MRT + Smagorinsky (Cs 0.1), 256^3-cell periodic Taylor-Green vortex, FP32.
+  tgv256  D3Q19 MRT + Smagorinsky (Cs 0.1), 256^3-cell periodic Taylor-Green vortex, FP32
+          arithmetic on FP32 (default) or BF16 (--populations bf16) shifted populations.
 
 value = whole-job MLUPS (lattice-node updates per second / 1e6; a level-l node counts 2^l
 updates per background step), populations resident in HBM.
-roofline: algorithmic bytes of the fused stream-collide kernel on the finest level, 2 * Q * 4
-bytes per node update, over its average launch time from HIP events on the library stream.
+roofline: algorithmic bytes of the fused stream-collide kernel on the finest level, 2 * Q * s
+bytes per node update (s = 4 for FP32 storage, 2 for BF16), over its average launch time from
+HIP events on the library stream.
 cpu_baseline: the FP64 CPU oracle (same algorithm, OpenMP over nodes) on a bounded sample.
 """
 import argparse
@@ -122,6 +124,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="amr4", choices=["amr4", "tgv256"])
+    ap.add_argument("--populations", default="fp32", choices=["fp32", "bf16"],
+                    help="storage of the shifted populations (bf16: tgv256 only; no learned closure)")
     ap.add_argument("--closure", default=DEFAULT_CLOSURE, help="trained closure (zllbm_amd.train_closure)")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP64 runs")
     ap.add_argument("--mlp-operands", default="fp16", choices=["fp16", "bf16"],
@@ -160,25 +164,31 @@ def main():
     else:
         W = workloads.tgv(256)
         les = hip.LES_SMAGORINSKY
+    bf16 = args.populations == "bf16"
+    if bf16 and model is not None:
+        print("bench.py: BF16 populations do not run with the learned closure (use --workload tgv256)", file=sys.stderr)
+        sys.exit(2)
     log("workload:", W.name, "nodes per level", W.nodes)
-    s = make_solver(hip, W, hip.FP32_SHIFTED, les, model)
+    s = make_solver(hip, W, hip.BF16_SHIFTED if bf16 else hip.FP32_SHIFTED, les, model)
     if args.one_stream:
         s.level_streams_enable(False)
     log("aux nodes per level", [s.info(l)[1] for l in range(len(W.nodes))])
     wall, k_ms, k_n, _ = timed(s, W, args.warmup, args.steps)
     mlups = W.updates_per_step * args.steps / wall / 1e6
     q = 19
-    bpu = 2 * q * 4
+    bpu = 2 * q * (2 if bf16 else 4)
     finest = W.grid.max_level
     avg_s = k_ms / 1e3 / max(k_n, 1)
     achieved = W.nodes[finest] * bpu / avg_s / 1e9
-    log("FP32 %s: %.3f ms per step, finest-level kernel %.4f ms x %d" % (
-        "learned closure" if model is not None else "Smagorinsky", wall / args.steps * 1e3, avg_s * 1e3, k_n))
+    log("%s %s: %.3f ms per step, finest-level kernel %.4f ms x %d" % (
+        "BF16" if bf16 else "FP32", "learned closure" if model is not None else "Smagorinsky",
+        wall / args.steps * 1e3, avg_s * 1e3, k_n))
     kernel = ("k_stream_collide_mlp" if model is not None else "k_stream_collide") + " (level %d)" % finest
     out.update({"value": round(mlups, 2), "unit": "MLUPS", "ms_per_step": round(wall / args.steps * 1e3, 4),
                 "config": {"workload": W.name, "nodes_per_level": W.nodes, "updates_per_step": W.updates_per_step,
                            "collision": "MRT", "closure": "learned (%s-operand MFMA), Smagorinsky fallback" % args.mlp_operands.upper()
-                           if model is not None else "Smagorinsky Cs 0.1", "populations": "FP32 shifted"},
+                           if model is not None else "Smagorinsky Cs 0.1",
+                           "populations": "BF16 shifted" if bf16 else "FP32 shifted"},
                 "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
                              "avg_launch_ms": round(avg_s * 1e3, 4), "bytes_per_update": bpu,
