@@ -12,10 +12,21 @@ _DIR = {1: "xmin", 2: "xmax", 3: "ymin", 4: "ymax", 5: "zmin", 6: "zmax"}
 _SCHEME = {1: "bounce_back", 2: "periodic", 3: "nee"}
 
 
-def write_params(path, kw, dims, collision="srt", precision="fp64"):
+def write_closure(path, model):
+    """Closure file of the driver: parameters, mean, spread, minimum, maximum, output scale."""
+    vals = [np.asarray(a, np.float64).ravel() for a in (model.params, model.mean, model.std, model.fmin, model.fmax)]
+    with open(path, "w") as fh:
+        for v in vals:
+            fh.write(" ".join(repr(float(np.float32(x))) for x in v) + "\n")
+        fh.write(repr(float(model.nu_scale)) + "\n")
+
+
+def write_params(path, kw, dims, collision="srt", precision="fp64", closure_path=None):
     lines = ["viscosity = %r" % kw["nu0"], "rho = 1.0",
              "velocity = " + " ".join(["0.0"] * dims), "collision = " + collision, "precision = " + precision]
-    if kw.get("les"):
+    if kw.get("les") == 2:
+        lines += ["les = learned", "cs = %r" % kw.get("cs", 0.1), "closure = " + closure_path]
+    elif kw.get("les"):
         lines += ["les = smagorinsky", "cs = %r" % kw.get("cs", 0.1), "les_tau_mode = %d" % kw.get("les_tau_mode", 1)]
     for d, (scheme, val) in kw["bc"].items():
         lines.append("bc.%s = %s %s" % (_DIR[d], _SCHEME[scheme], " ".join(repr(float(v)) for v in val)))
@@ -23,12 +34,15 @@ def write_params(path, kw, dims, collision="srt", precision="fp64"):
         fh.write("\n".join(lines) + "\n")
 
 
-def run(grid, kw, steps, workdir, collision="srt", precision="fp64"):
+def run(grid, kw, steps, workdir, collision="srt", precision="fp64", model=None):
     gpath = os.path.join(workdir, "grid.bin")
     ppath = os.path.join(workdir, "params.txt")
     opath = os.path.join(workdir, "out.bin")
+    cpath = os.path.join(workdir, "closure.txt")
     g.write_grid_file(grid, gpath)
-    write_params(ppath, kw, grid.dims, collision, precision)
+    if model is not None:
+        write_closure(cpath, model)
+    write_params(ppath, kw, grid.dims, collision, precision, cpath)
     proc = subprocess.run([DRIVER, gpath, ppath, str(steps), opath], capture_output=True, text=True, timeout=300)
     if proc.returncode != 0:
         return proc, None

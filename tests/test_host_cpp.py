@@ -33,3 +33,49 @@ def test_driver_matches_oracle(tmp_path, case):
     for l in range(grid.max_level + 1):
         ref = o.f(l)
         assert np.max(np.abs(out[l] - ref)) < 1e-11 * np.max(np.abs(ref))
+
+
+def _cavity3d(n=10, lid=0.05):
+    from zllbm_amd import grid as g
+    grid = g.uniform_grid(3, (n, n, n))
+    bc = sc.all_bc(3, sc.BB)
+    bc[g.ZMAX] = (sc.BB, (lid, 0.0, 0.0))
+    return grid, bc
+
+
+@pytest.mark.gpu
+def test_driver_learned_closure_matches_oracle(tmp_path):
+    """The learned closure selected through the host's SetLesModel("learned") and
+    SetLearnedClosure, on a 3D lid-driven cavity with FP64 populations, against the oracle (the
+    device network runs in FP32, the oracle's in FP64)."""
+    from tests.test_gpu_mlp import _random_model
+    grid, bc = _cavity3d()
+    nu = 0.002
+    kw = dict(nu0=nu, bc=bc, les=2, cs=0.1)
+    m = _random_model(3 * nu + 0.5)
+    proc, out = host_driver.run(grid, kw, 20, str(tmp_path), collision="mrt", model=m)
+    assert proc.returncode == 0, proc.stderr
+    o = Oracle(grid, collision=(1,), **kw)
+    o.set_mlp(m.params.astype(np.float64), m.mean, m.std, m.fmin, m.fmax, m.nu_scale)
+    o.time_marching(20)
+    ref = o.f(0)
+    assert np.max(np.abs(ref - np.tile(ref.mean(0), (len(ref), 1)))) > 1e-4     # the lid has driven a flow
+    assert np.max(np.abs(out[0] - ref)) < 2e-7 * np.max(np.abs(ref))
+
+
+@pytest.mark.gpu
+def test_driver_bf16_precision(tmp_path):
+    """precision = bf16 through the host setters: within BF16 accuracy of the oracle."""
+    grid, bc = _cavity3d()
+    kw = dict(nu0=0.01, bc=bc)
+    proc, out = host_driver.run(grid, kw, 20, str(tmp_path), collision="mrt", precision="bf16")
+    assert proc.returncode == 0, proc.stderr
+    o = Oracle(grid, collision=(1,), **kw)
+    o.time_marching(20)
+    from tests.test_gpu_parity import _macros
+    rho_o, u_o = _macros(o.f(0), 3)
+    rho_d, u_d = _macros(out[0], 3)
+    l2_u = np.sqrt(np.sum((u_d - u_o) ** 2) / np.sum(u_o ** 2))
+    print("host driver, BF16 vs oracle: L2(u) %.3e, max |rho - rho_ref| %.3e" % (l2_u, np.max(np.abs(rho_d - rho_o))))
+    assert l2_u < 1e-2 and np.max(np.abs(rho_d - rho_o)) < 1e-3
+

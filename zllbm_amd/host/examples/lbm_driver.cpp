@@ -7,7 +7,11 @@
 // usage: zllbm_lbm_driver <grid file> <parameter file> <background steps> <output file>
 //   grid file:  "ZLGRID1" + dims, levels, index bounds, then per level keys, flags and layers
 //   parameters: "key = value" lines (viscosity, rho, velocity, collision, les, cs, les_tau_mode,
-//               precision, bc.xmin ... bc.zmax = <bounce_back|periodic|nee> [u v w])
+//               precision = <fp64|fp32|bf16>, closure = <file>, mlp_operands = <fp16|bf16>,
+//               bc.xmin ... bc.zmax = <bounce_back|periodic|nee> [u v w])
+//   closure file (les = learned): whitespace-separated numbers, the 1409 network parameters,
+//               then 9 each of input mean, spread, training minimum and maximum, then the output
+//               scale
 //   output:     per level the node count and the post-stream populations in key order
 #include <cstdio>
 #include <cstring>
@@ -120,7 +124,28 @@ int main(int argc, char **argv) {
             else if (key == "les") { std::string t; val >> t; if (t != "none") lbm->SetLesModel(t); }
             else if (key == "cs") { double v; val >> v; lbm->SetSmagorinskyConstant(v); }
             else if (key == "les_tau_mode") { int v; val >> v; lbm->SetLesTauMode(v); }
-            else if (key == "precision") { std::string t; val >> t; lbm->SetPrecision(t == "fp32" ? ZLH_FP32_SHIFTED : ZLH_FP64); }
+            else if (key == "precision") {
+                std::string t; val >> t;
+                lbm->SetPrecision(t == "fp32" ? ZLH_FP32_SHIFTED : t == "bf16" ? ZLH_BF16_SHIFTED : ZLH_FP64);
+            }
+            else if (key == "mlp_operands") {
+                std::string t; val >> t;
+                lbm->SetMlpOperands(t == "bf16" ? ZLH_MLP_OPERANDS_BF16 : ZLH_MLP_OPERANDS_FP16);
+            }
+            else if (key == "closure") {
+                std::string path; val >> path;
+                std::ifstream cin(path);
+                auto take = [&](size_t n) {
+                    std::vector<float> v(n);
+                    for (auto &x : v)
+                        if (!(cin >> x)) throw ZlhError("closure file " + path + " is short");
+                    return v;
+                };
+                const std::vector<float> p = take(1409), mean = take(9), sd = take(9), lo = take(9), hi = take(9);
+                double scale;
+                if (!(cin >> scale)) throw ZlhError("closure file " + path + " is short");
+                lbm->SetLearnedClosure(p, mean, sd, lo, hi, scale);
+            }
             else if (key.rfind("bc.", 0) == 0) {
                 std::string scheme; val >> scheme;
                 std::vector<double> v; double x; while (val >> x) v.push_back(x);

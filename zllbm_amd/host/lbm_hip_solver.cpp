@@ -77,8 +77,19 @@ void SolverLbmHip::Check(zlh_status s, const char *what) const {
 }
 
 void SolverLbmHip::SetLesModel(const std::string &type) {
-    if (type == "smagorinsky") les_ = true;
-    else throw ZlhError("LES model " + type + " is not defined, please choose from: smagorinsky");
+    if (type == "smagorinsky") les_ = ZLH_LES_SMAGORINSKY;
+    else if (type == "learned") les_ = ZLH_LES_MLP;
+    else throw ZlhError("LES model " + type + " is not defined, please choose from: smagorinsky, learned");
+}
+
+void SolverLbmHip::SetLearnedClosure(const std::vector<float> &params, const std::vector<float> &mean,
+                                     const std::vector<float> &std_dev, const std::vector<float> &fmin,
+                                     const std::vector<float> &fmax, const DefReal nu_scale) {
+    constexpr size_t kParams = 32 * 9 + 32 + 32 * 32 + 32 + 32 + 1, kFeatures = 9;
+    if (params.size() != kParams || mean.size() != kFeatures || std_dev.size() != kFeatures ||
+        fmin.size() != kFeatures || fmax.size() != kFeatures)
+        throw ZlhError("learned closure: expected 1409 parameters and 9 values per feature table");
+    closure_ = Closure{params, mean, std_dev, fmin, fmax, nu_scale};
 }
 
 void SolverLbmHip::SetCollisionOperator(const DefInt i_level, const ELbmCollisionOperatorType type) {
@@ -129,7 +140,8 @@ void SolverLbmHip::BuildDevice() {
     d.nu0 = k0LbmViscosity_;
     d.rho0 = k0Rho_;
     for (int a = 0; a < k0SolverDims_ && a < (int)k0Velocity_.size(); ++a) d.u0[a] = k0Velocity_[a];
-    d.les = les_ ? ZLH_LES_SMAGORINSKY : ZLH_LES_NONE;
+    d.les = les_;
+    d.mlp_operands = mlp_operands_;
     d.les_tau_mode = les_tau_mode_;
     d.cs = cs_;
     d.has_force = bool_forces_ ? 1 : 0;
@@ -194,6 +206,12 @@ void SolverLbmHip::BuildDevice() {
               "zlh_ib_set_markers");
     }
     Check(zlh_build(ctx_), "zlh_build");
+    if (les_ == ZLH_LES_MLP) {
+        if (closure_.params.empty()) throw ZlhError("learned closure selected without SetLearnedClosure");
+        Check(zlh_mlp_set(ctx_, closure_.params.data(), closure_.mean.data(), closure_.std_dev.data(),
+                          closure_.fmin.data(), closure_.fmax.data(), closure_.nu_scale),
+              "zlh_mlp_set");
+    }
     // initial populations: node mirrors with f_ set, else feq(rho0, u0)
     Check(zlh_init_feq(ctx_), "zlh_init_feq");
     for (int l = 0; l <= gm.k0MaxLevel_; ++l) {

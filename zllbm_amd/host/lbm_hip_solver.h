@@ -17,7 +17,8 @@ namespace lbmproject {
 
 enum class ELbmCollisionOperatorType { kLbmSrt, kLbmMrt, kUndefined };
 enum class ELbmBoundaryConditionScheme { kUndefined = 0, kBounceBack = 1, kPeriodic = 2, kNonEqExtrapolation = 3 };
-enum class ELbmLesModelType { kSmagorinsky, kUndefined };
+// kLearned: the learned sub-grid closure of zlh_mlp_set (no counterpart in the reference)
+enum class ELbmLesModelType { kSmagorinsky, kLearned, kUndefined };
 
 // Fatal errors of the device library surface as this exception (the reference's
 // LogManager::LogError terminates; callers that want that behaviour let it propagate).
@@ -79,8 +80,20 @@ class SolverLbmHip : public amrproject::SolverInterface {
     DefReal GetDefaultDensity() const { return k0Rho_; }
     // LES (SetLesModel, les_models/lbm_les_models.cpp:17-41) with the Smagorinsky constant and the
     // relaxation-time mode exposed
-    void SetLesModel(const ELbmLesModelType type) { les_ = type == ELbmLesModelType::kSmagorinsky; }
+    void SetLesModel(const ELbmLesModelType type) {
+        les_ = type == ELbmLesModelType::kSmagorinsky ? ZLH_LES_SMAGORINSKY
+             : type == ELbmLesModelType::kLearned     ? ZLH_LES_MLP
+                                                      : ZLH_LES_NONE;
+    }
     void SetLesModel(const std::string &type);
+    // the network of the learned closure (zlh_mlp_set: 1409 parameters, then 9 each of input mean,
+    // spread, training minimum and maximum) and its output scale; Smagorinsky (physical tau mode,
+    // the constant of SetSmagorinskyConstant) is its fallback outside the training range
+    void SetLearnedClosure(const std::vector<float> &params, const std::vector<float> &mean,
+                           const std::vector<float> &std_dev, const std::vector<float> &fmin,
+                           const std::vector<float> &fmax, DefReal nu_scale);
+    // operands of the FP32 solver's matrix-core network: ZLH_MLP_OPERANDS_FP16 (default) or _BF16
+    void SetMlpOperands(const int operands) { mlp_operands_ = operands; }
     void SetSmagorinskyConstant(const DefReal cs) { cs_ = cs; }
     void SetLesTauMode(const int mode) { les_tau_mode_ = mode; }
     // collision operator per level (SetCollisionOperator, collision_model/lbm_collision_models.cpp:78-90)
@@ -120,10 +133,15 @@ class SolverLbmHip : public amrproject::SolverInterface {
     DefReal k0LbmViscosity_ = 1., k0Rho_ = 1.;
     std::vector<DefReal> k0Velocity_, k0ConstForce_;
     DefInt k0NumForces_ = 0;
-    bool les_ = false;
+    int les_ = ZLH_LES_NONE;
     DefReal cs_ = 0.16;   // the reference's fixed constant (lbm_les_models.h:49) unless set
     int les_tau_mode_ = ZLH_LES_TAU_PHYSICAL;
     int precision_ = ZLH_FP64;
+    int mlp_operands_ = ZLH_MLP_OPERANDS_FP16;
+    struct Closure {
+        std::vector<float> params, mean, std_dev, fmin, fmax;
+        double nu_scale = 0.;
+    } closure_;
     std::vector<ELbmCollisionOperatorType> collision_type_;
     std::map<int, BoundarySetting> bc_;
     struct Ib {
