@@ -3,7 +3,8 @@
 Workloads (synthetic data, zllbm_amd.workloads):
   amr4    (default) D3Q19 MRT-LES decaying turbulence on 4 levels over a periodic 32^3 box
           (256^3-equivalent), refined where |vorticity| exceeds its 80th percentile; FP32 shifted
-          populations with the learned closure (BF16 matrix-core network, Smagorinsky fallback).
+          populations with the learned closure (matrix-core network with FP16 operands, BF16
+          with --mlp-operands bf16; Smagorinsky fallback).
           Compared in the same run with the FP64 Smagorinsky form on the same grid (MLUPS), and
           with the FP64 form of the same closure (velocity and density L2).
   tgv256  D3Q19 MRT + Smagorinsky (Cs 0.1), 256^3-cell periodic Taylor-Green vortex, FP32

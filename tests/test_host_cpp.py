@@ -79,3 +79,55 @@ def test_driver_bf16_precision(tmp_path):
     print("host driver, BF16 vs oracle: L2(u) %.3e, max |rho - rho_ref| %.3e" % (l2_u, np.max(np.abs(rho_d - rho_o))))
     assert l2_u < 1e-2 and np.max(np.abs(rho_d - rho_o)) < 1e-3
 
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["amr2d", "amr3d"])
+def test_driver_checkpoint_format_and_exact_restart(tmp_path, case):
+    """The host writes the reference's grid-node checkpoint records (header, record order, flag
+    and post-stream populations), and a run restarted from them continues as the uninterrupted
+    run, bit for bit."""
+    grid, kw = sc.amr2d(levels=3) if case == "amr2d" else sc.amr3d()
+    q = 9 if grid.dims == 2 else 19
+    ck = str(tmp_path / "grid_nodes")
+    dirs = []
+    for name in ("first", "restart", "straight"):
+        (tmp_path / name).mkdir()
+        dirs.append(str(tmp_path / name))
+    proc, out10 = host_driver.run(grid, kw, 10, dirs[0], collision="mrt", checkpoint_out=ck)
+    assert proc.returncode == 0, proc.stderr
+    count, payload, data, rest = host_driver.read_checkpoint(ck, q)
+    assert count == sum(lev.n for lev in grid.levels) and payload == 4 + 8 * q and rest == 0
+    assert [(int(l), int(k)) for l, k in zip(data["level"], data["key"])] == host_driver.reference_record_order(grid)
+    # payload: the grid's flags and the populations the driver reports after 10 steps (both in
+    # key order; the grid's keys are sorted)
+    for l, lev in enumerate(grid.levels):
+        sel = np.flatnonzero(data["level"] == l)
+        sel = sel[np.argsort(data["key"][sel])]
+        np.testing.assert_array_equal(data["key"][sel], lev.keys)
+        np.testing.assert_array_equal(data["flag"][sel], lev.flags)
+        np.testing.assert_array_equal(data["f"][sel], out10[l])
+    proc, restarted = host_driver.run(grid, kw, 10, dirs[1], collision="mrt", checkpoint_in=ck)
+    assert proc.returncode == 0, proc.stderr
+    proc, straight = host_driver.run(grid, kw, 20, dirs[2], collision="mrt")
+    assert proc.returncode == 0, proc.stderr
+    for l in range(grid.max_level + 1):
+        np.testing.assert_array_equal(restarted[l], straight[l])
+
+
+@pytest.mark.parametrize("case", ["amr2d", "amr3d"])
+def test_checkpoint_order_rule_matches_reference_loops(case):
+    """The host's record order (sort by background cell, level, then z, y, x inside the cell)
+    equals the order of the reference writer's nested loops."""
+    from zllbm_amd import grid as g
+    grid, _ = sc.amr2d(levels=3) if case == "amr2d" else sc.amr3d()
+    dims = grid.dims
+    recs = []
+    for l, lev in enumerate(grid.levels):
+        c = g.morton_decode(dims, lev.keys).astype(np.int64)
+        m = (1 << l) - 1
+        for k, cc in zip(lev.keys, c):
+            recs.append((int(k) >> (dims * l), l, int(cc[2]) & m if dims == 3 else 0, int(cc[1]) & m,
+                         int(cc[0]) & m, int(k)))
+    recs.sort()
+    assert [(r[1], r[5]) for r in recs] == host_driver.reference_record_order(grid)

@@ -8,6 +8,7 @@
 //   grid file:  "ZLGRID1" + dims, levels, index bounds, then per level keys, flags and layers
 //   parameters: "key = value" lines (viscosity, rho, velocity, collision, les, cs, les_tau_mode,
 //               precision = <fp64|fp32|bf16>, closure = <file>, mlp_operands = <fp16|bf16>,
+//               checkpoint_in = <file> (start from it), checkpoint_out = <file> (write it at the end),
 //               bc.xmin ... bc.zmax = <bounce_back|periodic|nee> [u v w])
 //   closure file (les = learned): whitespace-separated numbers, the 1409 network parameters,
 //               then 9 each of input mean, spread, training minimum and maximum, then the output
@@ -156,10 +157,24 @@ int main(int argc, char **argv) {
                 lbm->SetDomainBoundaryCondition(dir_of(key.substr(3)), s, v);
             }
         }
+        std::string checkpoint_in, checkpoint_out;
+        {
+            std::ifstream pin2(argv[2]);
+            while (std::getline(pin2, line)) {
+                std::istringstream ls(line);
+                std::string key, eq, val;
+                if (ls >> key >> eq >> val && eq == "=") {
+                    if (key == "checkpoint_in") checkpoint_in = val;
+                    if (key == "checkpoint_out") checkpoint_out = val;
+                }
+            }
+        }
         solver->SolverInitial();
+        if (!checkpoint_in.empty()) lbm->ReadCheckPointGridNodes(checkpoint_in);
         std::cout << solver->GetSolverMethod() << std::endl;
         const int steps = std::atoi(argv[3]);
         for (int t = 0; t < steps; ++t) TimeMarching(&gm, (DefAmrLUint)t);
+        if (!checkpoint_out.empty()) lbm->WriteCheckPointGridNodes(checkpoint_out);
         std::ofstream out(argv[4], std::ios::binary);
         for (int l = 0; l < nlev; ++l) {
             lbm->CopyPopulationsToHost(l);

@@ -3,6 +3,9 @@
 #include "lbm_hip_solver.h"
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <tuple>
 
 namespace rootproject {
 namespace lbmproject {
@@ -14,6 +17,25 @@ SolverLbmHip *solver_of(const amrproject::GridInfoInterface &g) {
     if (!s) throw ZlhError("grid information has no LBM HIP solver");
     return s;
 }
+
+// index along axis a of a Morton key (x at bit dims i, y at dims i + 1, z at dims i + 2;
+// SFBitsetComputeIndices, grid/morton_aux.cpp:20-119, 336-351)
+uint32_t morton_axis(uint64_t key, int dims, int a) {
+    uint32_t v = 0;
+    for (int i = 0; a + dims * i < 64; ++i) v |= (uint32_t)((key >> (a + dims * i)) & 1u) << i;
+    return v;
+}
+
+// checkpoint record order of one node: background cell, level, then z, y, x inside the cell
+struct CheckpointOrder {
+    uint64_t cell;
+    int32_t level;
+    uint32_t z, y, x;
+    uint64_t key;
+    bool operator<(const CheckpointOrder &o) const {
+        return std::tie(cell, level, z, y, x) < std::tie(o.cell, o.level, o.z, o.y, o.x);
+    }
+};
 }  // namespace
 
 // ---------------------------------------------------------------------------------------------
@@ -271,6 +293,89 @@ void SolverLbmHip::CopyPopulationsToHost(DefInt level) {
         node->f_.assign(f.begin() + i * k0NumQ_, f.begin() + (i + 1) * k0NumQ_);
         node->rho_ = rho[i];
         node->velocity_.assign(u.begin() + i * k0SolverDims_, u.begin() + (i + 1) * k0SolverDims_);
+    }
+}
+
+void SolverLbmHip::WriteCheckPointGridNodes(const std::string &file_name) {
+    if (!ctx_) BuildDevice();
+    amrproject::GridManagerInterface &gm = *ptr_grid_manager_;
+    const int dims = k0SolverDims_;
+    std::vector<CheckpointOrder> order;
+    for (DefInt l = 0; l <= gm.k0MaxLevel_; ++l) {
+        CopyPopulationsToHost(l);
+        const uint32_t mask = (1u << l) - 1u;
+        for (auto &kv : gm.vec_ptr_grid_info_.at(l)->map_grid_node_) {
+            const uint64_t key = kv.first.to_ullong();
+            order.push_back({key >> (dims * l), (int32_t)l, dims == 3 ? morton_axis(key, dims, 2) & mask : 0u,
+                             morton_axis(key, dims, 1) & mask, morton_axis(key, dims, 0) & mask, key});
+        }
+    }
+    std::sort(order.begin(), order.end());
+    const int32_t nforce = bool_forces_ ? k0NumForces_ : 0;
+    const int32_t payload = (int32_t)(sizeof(DefInt) + sizeof(double) * (k0NumQ_ + nforce));
+    std::ofstream out(file_name, std::ios::binary);
+    if (!out) throw ZlhError("cannot write checkpoint " + file_name);
+    const uint32_t count = (uint32_t)order.size();   // DefAmrLUint
+    out.write(reinterpret_cast<const char *>(&count), sizeof(count));
+    out.write(reinterpret_cast<const char *>(&payload), sizeof(payload));
+    const std::vector<double> zeros(nforce, 0.);
+    for (const CheckpointOrder &r : order) {
+        const auto &node = *dynamic_cast<const GridNodeLbmHip *>(
+            gm.vec_ptr_grid_info_.at(r.level)->map_grid_node_.at(DefSFBitset(r.key)).get());
+        const int32_t flag = node.flag_status_;
+        out.write(reinterpret_cast<const char *>(&r.key), sizeof(uint64_t));
+        out.write(reinterpret_cast<const char *>(&r.level), sizeof(int32_t));
+        out.write(reinterpret_cast<const char *>(&flag), sizeof(int32_t));
+        out.write(reinterpret_cast<const char *>(node.f_.data()), sizeof(double) * k0NumQ_);
+        if (nforce) out.write(reinterpret_cast<const char *>(zeros.data()), sizeof(double) * nforce);
+    }
+    if (!out) throw ZlhError("checkpoint " + file_name + ": write failed");
+}
+
+void SolverLbmHip::ReadCheckPointGridNodes(const std::string &file_name) {
+    amrproject::GridManagerInterface &gm = *ptr_grid_manager_;
+    std::ifstream in(file_name, std::ios::binary);
+    if (!in) throw ZlhError("cannot read checkpoint " + file_name);
+    uint32_t count = 0;
+    int32_t payload = 0;
+    in.read(reinterpret_cast<char *>(&count), sizeof(count));
+    in.read(reinterpret_cast<char *>(&payload), sizeof(payload));
+    const int32_t nforce = bool_forces_ ? k0NumForces_ : 0;
+    if (!in || payload != (int32_t)(sizeof(DefInt) + sizeof(double) * (k0NumQ_ + nforce)))
+        throw ZlhError("checkpoint " + file_name + ": payload size does not match this solver's lattice and forces");
+    size_t nodes = 0;
+    for (DefInt l = 0; l <= gm.k0MaxLevel_; ++l) nodes += gm.vec_ptr_grid_info_.at(l)->map_grid_node_.size();
+    if (count != nodes) throw ZlhError("checkpoint " + file_name + ": node count differs from the grid");
+    std::vector<char> rec(sizeof(uint64_t) + sizeof(int32_t) + payload);
+    for (uint32_t i = 0; i < count; ++i) {
+        if (!in.read(rec.data(), (std::streamsize)rec.size())) throw ZlhError("checkpoint " + file_name + " is truncated");
+        uint64_t key;
+        int32_t level, flag;
+        std::memcpy(&key, rec.data(), sizeof(key));
+        std::memcpy(&level, rec.data() + 8, sizeof(level));
+        std::memcpy(&flag, rec.data() + 12, sizeof(flag));
+        if (level < 0 || level > gm.k0MaxLevel_) throw ZlhError("checkpoint " + file_name + ": bad level");
+        auto &map = gm.vec_ptr_grid_info_.at(level)->map_grid_node_;
+        auto it = map.find(DefSFBitset(key));
+        if (it == map.end()) throw ZlhError("checkpoint " + file_name + ": node not in the grid");
+        auto *node = dynamic_cast<GridNodeLbmHip *>(it->second.get());
+        if (!node) throw ZlhError("checkpoint " + file_name + ": node is not an LBM node");
+        node->flag_status_ = flag;
+        node->f_.resize(k0NumQ_);
+        std::memcpy(node->f_.data(), rec.data() + 16, sizeof(double) * k0NumQ_);
+    }
+    if (!ctx_) return;   // BuildDevice uploads the mirrors
+    for (DefInt l = 0; l <= gm.k0MaxLevel_; ++l) {
+        auto &map = gm.vec_ptr_grid_info_.at(l)->map_grid_node_;
+        std::vector<std::pair<uint64_t, const GridNodeLbmHip *>> nodes_l;
+        for (auto &kv : map) nodes_l.emplace_back(kv.first.to_ullong(), dynamic_cast<const GridNodeLbmHip *>(kv.second.get()));
+        std::sort(nodes_l.begin(), nodes_l.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+        std::vector<double> f(nodes_l.size() * k0NumQ_);
+        for (size_t i = 0; i < nodes_l.size(); ++i) {
+            if ((int)nodes_l[i].second->f_.size() != k0NumQ_) throw ZlhError("checkpoint misses a node");
+            std::copy(nodes_l[i].second->f_.begin(), nodes_l[i].second->f_.end(), f.begin() + i * k0NumQ_);
+        }
+        Check(zlh_upload_populations(ctx_, l, f.data()), "zlh_upload_populations");
     }
 }
 

@@ -126,6 +126,21 @@ class SolverLbmHip : public amrproject::SolverInterface {
     void BuildDevice();
     // copy the device populations of a level into the node mirrors (GridNodeLbmHip::f_, rho_, velocity_)
     void CopyPopulationsToHost(DefInt level);
+    // checkpoint of every level's nodes in the record format of the reference's grid_nodes file
+    // for one rank (MpiManager::WriteCheckPointGridNodes, mpi/mpi_checkpoint.cpp:272-390):
+    // header {uint32 node count, int32 payload bytes}, then per node {uint64 Morton key, int32
+    // level, payload}, payload = GridNodeLbm::CopyANodeToBufferForCheckpoint
+    // (lbm_interface.h:57-67): int32 flag, Q doubles of post-stream populations, and, with forces,
+    // GetNumForces() doubles.  Records are ordered as the reference writes them: by background
+    // cell (Morton code ascending), then level, then z, y, x inside the cell
+    // (SFBitsetHigherLevelInACell, grid/sfbitset_aux.cpp:140-156, 1958-1985).  The force block is
+    // written as zeros: this solver recomputes node forces from the populations every step.
+    void WriteCheckPointGridNodes(const std::string &file_name);
+    // ReadCheckPointGridNodes (mpi/mpi_checkpoint.cpp:392-520) for one rank: every record's flag
+    // and populations go to the node mirrors (ReadANodeFromBufferForCheckpoint), and then to the
+    // device (uploaded now if it is built, else by BuildDevice).  Every node of the grid must
+    // have a record, and every record a node.
+    void ReadCheckPointGridNodes(const std::string &file_name);
     zlh_ctx *Context() const { return ctx_; }
 
  private:
