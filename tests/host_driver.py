@@ -74,7 +74,7 @@ def write_params(path, kw, dims, collision="srt", precision="fp64", closure_path
 
 
 def run(grid, kw, steps, workdir, collision="srt", precision="fp64", model=None, checkpoint_in=None,
-        checkpoint_out=None):
+        checkpoint_out=None, vtu_out=None):
     gpath = os.path.join(workdir, "grid.bin")
     ppath = os.path.join(workdir, "params.txt")
     opath = os.path.join(workdir, "out.bin")
@@ -87,6 +87,8 @@ def run(grid, kw, steps, workdir, collision="srt", precision="fp64", model=None,
         extra.append("checkpoint_in = " + checkpoint_in)
     if checkpoint_out:
         extra.append("checkpoint_out = " + checkpoint_out)
+    if vtu_out:
+        extra.append("vtu_out = " + vtu_out)
     write_params(ppath, kw, grid.dims, collision, precision, cpath, extra)
     proc = subprocess.run([DRIVER, gpath, ppath, str(steps), opath], capture_output=True, text=True, timeout=300)
     if proc.returncode != 0:

@@ -131,3 +131,48 @@ def test_checkpoint_order_rule_matches_reference_loops(case):
                          int(cc[0]) & m, int(k)))
     recs.sort()
     assert [(r[1], r[5]) for r in recs] == host_driver.reference_record_order(grid)
+
+
+def _vtu_arrays(path):
+    import xml.etree.ElementTree as ET
+    piece = ET.parse(path).getroot().find("UnstructuredGrid/Piece")
+    arrays = {}
+    for da in piece.iter("DataArray"):
+        name = da.get("Name") or "points"
+        dtype = {"Int32": np.int32, "Int64": np.int64, "UInt8": np.uint8, "Float32": np.float32}[da.get("type")]
+        arrays[name] = np.array(da.text.split(), dtype=np.float64).astype(dtype)
+    return int(piece.get("NumberOfPoints")), int(piece.get("NumberOfCells")), arrays
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["amr2d", "amr3d"])
+def test_driver_vtu_output(tmp_path, case):
+    """VTU files of every level: flags, FP32 density and velocity of the final populations, node
+    coordinates, and one pixel / voxel per node whose cell's corners are all nodes of the level."""
+    from zllbm_amd import grid as g
+    from tests.test_gpu_parity import _macros
+    grid, kw = sc.amr2d(levels=3) if case == "amr2d" else sc.amr3d()
+    dims = grid.dims
+    prefix = str(tmp_path / "flow")
+    proc, out = host_driver.run(grid, kw, 5, str(tmp_path), collision="mrt", vtu_out=prefix)
+    assert proc.returncode == 0, proc.stderr
+    pvtu = open(prefix + ".pvtu").read()
+    for l, lev in enumerate(grid.levels):
+        assert 'Source="flow_level%d.vtu"' % l in pvtu
+        npts, ncell, a = _vtu_arrays(prefix + "_level%d.vtu" % l)
+        assert npts == lev.n
+        np.testing.assert_array_equal(a["flag"], lev.flags)
+        rho, u = _macros(out[l], dims)
+        np.testing.assert_allclose(a["rho"], rho.astype(np.float32), rtol=2e-7)
+        np.testing.assert_allclose(a["velocity"].reshape(-1, dims), u.astype(np.float32), rtol=1e-5, atol=1e-9)
+        ijk = g.morton_decode(dims, lev.keys).astype(np.int64)
+        pts = a["points"].reshape(-1, 3)[:, :dims]
+        np.testing.assert_array_equal(pts, (ijk / float(1 << l)).astype(np.float32))
+        # cells: every node whose +1 neighbours along all axes (and diagonals) exist
+        pos = {tuple(c): i for i, c in enumerate(ijk)}
+        corners = [np.array([(v >> a) & 1 for a in range(dims)]) for v in range(1 << dims)]
+        expect = [[pos[tuple(c + d)] for d in corners] for c in ijk if all(tuple(c + d) in pos for d in corners)]
+        assert ncell == len(expect)
+        np.testing.assert_array_equal(a["connectivity"].reshape(-1, 1 << dims), np.array(expect).reshape(-1, 1 << dims))
+        assert np.all(a["types"] == (11 if dims == 3 else 8))
+        np.testing.assert_array_equal(a["offsets"], (1 << dims) * np.arange(1, ncell + 1))

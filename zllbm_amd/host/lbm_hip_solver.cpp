@@ -26,6 +26,15 @@ uint32_t morton_axis(uint64_t key, int dims, int a) {
     return v;
 }
 
+// Morton key of an index triple (SFBitsetEncoding, grid/morton_aux.cpp:274-286)
+uint64_t morton_key(int dims, uint32_t x, uint32_t y, uint32_t z) {
+    uint64_t k = 0;
+    const uint32_t c[3] = {x, y, z};
+    for (int i = 0; dims * i < 64; ++i)
+        for (int a = 0; a < dims && a + dims * i < 64; ++a) k |= (uint64_t)((c[a] >> i) & 1u) << (a + dims * i);
+    return k;
+}
+
 // checkpoint record order of one node: background cell, level, then z, y, x inside the cell
 struct CheckpointOrder {
     uint64_t cell;
@@ -377,6 +386,85 @@ void SolverLbmHip::ReadCheckPointGridNodes(const std::string &file_name) {
         }
         Check(zlh_upload_populations(ctx_, l, f.data()), "zlh_upload_populations");
     }
+}
+
+void SolverLbmHip::WriteVtu(const std::string &prefix) {
+    if (!ctx_) BuildDevice();
+    amrproject::GridManagerInterface &gm = *ptr_grid_manager_;
+    const int dims = k0SolverDims_;
+    std::vector<std::string> pieces;
+    for (DefInt l = 0; l <= gm.k0MaxLevel_; ++l) {
+        CopyPopulationsToHost(l);
+        auto &map = gm.vec_ptr_grid_info_.at(l)->map_grid_node_;
+        std::vector<uint64_t> keys;
+        keys.reserve(map.size());
+        for (auto &kv : map) keys.push_back(kv.first.to_ullong());
+        std::sort(keys.begin(), keys.end());
+        auto index_of = [&](uint64_t k) -> int64_t {
+            auto it = std::lower_bound(keys.begin(), keys.end(), k);
+            return (it != keys.end() && *it == k) ? (int64_t)(it - keys.begin()) : -1;
+        };
+        // cells: corner order of VTK_PIXEL (x fastest, then y) and VTK_VOXEL (then z)
+        std::vector<int64_t> conn;
+        const int ncorner = dims == 3 ? 8 : 4;
+        for (uint64_t k : keys) {
+            const uint32_t x = morton_axis(k, dims, 0), y = morton_axis(k, dims, 1);
+            const uint32_t z = dims == 3 ? morton_axis(k, dims, 2) : 0u;
+            int64_t c[8];
+            bool full = true;
+            for (int v = 0; v < ncorner && full; ++v) {
+                c[v] = index_of(morton_key(dims, x + (v & 1), y + ((v >> 1) & 1), z + ((v >> 2) & 1)));
+                full = c[v] >= 0;
+            }
+            if (full) conn.insert(conn.end(), c, c + ncorner);
+        }
+        const size_t ncell = conn.size() / ncorner;
+        const std::string file = prefix + "_level" + std::to_string(l) + ".vtu";
+        std::FILE *fp = std::fopen(file.c_str(), "w");
+        if (!fp) throw ZlhError("cannot write " + file);
+        std::fprintf(fp, "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\">\n");
+        std::fprintf(fp, " <UnstructuredGrid>\n  <Piece NumberOfPoints=\"%zu\" NumberOfCells=\"%zu\">\n", keys.size(), ncell);
+        std::fprintf(fp, "    <PointData>\n      <DataArray type=\"Int32\" Name=\"flag\" format=\"ascii\">\n");
+        for (uint64_t k : keys) std::fprintf(fp, " %d", (int)map.at(DefSFBitset(k))->flag_status_);
+        std::fprintf(fp, "\n      </DataArray>\n      <DataArray type=\"Float32\" Name=\"rho\" format=\"ascii\">\n");
+        for (uint64_t k : keys) {
+            const auto *n = dynamic_cast<const GridNodeLbmHip *>(map.at(DefSFBitset(k)).get());
+            std::fprintf(fp, " %.9g", (double)(float)n->rho_);
+        }
+        std::fprintf(fp, "\n      </DataArray>\n      <DataArray type=\"Float32\" Name=\"velocity\" NumberOfComponents=\"%d\" format=\"ascii\">\n", dims);
+        for (uint64_t k : keys) {
+            const auto *n = dynamic_cast<const GridNodeLbmHip *>(map.at(DefSFBitset(k)).get());
+            for (int a = 0; a < dims; ++a) std::fprintf(fp, " %.9g", (double)(float)n->velocity_[a]);
+        }
+        std::fprintf(fp, "\n      </DataArray>\n    </PointData>\n");
+        std::fprintf(fp, "   <Points>\n    <DataArray type=\"Float32\" NumberOfComponents=\"3\" format=\"ascii\">\n");
+        const double h = 1.0 / (double)(1u << l);
+        for (uint64_t k : keys)
+            std::fprintf(fp, " %.9g %.9g %.9g", (double)(float)(morton_axis(k, dims, 0) * h),
+                         (double)(float)(morton_axis(k, dims, 1) * h),
+                         dims == 3 ? (double)(float)(morton_axis(k, dims, 2) * h) : 0.);
+        std::fprintf(fp, "\n    </DataArray>\n   </Points>\n   <Cells>\n");
+        std::fprintf(fp, "    <DataArray type=\"Int64\" Name=\"connectivity\" format=\"ascii\">\n");
+        for (int64_t c : conn) std::fprintf(fp, " %lld", (long long)c);
+        std::fprintf(fp, "\n    </DataArray>\n    <DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\">\n");
+        for (size_t i = 1; i <= ncell; ++i) std::fprintf(fp, " %lld", (long long)(i * ncorner));
+        std::fprintf(fp, "\n    </DataArray>\n    <DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n");
+        for (size_t i = 0; i < ncell; ++i) std::fprintf(fp, " %d", dims == 3 ? 11 : 8);
+        std::fprintf(fp, "\n    </DataArray>\n   </Cells>\n  </Piece>\n </UnstructuredGrid>\n</VTKFile>\n");
+        if (std::fclose(fp) != 0) throw ZlhError("write failed: " + file);
+        pieces.push_back(file.substr(file.find_last_of('/') + 1));
+    }
+    const std::string pfile = prefix + ".pvtu";
+    std::FILE *fp = std::fopen(pfile.c_str(), "w");
+    if (!fp) throw ZlhError("cannot write " + pfile);
+    std::fprintf(fp, "<?xml version=\"1.0\"?>\n<VTKFile type=\"PUnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\">\n");
+    std::fprintf(fp, " <PUnstructuredGrid GhostLevel=\"0\">\n  <PPointData>\n");
+    std::fprintf(fp, "   <PDataArray type=\"Int32\" Name=\"flag\"/>\n   <PDataArray type=\"Float32\" Name=\"rho\"/>\n");
+    std::fprintf(fp, "   <PDataArray type=\"Float32\" Name=\"velocity\" NumberOfComponents=\"%d\"/>\n  </PPointData>\n", dims);
+    std::fprintf(fp, "  <PPoints>\n   <PDataArray type=\"Float32\" NumberOfComponents=\"3\"/>\n  </PPoints>\n");
+    for (const std::string &p : pieces) std::fprintf(fp, "  <Piece Source=\"%s\"/>\n", p.c_str());
+    std::fprintf(fp, " </PUnstructuredGrid>\n</VTKFile>\n");
+    if (std::fclose(fp) != 0) throw ZlhError("write failed: " + pfile);
 }
 
 // AmrManager::TimeMarching (amr_manager.cpp:279-300)

@@ -141,6 +141,14 @@ class SolverLbmHip : public amrproject::SolverInterface {
     // device (uploaded now if it is built, else by BuildDevice).  Every node of the grid must
     // have a record, and every record a node.
     void ReadCheckPointGridNodes(const std::string &file_name);
+    // VTU output of every level (VtkWriterManager::WriteVtuAll, io/vtk_writer_manager.cpp:220-560,
+    // with the LBM variables of GridInfoLbmInteface::SetupOutputVariables / OutputOneVariable,
+    // lbm_grid_interface.cpp:614-732): <prefix>_level<l>.vtu per level, an UnstructuredGrid in
+    // ASCII with point data flag (Int32), rho and velocity (Float32, dims components), points in
+    // background units (Float32, 3 components), and a VTK_PIXEL / VTK_VOXEL cell at every node
+    // whose cell's other corners are nodes of the level; <prefix>.pvtu lists the level files.
+    // Nodes are written in Morton key order.
+    void WriteVtu(const std::string &prefix);
     zlh_ctx *Context() const { return ctx_; }
 
  private:
