@@ -176,3 +176,48 @@ def test_driver_vtu_output(tmp_path, case):
         np.testing.assert_array_equal(a["connectivity"].reshape(-1, 1 << dims), np.array(expect).reshape(-1, 1 << dims))
         assert np.all(a["types"] == (11 if dims == 3 else 8))
         np.testing.assert_array_equal(a["offsets"], (1 << dims) * np.arange(1, ncell + 1))
+
+
+def _write_checkpoint(path, grid, q, records=None, payload=None):
+    """A grid-node checkpoint of the grid at rest (f = 1/q per direction; values do not matter
+    here), with optional record list override and header payload size."""
+    rec = np.dtype([("key", "<u8"), ("level", "<i4"), ("flag", "<i4"), ("f", "<f8", (q,))])
+    if records is None:
+        records = [(int(k), l, int(fl)) for l, lev in enumerate(grid.levels) for k, fl in zip(lev.keys, lev.flags)]
+    data = np.zeros(len(records), rec)
+    for i, (k, l, fl) in enumerate(records):
+        data[i] = (k, l, fl, np.full(q, 1.0 / q))
+    with open(path, "wb") as fh:
+        np.array([len(records)], np.uint32).tofile(fh)
+        np.array([4 + 8 * q if payload is None else payload], np.int32).tofile(fh)
+        data.tofile(fh)
+
+
+@pytest.mark.parametrize("defect", ["payload", "duplicate", "foreign", "truncated"])
+def test_driver_rejects_bad_checkpoint(tmp_path, defect):
+    """A checkpoint that does not fit the grid stops the driver with a message before any device
+    work: a wrong payload size, a node recorded twice (and another not at all), a key the grid
+    does not have, a truncated file."""
+    grid, kw = sc.amr2d(levels=2)
+    q = 9
+    recs = [(int(k), l, int(fl)) for l, lev in enumerate(grid.levels) for k, fl in zip(lev.keys, lev.flags)]
+    path = str(tmp_path / "ck")
+    if defect == "payload":
+        _write_checkpoint(path, grid, q, payload=4 + 8 * q + 8)
+        msg = "payload size"
+    elif defect == "duplicate":
+        recs[1] = recs[0]
+        _write_checkpoint(path, grid, q, recs)
+        msg = "two records"
+    elif defect == "foreign":
+        recs[0] = (int(recs[0][0]) | (1 << 62), recs[0][1], recs[0][2])
+        _write_checkpoint(path, grid, q, recs)
+        msg = "not in the grid"
+    else:
+        _write_checkpoint(path, grid, q)
+        with open(path, "r+b") as fh:
+            fh.truncate(8 + 50 * (16 + 8 * q))
+        msg = "truncated"
+    proc, out = host_driver.run(grid, kw, 1, str(tmp_path), checkpoint_in=path)
+    assert proc.returncode == 1 and out is None
+    assert msg in proc.stderr, proc.stderr

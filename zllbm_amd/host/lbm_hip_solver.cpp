@@ -329,8 +329,10 @@ void SolverLbmHip::WriteCheckPointGridNodes(const std::string &file_name) {
     out.write(reinterpret_cast<const char *>(&payload), sizeof(payload));
     const std::vector<double> zeros(nforce, 0.);
     for (const CheckpointOrder &r : order) {
-        const auto &node = *dynamic_cast<const GridNodeLbmHip *>(
+        const auto *ptr = dynamic_cast<const GridNodeLbmHip *>(
             gm.vec_ptr_grid_info_.at(r.level)->map_grid_node_.at(DefSFBitset(r.key)).get());
+        if (!ptr || (int)ptr->f_.size() != k0NumQ_) throw ZlhError("checkpoint: a node is not an LBM node");
+        const GridNodeLbmHip &node = *ptr;
         const int32_t flag = node.flag_status_;
         out.write(reinterpret_cast<const char *>(&r.key), sizeof(uint64_t));
         out.write(reinterpret_cast<const char *>(&r.level), sizeof(int32_t));
@@ -356,6 +358,7 @@ void SolverLbmHip::ReadCheckPointGridNodes(const std::string &file_name) {
     for (DefInt l = 0; l <= gm.k0MaxLevel_; ++l) nodes += gm.vec_ptr_grid_info_.at(l)->map_grid_node_.size();
     if (count != nodes) throw ZlhError("checkpoint " + file_name + ": node count differs from the grid");
     std::vector<char> rec(sizeof(uint64_t) + sizeof(int32_t) + payload);
+    std::vector<std::vector<uint64_t>> seen(gm.k0MaxLevel_ + 1);
     for (uint32_t i = 0; i < count; ++i) {
         if (!in.read(rec.data(), (std::streamsize)rec.size())) throw ZlhError("checkpoint " + file_name + " is truncated");
         uint64_t key;
@@ -369,9 +372,16 @@ void SolverLbmHip::ReadCheckPointGridNodes(const std::string &file_name) {
         if (it == map.end()) throw ZlhError("checkpoint " + file_name + ": node not in the grid");
         auto *node = dynamic_cast<GridNodeLbmHip *>(it->second.get());
         if (!node) throw ZlhError("checkpoint " + file_name + ": node is not an LBM node");
+        seen[level].push_back(key);
         node->flag_status_ = flag;
         node->f_.resize(k0NumQ_);
         std::memcpy(node->f_.data(), rec.data() + 16, sizeof(double) * k0NumQ_);
+    }
+    // with the count equal to the grid's, a repeated record means a node without one
+    for (auto &v : seen) {
+        std::sort(v.begin(), v.end());
+        if (std::adjacent_find(v.begin(), v.end()) != v.end())
+            throw ZlhError("checkpoint " + file_name + ": a node has two records");
     }
     if (!ctx_) return;   // BuildDevice uploads the mirrors
     for (DefInt l = 0; l <= gm.k0MaxLevel_; ++l) {
@@ -427,15 +437,20 @@ void SolverLbmHip::WriteVtu(const std::string &prefix) {
         std::fprintf(fp, "    <PointData>\n      <DataArray type=\"Int32\" Name=\"flag\" format=\"ascii\">\n");
         for (uint64_t k : keys) std::fprintf(fp, " %d", (int)map.at(DefSFBitset(k))->flag_status_);
         std::fprintf(fp, "\n      </DataArray>\n      <DataArray type=\"Float32\" Name=\"rho\" format=\"ascii\">\n");
+        std::vector<const GridNodeLbmHip *> nodes;
+        nodes.reserve(keys.size());
         for (uint64_t k : keys) {
             const auto *n = dynamic_cast<const GridNodeLbmHip *>(map.at(DefSFBitset(k)).get());
-            std::fprintf(fp, " %.9g", (double)(float)n->rho_);
+            if (!n || (int)n->velocity_.size() < dims) {
+                std::fclose(fp);
+                throw ZlhError("VTU output: a node of level " + std::to_string(l) + " is not an LBM node");
+            }
+            nodes.push_back(n);
         }
+        for (const GridNodeLbmHip *n : nodes) std::fprintf(fp, " %.9g", (double)(float)n->rho_);
         std::fprintf(fp, "\n      </DataArray>\n      <DataArray type=\"Float32\" Name=\"velocity\" NumberOfComponents=\"%d\" format=\"ascii\">\n", dims);
-        for (uint64_t k : keys) {
-            const auto *n = dynamic_cast<const GridNodeLbmHip *>(map.at(DefSFBitset(k)).get());
+        for (const GridNodeLbmHip *n : nodes)
             for (int a = 0; a < dims; ++a) std::fprintf(fp, " %.9g", (double)(float)n->velocity_[a]);
-        }
         std::fprintf(fp, "\n      </DataArray>\n    </PointData>\n");
         std::fprintf(fp, "   <Points>\n    <DataArray type=\"Float32\" NumberOfComponents=\"3\" format=\"ascii\">\n");
         const double h = 1.0 / (double)(1u << l);
