@@ -6,10 +6,11 @@ For every configuration:
 * the FP32 shifted-population solver against the FP64 solver, both on the GPU, stepped in lockstep
   from the same initial state at the configuration's full size; likewise the BF16-storage solver
   for the configurations without the learned closure (1 to 3);
-* the FP64 GPU solver against the FP64 oracle on the CPU where the oracle finishes in a minute or
-  so: configurations 1 and 2 at full size and length, and configuration 4's closure on a
-  2-level 24^3 box. (Elsewhere the parity tests pin the FP64 solver to the oracle at smaller
-  sizes, to 1e-11.)
+* the FP64 GPU solver against the FP64 oracle on the CPU (16 threads) where the oracle finishes
+  in a minute or so: configurations 1 and 2 at full size and length, configuration 3's TGV at
+  64^3 (same u0, Re and noise) for 100 steps, configuration 4's closure on a 2-level 24^3 box
+  for 20 steps, and configuration 5 at full size (5 levels, IB sphere, learned closure) for 10
+  background steps.
 
 Throughput: MLUPS (node updates of all levels per background step) of the FP32 and FP64 forms,
 from graph-replayed background steps with the populations resident on the GPU.
@@ -106,6 +107,8 @@ def fp32_vs_fp64(W, steps, every, model=None, precision=hip.FP32_SHIFTED):
 
 
 def gpu_vs_oracle(W, steps, model=None):
+    from oracle.zlbm_oracle import lib as oracle_lib
+    oracle_lib().zo_set_threads(min(16, os.cpu_count() or 1))
     s = gpu_solver(W, hip.FP64, model)
     o = oracle_solver(W, model)
     t0 = time.perf_counter()
@@ -171,6 +174,8 @@ def main():
         rec = {"config": 3, "workload": W.name, "throughput": throughput(W, 200),
                "fp32_vs_fp64": summary(fp32_vs_fp64(W, 2000, 500)),
                "bf16_vs_fp64": summary(fp32_vs_fp64(W, 2000, 500, precision=hip.BF16_SHIFTED))}
+        small = workloads.tgv(64)
+        rec["fp64_vs_oracle"] = dict(gpu_vs_oracle(small, 100), workload=small.name)
         print(json.dumps(rec), flush=True)
     if 4 in want:
         W = workloads.decaying_turbulence(32, 4)
@@ -186,7 +191,8 @@ def main():
         W = workloads.sphere_amr()
         log("config 5:", W.name, W.nodes)
         rec = {"config": 5, "workload": W.name, "throughput": throughput(W, 200, model),
-               "fp32_vs_fp64": summary(fp32_vs_fp64(W, 400, 100, model))}
+               "fp32_vs_fp64": summary(fp32_vs_fp64(W, 400, 100, model)),
+               "fp64_vs_oracle": gpu_vs_oracle(W, 10, model)}
         print(json.dumps(rec), flush=True)
 
 
