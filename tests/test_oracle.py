@@ -3,8 +3,8 @@
 The reference ships no tests or golden vectors and its build needs MPI and a C++23 header the
 image lacks, so the oracle is checked against: constants restated from the reference's headers
 (typed below as golden values with their file:line), closed forms and invariants of the lattice
-Boltzmann method, an independent numpy restatement for periodic grids, and behaviour observed
-on the reference binary during the survey.
+Boltzmann method, an independent numpy restatement for periodic grids, and a behaviour of the
+reference's shipped Smagorinsky mode that follows from its relaxation-time formula.
 """
 import numpy as np
 import pytest
@@ -237,9 +237,9 @@ def test_smagorinsky_tau_closed_form():
 
 
 def test_reference_smagorinsky_blows_up():
-    """Survey observation on the reference binary: D3Q19 8^3 periodic TGV, nu 0.01, u0 0.05,
-    LES on (shipped tau_eff = tau_t, Cs 0.16), 20 steps -> every population non-finite, for SRT
-    and MRT; with LES off every population stays finite."""
+    """The shipped Smagorinsky mode sets tau_eff = tau_t (lbm_solver_interface.cpp:276-295), which
+    is zero at equilibrium: D3Q19 8^3 periodic TGV, nu 0.01, u0 0.05, Cs 0.16, 20 steps -> every
+    population non-finite, for SRT and MRT; with LES off every population stays finite."""
     grid, kw = sc.periodic_tgv(3, 8, nu=0.01)
     rho, u = sc.tgv_fields(grid, 0.05)
     for coll in (0, 1):
@@ -392,7 +392,8 @@ def _reference_order_loop(max_level):
 def test_time_stepping_order():
     assert list(zo.time_stepping_order(0)) == [0]
     assert list(zo.time_stepping_order(1)) == [0, 1, 1]
-    assert list(zo.time_stepping_order(2)) == [0, 1, 2, 2, 1, 2, 2]      # survey section 3.2
+    # MultiTimeSteppingC2F (grid_manager.cpp:997-1024): each coarse step, then two of the next level
+    assert list(zo.time_stepping_order(2)) == [0, 1, 2, 2, 1, 2, 2]
     for ml in range(2, 7):
         assert list(zo.time_stepping_order(ml)) == _reference_order_loop(ml)
 
