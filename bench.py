@@ -125,6 +125,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--workload", default="amr4", choices=["amr4", "tgv256"])
+    ap.add_argument("--collision", default="mrt", choices=["mrt", "srt"],
+                    help="collision operator of every level (srt: to price the MRT moment transform)")
     ap.add_argument("--populations", default="fp32", choices=["fp32", "bf16"],
                     help="storage of the shifted populations (bf16: tgv256 only; no learned closure)")
     ap.add_argument("--closure", default=DEFAULT_CLOSURE, help="trained closure (zllbm_amd.train_closure)")
@@ -165,6 +167,8 @@ def main():
     else:
         W = workloads.tgv(256)
         les = hip.LES_SMAGORINSKY
+    if args.collision == "srt":
+        W.solver["collision"] = (hip.SRT,)
     bf16 = args.populations == "bf16"
     if bf16 and model is not None:
         print("bench.py: BF16 populations do not run with the learned closure (use --workload tgv256)", file=sys.stderr)
@@ -187,7 +191,7 @@ def main():
     kernel = ("k_stream_collide_mlp" if model is not None else "k_stream_collide") + " (level %d)" % finest
     out.update({"value": round(mlups, 2), "unit": "MLUPS", "ms_per_step": round(wall / args.steps * 1e3, 4),
                 "config": {"workload": W.name, "nodes_per_level": W.nodes, "updates_per_step": W.updates_per_step,
-                           "collision": "MRT", "closure": "learned (%s-operand MFMA), Smagorinsky fallback" % args.mlp_operands.upper()
+                           "collision": args.collision.upper(), "closure": "learned (%s-operand MFMA), Smagorinsky fallback" % args.mlp_operands.upper()
                            if model is not None else "Smagorinsky Cs 0.1",
                            "populations": "BF16 shifted" if bf16 else "FP32 shifted"},
                 "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
