@@ -39,6 +39,16 @@ def log(*a):
 
 MLP_OPERANDS = "fp16"
 
+# HBM bytes per launch of the roofline kernel, measured by rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs of this script on the same workload and options, with the counters scaled by
+# what they read for a plain 1 GiB copy (bytes = 2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE); the
+# summaries, with their commands, are the named files under profiles/
+PROFILED_TRAFFIC = {
+    ("amr4", "mrt", "fp32", "fp16"): (584.4e6, "profiles/k_stream_collide_mlp_amr4.md"),
+    ("tgv256", "mrt", "fp32", None): (2884.4e6, "profiles/k_stream_collide_tgv256.md"),
+    ("tgv256", "mrt", "bf16", None): (1492.6e6, "profiles/k_stream_collide_bf16_tgv256.md"),
+}
+
 
 def make_solver(hip, W, precision, les, model=None):
     kw = dict(W.solver)
@@ -189,13 +199,16 @@ def main():
         "BF16" if bf16 else "FP32", "learned closure" if model is not None else "Smagorinsky",
         wall / args.steps * 1e3, avg_s * 1e3, k_n))
     kernel = ("k_stream_collide_mlp" if model is not None else "k_stream_collide") + " (level %d)" % finest
+    traffic, traffic_src = PROFILED_TRAFFIC.get(
+        (args.workload, args.collision, args.populations, args.mlp_operands if model is not None else None), (None, None))
     out.update({"value": round(mlups, 2), "unit": "MLUPS", "ms_per_step": round(wall / args.steps * 1e3, 4),
                 "config": {"workload": W.name, "nodes_per_level": W.nodes, "updates_per_step": W.updates_per_step,
                            "collision": args.collision.upper(), "closure": "learned (%s-operand MFMA), Smagorinsky fallback" % args.mlp_operands.upper()
                            if model is not None else "Smagorinsky Cs 0.1",
                            "populations": "BF16 shifted" if bf16 else "FP32 shifted"},
                 "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+                             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                             "traffic_unit": "bytes per launch", "traffic_source": traffic_src, "kernel": kernel,
                              "avg_launch_ms": round(avg_s * 1e3, 4), "bytes_per_update": bpu,
                              "updates_per_launch": W.nodes[finest]}})
     if model is not None:
