@@ -166,6 +166,9 @@ def main():
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.populations == "bf16" and args.workload == "amr4":
+        print("bench.py: BF16 populations do not run with the learned closure (use --workload tgv256)", file=sys.stderr)
+        sys.exit(2)
     from zllbm_amd import closure, hip, workloads
     out = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic"}
@@ -180,9 +183,6 @@ def main():
     if args.collision == "srt":
         W.solver["collision"] = (hip.SRT,)
     bf16 = args.populations == "bf16"
-    if bf16 and model is not None:
-        print("bench.py: BF16 populations do not run with the learned closure (use --workload tgv256)", file=sys.stderr)
-        sys.exit(2)
     log("workload:", W.name, "nodes per level", W.nodes)
     s = make_solver(hip, W, hip.BF16_SHIFTED if bf16 else hip.FP32_SHIFTED, les, model)
     if args.one_stream:

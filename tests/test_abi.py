@@ -65,3 +65,24 @@ def test_binding_arity_matches_header():
     for name in hip.EXPORTED:
         assert name in arity, name
         assert len(getattr(L, name).argtypes) == arity[name], name
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with status 2 and says so, before any device work."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    proc = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                          text=True, timeout=120)
+    assert proc.returncode == 2
+    assert "one GPU" in proc.stderr
+
+
+def test_bench_refuses_bf16_with_learned_closure():
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    proc = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--populations", "bf16"],
+                          capture_output=True, text=True, timeout=300)
+    assert proc.returncode == 2
+    assert "learned closure" in proc.stderr
