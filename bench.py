@@ -9,6 +9,8 @@ Workloads (synthetic data, zllbm_amd.workloads):
           with the FP64 form of the same closure (velocity and density L2).
   tgv256  D3Q19 MRT + Smagorinsky (Cs 0.1), 256^3-cell periodic Taylor-Green vortex, FP32
           arithmetic on FP32 (default) or BF16 (--populations bf16) shifted populations.
+  sphere5 D3Q19 MRT + learned closure, 5 levels over a 32x16x16 channel (512x256x256
+          finest-equivalent), direct-forcing immersed sphere of 48 finest cells (7,239 markers).
 
 value = whole-job MLUPS (lattice-node updates per second / 1e6; a level-l node counts 2^l
 updates per background step), populations resident in HBM.
@@ -55,8 +57,12 @@ def make_solver(hip, W, precision, les, model=None):
     kw.update(les=les)
     if les == hip.LES_MLP:
         kw.update(mlp_operands=hip.MLP_OPERANDS_FP16 if MLP_OPERANDS == "fp16" else hip.MLP_OPERANDS_BF16)
-    s = hip.Solver(W.grid, precision=precision, **kw).build()
-    if model is not None:
+    s = hip.Solver(W.grid, precision=precision, **kw)
+    ib = getattr(W, "ib", None)
+    if ib is not None:
+        s.set_ib(ib.level, ib.xyz, ib.area, np.zeros_like(ib.xyz), stencil_dis=2)
+    s.build()
+    if model is not None and kw.get("les") == hip.LES_MLP:
         s.mlp_set(model)
     for lvl, (rho, u) in enumerate(W.init):
         s.init_macro(lvl, rho, u)
@@ -134,7 +140,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="amr4", choices=["amr4", "tgv256"])
+    ap.add_argument("--workload", default="amr4", choices=["amr4", "tgv256", "sphere5"])
     ap.add_argument("--collision", default="mrt", choices=["mrt", "srt"],
                     help="collision operator of every level (srt: to price the MRT moment transform)")
     ap.add_argument("--populations", default="fp32", choices=["fp32", "bf16"],
@@ -166,7 +172,7 @@ def main():
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
-    if args.populations == "bf16" and args.workload == "amr4":
+    if args.populations == "bf16" and args.workload != "tgv256":
         print("bench.py: BF16 populations do not run with the learned closure (use --workload tgv256)", file=sys.stderr)
         sys.exit(2)
     from zllbm_amd import closure, hip, workloads
@@ -175,6 +181,10 @@ def main():
     model = None
     if args.workload == "amr4":
         W = workloads.decaying_turbulence(32, 4)
+        model = closure.ClosureModel.load(args.closure)
+        les = hip.LES_MLP
+    elif args.workload == "sphere5":
+        W = workloads.sphere_amr()
         model = closure.ClosureModel.load(args.closure)
         les = hip.LES_MLP
     else:

@@ -265,15 +265,14 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// immersed boundary, steps 2-3: one wavefront per marker; lanes hold the stencil nodes.
-// Velocity interpolation by a wavefront reduction, Lagrangian force F = k (U - u_ref), and the
-// FP64 atomic spread F delta area / (dt dx^(d-1)) (DirectForcingScheme{2D,3D},
-// immersed_boundary.cpp:108-364)
+// immersed boundary, step 2: one wavefront per marker; lanes hold the stencil nodes.  Velocity
+// interpolation by a wavefront reduction and the Lagrangian force F = k (U - u_ref)
+// (DirectForcingScheme{2D,3D}, immersed_boundary.cpp:108-364)
 template <int D>
 __global__ __launch_bounds__(256) void k_ib_markers(int64_t n_markers, int support, const int32_t *node_ai,
                                                     const double *weight, const double *node_u,
-                                                    const double *marker_vel, const double *marker_scale,
-                                                    double stiffness, double *marker_force, double *aux_force) {
+                                                    const double *marker_vel, double stiffness,
+                                                    double *marker_force) {
     const int64_t m = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (m >= n_markers) return;
@@ -293,10 +292,31 @@ __global__ __launch_bounds__(256) void k_ib_markers(int64_t n_markers, int suppo
     }
     if (lane == 0)
         for (int a = 0; a < 3; ++a) marker_force[m * 3 + a] = fm[a];
-    if (ai >= 0) {
-        const double s = w * marker_scale[m];
-        for (int a = 0; a < D; ++a) atomicAdd(&aux_force[(int64_t)ai * 3 + a], fm[a] * s);
+}
+
+// immersed boundary, step 3: the spread F delta area / (dt dx^(d-1)) of every marker onto its
+// stencil nodes (immersed_boundary.cpp:108-364), as a gather: one thread per supported node sums
+// its markers' terms in FP64, in ascending marker order, and stores the node force.  Neighbouring
+// markers share most of their stencil nodes, so a per-marker scatter would serialise on atomics.
+template <int D>
+__global__ __launch_bounds__(256) void k_ib_spread(int64_t n_reset, const int32_t *__restrict__ reset_ai,
+                                                   const int32_t *__restrict__ begin,
+                                                   const int32_t *__restrict__ entry, int support,
+                                                   const double *__restrict__ weight,
+                                                   const double *__restrict__ marker_scale,
+                                                   const double *__restrict__ marker_force,
+                                                   double *__restrict__ aux_force) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_reset) return;
+    double acc[3] = {0., 0., 0.};
+    for (int32_t e = begin[r]; e < begin[r + 1]; ++e) {
+        const int32_t k = entry[e];
+        const int32_t m = k / support;
+        const double s = weight[k] * marker_scale[m];
+        for (int a = 0; a < D; ++a) acc[a] += marker_force[(int64_t)m * 3 + a] * s;
     }
+    const int64_t ai = reset_ai[r];
+    for (int a = 0; a < D; ++a) aux_force[ai * 3 + a] = acc[a];
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -689,6 +709,20 @@ inline zlh_status upload_ib(zlh_ctx *ctx, int level) {
         if (!seen[ai[k]]) { seen[ai[k]] = 1; reset.push_back(ai[k]); }
     }
     T.n_reset = (int64_t)reset.size();
+    // node-major view of the marker table for the spread: the entries naming each supported
+    // node, in ascending entry (marker) order
+    if (ai.size() >= ((size_t)1 << 31)) return ctx->fail(ZLH_ENOMEM, "immersed boundary table too large");
+    std::vector<int32_t> pos(L.naux, -1), begin(reset.size() + 1, 0), entry;
+    for (size_t r = 0; r < reset.size(); ++r) pos[reset[r]] = (int32_t)r;
+    for (int32_t a : ai)
+        if (a >= 0) ++begin[pos[a] + 1];
+    for (size_t r = 0; r < reset.size(); ++r) begin[r + 1] += begin[r];
+    entry.resize(begin.back());
+    std::vector<int32_t> fill(begin.begin(), begin.end() - 1);
+    for (size_t k = 0; k < ai.size(); ++k)
+        if (ai[k] >= 0) entry[fill[pos[ai[k]]]++] = (int32_t)k;
+    if ((s = upload(ctx, T.spread_begin, begin))) return s;
+    if ((s = upload(ctx, T.spread_entry, entry))) return s;
     if ((s = upload(ctx, T.node_ai, ai))) return s;
     if ((s = upload(ctx, T.weight, P.weight))) return s;
     if ((s = upload(ctx, T.reset_ai, reset))) return s;
@@ -750,8 +784,11 @@ zlh_status launch_ib_t(zlh_ctx *ctx, int level) {
                        B.node_u.as<double>(), L.sp);
     hipLaunchKernelGGL((k_ib_markers<D>), dim3(blocks(B.n_markers * 64)), dim3(256), 0, ctx->stream, B.n_markers,
                        B.support, B.node_ai.as<int32_t>(), B.weight.as<double>(), B.node_u.as<double>(),
-                       B.marker_vel.as<double>(), B.marker_scale.as<double>(), B.stiffness,
-                       B.marker_force.as<double>(), L.aux_force_d.as<double>());
+                       B.marker_vel.as<double>(), B.stiffness, B.marker_force.as<double>());
+    hipLaunchKernelGGL((k_ib_spread<D>), dim3(blocks(B.n_reset)), dim3(256), 0, ctx->stream, B.n_reset,
+                       B.reset_ai.as<int32_t>(), B.spread_begin.as<int32_t>(), B.spread_entry.as<int32_t>(), B.support,
+                       B.weight.as<double>(), B.marker_scale.as<double>(), B.marker_force.as<double>(),
+                       L.aux_force_d.as<double>());
     ZLH_TRY(hipGetLastError());
     return ZLH_OK;
 }

@@ -70,6 +70,9 @@ struct IbTables {
     DevBuf node_ai;             // [n_markers][support] aux index, -1 for unused entries
     DevBuf weight;              // [n_markers][support] delta weight (double)
     DevBuf reset_ai;            // [n_reset] aux indices of every supported node
+    DevBuf spread_begin;        // [n_reset + 1] offsets into spread_entry per supported node
+    DevBuf spread_entry;        // marker-table entries (marker * support + lane) that name each
+                                // supported node, ascending per node
     DevBuf node_u;              // [naux][3] velocity recomputed at the reset (double)
     DevBuf marker_vel;          // [n_markers][3]
     DevBuf marker_scale;        // [n_markers] area / (dt dx^(dims-1))
