@@ -301,19 +301,18 @@ __global__ __launch_bounds__(256) void k_ib_markers(int64_t n_markers, int suppo
 template <int D>
 __global__ __launch_bounds__(256) void k_ib_spread(int64_t n_reset, const int32_t *__restrict__ reset_ai,
                                                    const int32_t *__restrict__ begin,
-                                                   const int32_t *__restrict__ entry, int support,
-                                                   const double *__restrict__ weight,
-                                                   const double *__restrict__ marker_scale,
+                                                   const int32_t *__restrict__ marker,
+                                                   const double *__restrict__ coef,
                                                    const double *__restrict__ marker_force,
                                                    double *__restrict__ aux_force) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n_reset) return;
     double acc[3] = {0., 0., 0.};
+#pragma unroll 4
     for (int32_t e = begin[r]; e < begin[r + 1]; ++e) {
-        const int32_t k = entry[e];
-        const int32_t m = k / support;
-        const double s = weight[k] * marker_scale[m];
-        for (int a = 0; a < D; ++a) acc[a] += marker_force[(int64_t)m * 3 + a] * s;
+        const int64_t m = marker[e];
+        const double s = coef[e];
+        for (int a = 0; a < D; ++a) acc[a] += marker_force[m * 3 + a] * s;
     }
     const int64_t ai = reset_ai[r];
     for (int a = 0; a < D; ++a) aux_force[ai * 3 + a] = acc[a];
@@ -708,26 +707,37 @@ inline zlh_status upload_ib(zlh_ctx *ctx, int level) {
         if (ai[k] < 0) return ctx->fail(ZLH_EDEVICE, "immersed boundary node not deferred");
         if (!seen[ai[k]]) { seen[ai[k]] = 1; reset.push_back(ai[k]); }
     }
+    // supported nodes in aux order, so the reset's and the spread's row accesses coalesce
+    std::sort(reset.begin(), reset.end());
     T.n_reset = (int64_t)reset.size();
     // node-major view of the marker table for the spread: the entries naming each supported
     // node, in ascending entry (marker) order
     if (ai.size() >= ((size_t)1 << 31)) return ctx->fail(ZLH_ENOMEM, "immersed boundary table too large");
-    std::vector<int32_t> pos(L.naux, -1), begin(reset.size() + 1, 0), entry;
+    // each term's marker and coefficient weight x area / (dt dx^(dims-1)) (the same double
+    // product the marker-major form forms), in ascending marker order per node
+    std::vector<int32_t> pos(L.naux, -1), begin(reset.size() + 1, 0), marker;
+    std::vector<double> coef;
     for (size_t r = 0; r < reset.size(); ++r) pos[reset[r]] = (int32_t)r;
     for (int32_t a : ai)
         if (a >= 0) ++begin[pos[a] + 1];
     for (size_t r = 0; r < reset.size(); ++r) begin[r + 1] += begin[r];
-    entry.resize(begin.back());
+    marker.resize(begin.back());
+    coef.resize(begin.back());
     std::vector<int32_t> fill(begin.begin(), begin.end() - 1);
-    for (size_t k = 0; k < ai.size(); ++k)
-        if (ai[k] >= 0) entry[fill[pos[ai[k]]]++] = (int32_t)k;
+    for (size_t k = 0; k < ai.size(); ++k) {
+        if (ai[k] < 0) continue;
+        const int32_t e = fill[pos[ai[k]]]++;
+        const size_t m = k / (size_t)P.support;
+        marker[e] = (int32_t)m;
+        coef[e] = P.weight[k] * P.scale[m];
+    }
     if ((s = upload(ctx, T.spread_begin, begin))) return s;
-    if ((s = upload(ctx, T.spread_entry, entry))) return s;
+    if ((s = upload(ctx, T.spread_marker, marker))) return s;
+    if ((s = upload(ctx, T.spread_weight, coef))) return s;
     if ((s = upload(ctx, T.node_ai, ai))) return s;
     if ((s = upload(ctx, T.weight, P.weight))) return s;
     if ((s = upload(ctx, T.reset_ai, reset))) return s;
     if ((s = upload(ctx, T.marker_vel, L.ib.vel))) return s;
-    if ((s = upload(ctx, T.marker_scale, P.scale))) return s;
     if ((s = dalloc(ctx, T.marker_force, sizeof(double) * 3 * T.n_markers))) return s;
     if ((s = dalloc(ctx, T.node_u, sizeof(double) * 3 * std::max<int64_t>(L.naux, 1)))) return s;
     return ZLH_OK;
@@ -786,9 +796,8 @@ zlh_status launch_ib_t(zlh_ctx *ctx, int level) {
                        B.support, B.node_ai.as<int32_t>(), B.weight.as<double>(), B.node_u.as<double>(),
                        B.marker_vel.as<double>(), B.stiffness, B.marker_force.as<double>());
     hipLaunchKernelGGL((k_ib_spread<D>), dim3(blocks(B.n_reset)), dim3(256), 0, ctx->stream, B.n_reset,
-                       B.reset_ai.as<int32_t>(), B.spread_begin.as<int32_t>(), B.spread_entry.as<int32_t>(), B.support,
-                       B.weight.as<double>(), B.marker_scale.as<double>(), B.marker_force.as<double>(),
-                       L.aux_force_d.as<double>());
+                       B.reset_ai.as<int32_t>(), B.spread_begin.as<int32_t>(), B.spread_marker.as<int32_t>(),
+                       B.spread_weight.as<double>(), B.marker_force.as<double>(), L.aux_force_d.as<double>());
     ZLH_TRY(hipGetLastError());
     return ZLH_OK;
 }

@@ -69,13 +69,12 @@ struct IbTables {
     int64_t n_reset = 0;
     DevBuf node_ai;             // [n_markers][support] aux index, -1 for unused entries
     DevBuf weight;              // [n_markers][support] delta weight (double)
-    DevBuf reset_ai;            // [n_reset] aux indices of every supported node
-    DevBuf spread_begin;        // [n_reset + 1] offsets into spread_entry per supported node
-    DevBuf spread_entry;        // marker-table entries (marker * support + lane) that name each
-                                // supported node, ascending per node
+    DevBuf reset_ai;            // [n_reset] aux indices of every supported node, ascending
+    DevBuf spread_begin;        // [n_reset + 1] offsets into the spread arrays per supported node
+    DevBuf spread_marker;       // marker of each spread term, ascending per node
+    DevBuf spread_weight;       // its delta weight times the marker's area / (dt dx^(dims-1))
     DevBuf node_u;              // [naux][3] velocity recomputed at the reset (double)
     DevBuf marker_vel;          // [n_markers][3]
-    DevBuf marker_scale;        // [n_markers] area / (dt dx^(dims-1))
     DevBuf marker_force;        // [n_markers][3] Lagrangian force
     double stiffness = 2.;
 };
