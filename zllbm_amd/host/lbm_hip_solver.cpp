@@ -320,6 +320,7 @@ void SolverLbmHip::WriteCheckPointGridNodes(const std::string &file_name) {
         }
     }
     std::sort(order.begin(), order.end());
+    if (order.size() > 0xffffffffull) throw ZlhError("checkpoint: more nodes than the record count (uint32) holds");
     const int32_t nforce = bool_forces_ ? k0NumForces_ : 0;
     const int32_t payload = (int32_t)(sizeof(DefInt) + sizeof(double) * (k0NumQ_ + nforce));
     std::ofstream out(file_name, std::ios::binary);
