@@ -49,6 +49,7 @@ PROFILED_TRAFFIC = {
     ("amr4", "mrt", "fp32", "fp16"): (584.4e6, "profiles/k_stream_collide_mlp_amr4.md"),
     ("tgv256", "mrt", "fp32", None): (2884.4e6, "profiles/k_stream_collide_tgv256.md"),
     ("tgv256", "mrt", "bf16", None): (1492.6e6, "profiles/k_stream_collide_bf16_tgv256.md"),
+    ("sphere5", "mrt", "fp32", "fp16"): (82.8e6, "profiles/k_ib_sphere5.md"),
 }
 
 
