@@ -70,10 +70,21 @@ def make_solver(hip, W, precision, les, model=None):
     return s
 
 
+def warm(s, warmup):
+    """`warmup` untimed background steps that leave captured the graph the timed steps replay.
+    Replays run two steps each and keep level 0's buffer parity; an odd count therefore starts
+    with one direct step, so that the parity the timed region starts from is the one whose graph
+    the warm-up replayed (a capture would otherwise land in the timed region)."""
+    if warmup % 2:
+        s.time_marching(1)
+        warmup -= 1
+    s.time_marching(warmup)
+
+
 def timed(s, W, warmup, steps, kernel_steps=20):
     """Wall time of `steps` background steps (graph replays) after `warmup`, then the finest
     level's fused-kernel time from per-launch events over `kernel_steps` direct-launch steps."""
-    s.time_marching(warmup)
+    warm(s, warmup)
     s.synchronize()
     t0 = time.perf_counter()
     s.time_marching(steps)
@@ -110,11 +121,12 @@ def l2_curve(hip, W, les, model, steps, every):
     return out
 
 
-def cpu_baseline(model, seconds=15.0):
-    """FP64 oracle on a bounded sample of the same workload: 2 levels over a 24^3 box."""
+def cpu_baseline(model, seconds=15.0, threads=None):
+    """FP64 oracle on a bounded sample of the same workload: 2 levels over a 24^3 box, on
+    `threads` OpenMP threads (default: every host core)."""
     from oracle.zlbm_oracle import Oracle, lib
     from zllbm_amd import workloads
-    threads = os.cpu_count() or 1
+    threads = threads or os.cpu_count() or 1
     lib().zo_set_threads(threads)
     W = workloads.decaying_turbulence(24, 2, buffer=2, les=2 if model is not None else 1)
     o = Oracle(W.grid, **W.solver)
@@ -218,7 +230,9 @@ def main():
                            if model is not None else "Smagorinsky Cs 0.1",
                            "populations": "BF16 shifted" if bf16 else "FP32 shifted"},
                 "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                             "frac": round(achieved / HBM_PEAK_GBS, 4),
+                             # whole step: every update at the kernel's algorithmic bytes, over the step time
+                             "frac_step": round(mlups * 1e6 * bpu / (HBM_PEAK_GBS * 1e9), 4), "traffic": traffic,
                              "traffic_unit": "bytes per launch", "traffic_source": traffic_src, "kernel": kernel,
                              "avg_launch_ms": round(avg_s * 1e3, 4), "bytes_per_update": bpu,
                              "updates_per_launch": W.nodes[finest]}})
@@ -258,6 +272,9 @@ def main():
                                              "initial state"}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(model)
+        one = cpu_baseline(model, seconds=10.0, threads=1)
+        out["cpu_baseline"]["one_thread"] = {"value": one["value"], "unit": one["unit"], "cores": 1,
+                                             "sample": one["sample"]}
     print(json.dumps(out), flush=True)
 
 

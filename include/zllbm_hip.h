@@ -101,6 +101,11 @@ zlh_status zlh_ib_set_markers(zlh_ctx *ctx, int level, const double *xyz, const 
 #define ZLH_PROFILE_UNIFORM 0
 #define ZLH_PROFILE_PARABOLIC 1
 zlh_status zlh_set_boundary_profile(zlh_ctx *ctx, int dir, int profile);
+/* largest number of 64-node bricks one level may hold with these settings (no reference
+ * counterpart; no device call): slot indices are 32-bit, and the FP32 learned closure's fused
+ * kernel addresses its pulls with 32-bit byte offsets (bricks x Q x 256 bytes <= 2^32, about
+ * 883k bricks = 56.5M nodes in D3Q19).  zlh_build refuses a level above it (ZLH_EINVAL). */
+zlh_status zlh_max_level_bricks(const zlh_solver_desc *desc, int64_t *max_bricks);
 /* device slot and neighbour tables, boundary / interface / IB tables
  * (replaces the per-access unordered_map lookups of lbm_d3q19_solver.cpp:20-135) */
 zlh_status zlh_build(zlh_ctx *ctx);

@@ -86,3 +86,45 @@ def test_bench_refuses_bf16_with_learned_closure():
                           capture_output=True, text=True, timeout=300)
     assert proc.returncode == 2
     assert "learned closure" in proc.stderr
+
+
+def test_level_size_limit_without_device():
+    """The FP32 learned closure's fused kernel forms 32-bit byte offsets of its pulls, so a level
+    holds at most 2^32 / (Q 64 4) bricks with it (883,011 in D3Q19); other settings keep the
+    32-bit slot-index bound.  zlh_build refuses larger levels with this limit."""
+    from zllbm_amd import grid as g
+    grid = g.uniform_grid(3, (4, 4, 4))
+    general = (2 ** 31 // 19 - 1) // 64
+    assert hip.max_level_bricks(hip.make_desc(grid, nu0=0.01)) == general
+    assert hip.max_level_bricks(hip.make_desc(grid, nu0=0.01, precision=hip.FP32_SHIFTED)) == general
+    assert hip.max_level_bricks(hip.make_desc(grid, nu0=0.01, precision=hip.FP64, les=hip.LES_MLP)) == general
+    mlp = hip.max_level_bricks(hip.make_desc(grid, nu0=0.01, precision=hip.FP32_SHIFTED, les=hip.LES_MLP))
+    assert mlp == 2 ** 32 // (19 * 64 * 4) == 883011
+    # the largest admitted level's last pull offset stays below 2^32
+    assert (mlp - 1) * 19 * 256 + 19 * 256 - 1 < 2 ** 32
+    grid2 = g.uniform_grid(2, (4, 4))
+    assert hip.max_level_bricks(hip.make_desc(grid2, nu0=0.01, precision=hip.FP32_SHIFTED,
+                                              les=hip.LES_MLP)) == min(general, 2 ** 32 // (9 * 256))
+    bad = hip.make_desc(grid, nu0=0.01)
+    bad.dims = 5
+    with pytest.raises(ValueError):
+        hip.max_level_bricks(bad)
+
+
+def test_wrapper_refuses_short_arrays():
+    """init_macro / set_f check their array lengths before the library reads n * dims or n * Q
+    values through the raw pointer."""
+    import numpy as np
+    from zllbm_amd import grid as g
+    grid = g.uniform_grid(2, (4, 4))
+    s = hip.Solver.__new__(hip.Solver)       # no device: only the wrapper's checks run
+    s.grid, s.dims, s.q, s.h = grid, 2, 9, None
+    n = grid.levels[0].n
+    with pytest.raises(ValueError):
+        s.init_macro(0, np.ones(n - 1), np.zeros((n, 2)))
+    with pytest.raises(ValueError):
+        s.init_macro(0, np.ones(n), np.zeros((n, 3)))
+    with pytest.raises(ValueError):
+        s.set_f(0, np.zeros((n, 8)))
+    with pytest.raises(ValueError):
+        hip._checked(np.zeros(3), np.uint64, 4, "keys")

@@ -189,38 +189,47 @@ __global__ void k_upload(BrickLevel<T> lv, const double *f_in, int64_t n, const 
     for (int q = 0; q < Q; ++q) fa[pop_at<Q>(q, slot)] = T(f[q]);
 }
 
+// post-stream population q of one node (brick slot, aux index or -1), in double and unshifted:
+// deferred nodes hold it in the aux array; a bulk node's is the pull of the post-collision value
+// at x - c_q from the buffer the last fused kernel read (before the first step, the uploaded
+// value itself)
+template <int D, typename T, bool SHIFT>
+__device__ __forceinline__ double post_stream_value(const BrickLevel<T> &lv, int64_t slot, int32_t ai, int q,
+                                                    const T *last, const T *aux_f, int fresh) {
+    using L = Lattice<D>;
+    using B = Brick<D>;
+    constexpr int Q = L::Q;
+    double v;
+    if (ai >= 0) {
+        v = double(aux_f[q * lv.naux + ai]);
+    } else if (fresh || q == 0) {
+        v = double(last[pop_at<Q>(q, slot)]);
+    } else {
+        const int64_t b = slot >> 6;
+        const int lane = (int)(slot & 63);
+        const int lx = B::lane_coord(lane, 0), ly = B::lane_coord(lane, 1), lz = D == 3 ? B::lane_coord(lane, 2) : 0;
+        const int sx = lx - L::cx[q], sy = ly - L::cy[q], sz = lz - L::cz[q];
+        const int ox = sx < 0 ? -1 : (sx >= B::edge ? 1 : 0);
+        const int oy = sy < 0 ? -1 : (sy >= B::edge ? 1 : 0);
+        const int oz = D == 3 ? (sz < 0 ? -1 : (sz >= B::edge ? 1 : 0)) : 0;
+        const int k = D == 3 ? (oz + 1) * 9 + (oy + 1) * 3 + (ox + 1) : (oy + 1) * 3 + (ox + 1);
+        const int64_t sb = lv.nbr[b * B::nnbr + k];
+        const int sl = B::lane_of(sx & (B::edge - 1), sy & (B::edge - 1), sz & (B::edge - 1));
+        v = double(last[pop_at<Q>(q, sb * 64 + sl)]);
+    }
+    return v + (SHIFT ? L::w[q] : 0.);
+}
+
 // post-stream populations of every node, [Q][n] double in key order (unshifted)
 template <int D, typename T, bool SHIFT>
 __global__ void k_materialize(BrickLevel<T> lv, const int32_t *node_slot, const int32_t *node_aux, int64_t n,
                               const T *last, const T *aux_f, int fresh, double *out) {
-    using L = Lattice<D>;
-    using B = Brick<D>;
-    constexpr int Q = L::Q;
+    constexpr int Q = Lattice<D>::Q;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int32_t ai = node_aux[i];
     const int64_t slot = node_slot[i];
-    const int64_t b = slot >> 6;
-    const int lane = (int)(slot & 63);
-    const int lx = B::lane_coord(lane, 0), ly = B::lane_coord(lane, 1), lz = D == 3 ? B::lane_coord(lane, 2) : 0;
-    for (int q = 0; q < Q; ++q) {
-        double v;
-        if (ai >= 0) {
-            v = double(aux_f[q * lv.naux + ai]);
-        } else if (fresh || q == 0) {
-            v = double(last[pop_at<Q>(q, slot)]);
-        } else {
-            const int sx = lx - L::cx[q], sy = ly - L::cy[q], sz = lz - L::cz[q];
-            const int ox = sx < 0 ? -1 : (sx >= B::edge ? 1 : 0);
-            const int oy = sy < 0 ? -1 : (sy >= B::edge ? 1 : 0);
-            const int oz = D == 3 ? (sz < 0 ? -1 : (sz >= B::edge ? 1 : 0)) : 0;
-            const int k = D == 3 ? (oz + 1) * 9 + (oy + 1) * 3 + (ox + 1) : (oy + 1) * 3 + (ox + 1);
-            const int64_t sb = lv.nbr[b * B::nnbr + k];
-            const int sl = B::lane_of(sx & (B::edge - 1), sy & (B::edge - 1), sz & (B::edge - 1));
-            v = double(last[pop_at<Q>(q, sb * 64 + sl)]);
-        }
-        out[q * n + i] = v + (SHIFT ? L::w[q] : 0.);
-    }
+    for (int q = 0; q < Q; ++q) out[q * n + i] = post_stream_value<D, T, SHIFT>(lv, slot, ai, q, last, aux_f, fresh);
 }
 
 __global__ void k_sum4(const double *fq, int64_t n, int Q, const int *cx, const int *cy, const int *cz,
@@ -765,8 +774,11 @@ zlh_status build_level_slots(zlh_ctx *ctx, int level) {
         if (L.bkeys.empty() || L.bkeys.back() != (k >> 6)) L.bkeys.push_back(k >> 6);
     L.nbrick = (int64_t)L.bkeys.size();
     const int64_t nslot = L.nbrick * 64;
-    if (nslot >= ((int64_t)1 << 31) / 19)
-        return ctx->fail(ZLH_EINVAL, "level %d too large (%lld slots)", level, (long long)nslot);
+    int64_t max_bricks = 0;
+    (void)zlh_max_level_bricks(&ctx->d, &max_bricks);
+    if (L.nbrick > max_bricks)
+        return ctx->fail(ZLH_EINVAL, "level %d too large: %lld bricks of 64 nodes, at most %lld with these settings",
+                         level, (long long)L.nbrick, (long long)max_bricks);
     zlh_status s;
     if ((s = upload(ctx, L.keys_d, L.keys))) return s;
     if ((s = upload(ctx, L.bkeys_d, L.bkeys))) return s;
@@ -981,6 +993,20 @@ void zlh_destroy(zlh_ctx *ctx) {
 
 const char *zlh_last_error(const zlh_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+zlh_status zlh_max_level_bricks(const zlh_solver_desc *desc, int64_t *max_bricks) {
+    if (!desc || !max_bricks || (desc->dims != 2 && desc->dims != 3)) return ZLH_EINVAL;
+    // slot and aux indices are 32-bit: 64 nbrick 19 < 2^31
+    int64_t m = (((int64_t)1 << 31) / 19 - 1) / 64;
+    // the FP32 learned closure's fused kernel pulls through 32-bit byte offsets
+    // nbrick Q 64 4 (pull_to_lds in kernels.hpp): nbrick Q 256 <= 2^32
+    if (desc->les == ZLH_LES_MLP && desc->precision == ZLH_FP32_SHIFTED) {
+        const int64_t q = desc->dims == 2 ? 9 : 19;
+        m = std::min<int64_t>(m, ((int64_t)1 << 32) / (q * 64 * 4));
+    }
+    *max_bricks = m;
+    return ZLH_OK;
+}
+
 zlh_status zlh_set_level_nodes(zlh_ctx *ctx, int level, const uint64_t *keys, const int32_t *flags, int64_t n) {
     if (!ctx) return ZLH_EINVAL;
     if (level < 0 || level > ctx->d.max_level) return ctx->fail(ZLH_EINVAL, "level %d out of range", level);
@@ -1137,8 +1163,8 @@ zlh_status march_launches(zlh_ctx *ctx, int nsteps) {
     // InformationFromGridOfDifferentLevel (lbm_solver_interface.cpp:224-238): refined levels
     // exchange with their parent when their substep count is even.  With level streams, level l
     // runs on lstream[l]; the transfer into level l (on lstream[l]) waits for the latest step of
-    // level l - 1 (and for the latest transfer into level l + 1 if the two share nodes of level
-    // l), and level l - 1's next step waits for it.  Nothing else is shared between levels, so a
+    // level l - 1 (and, where the host found that they share nodes, for the latest transfers into
+    // levels l + 1 and l - 1), and level l - 1's next step waits for it.  Nothing else is shared between levels, so a
     // coarse level's step overlaps the finer levels' substeps.  Timed runs keep one stream, so
     // that per-launch events time a kernel alone.
     const bool split = ctx->level_streams && !ctx->timing && nl > 1;
@@ -1148,6 +1174,7 @@ zlh_status march_launches(zlh_ctx *ctx, int nsteps) {
         for (int l = 0; l < nl; ++l) ZLH_TRY(hipStreamWaitEvent(ctx->lstream[l], ctx->ev_fork, 0));
     }
     std::vector<uint8_t> after_xfer(nl, 0);   // level l's next step must wait for ev_xfer[l + 1]
+    std::vector<uint8_t> xfer_recorded(nl, 0);   // ev_xfer[l] recorded in this call
     zlh_status s = ZLH_OK;
     hipError_t e = hipSuccess;
     for (int t = 0; t < nsteps && !s && e == hipSuccess; ++t) {
@@ -1163,15 +1190,22 @@ zlh_status march_launches(zlh_ctx *ctx, int nsteps) {
             if (split && (e = hipEventRecord(ctx->ev_step[l], ctx->stream)) != hipSuccess) break;
             if (l > 0 && tsl[l] % 2 == 0) {
                 if (split && (e = hipStreamWaitEvent(ctx->stream, ctx->ev_step[l - 1], 0)) != hipSuccess) break;
-                // and after the latest transfer into level l + 1, as in the serial order, when the
-                // two touch common nodes of level l (interfaces that come within a stencil)
+                // where the transfers into l and l + 1 touch common nodes of level l (interfaces
+                // that come within a stencil), both keep the serial order: the transfer into l
+                // after the latest transfer into l + 1, and the transfer into l after the latest
+                // transfer into l - 1 (which shares nodes of level l - 1 with it)
                 if (split && l < nl - 1 && ctx->lv[l].xfer_overlap &&
                     (e = hipStreamWaitEvent(ctx->stream, ctx->ev_xfer[l + 1], 0)) != hipSuccess)
+                    break;
+                // (a transfer into l - 1 of an earlier call is ordered by the fork and join)
+                if (split && l >= 2 && ctx->lv[l - 1].xfer_overlap && xfer_recorded[l - 1] &&
+                    (e = hipStreamWaitEvent(ctx->stream, ctx->ev_xfer[l - 1], 0)) != hipSuccess)
                     break;
                 if ((s = zlh_transfer_levels(ctx, l))) break;
                 if (split) {
                     if ((e = hipEventRecord(ctx->ev_xfer[l], ctx->stream)) != hipSuccess) break;
                     after_xfer[l - 1] = 1;
+                    xfer_recorded[l] = 1;
                 }
             }
         }

@@ -45,6 +45,7 @@ EXPORTED = (
     "zlh_download_source_table", "zlh_ib_download_forces", "zlh_conservation", "zlh_level_info", "zlh_transfer_info", "zlh_level_streams_enable", "zlh_set_boundary_profile",
     "zlh_synchronize", "zlh_timing_enable", "zlh_timing_read", "zlh_mlp_set", "zlh_mlp_gradient",
     "zlh_mlp_train", "zlh_mlp_training_form", "zlh_mlp_eval", "zlh_closure_features", "zlh_graphs_enable",
+    "zlh_max_level_bricks",
 )
 
 
@@ -93,6 +94,7 @@ def lib():
         L.zlh_graphs_enable.argtypes = [P, C.c_int]
         L.zlh_level_streams_enable.argtypes = [P, C.c_int]
         L.zlh_set_boundary_profile.argtypes = [P, C.c_int, C.c_int]
+        L.zlh_max_level_bricks.argtypes = [C.POINTER(SolverDesc), C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -126,6 +128,23 @@ def make_desc(grid: "_grid.Grid", *, nu0: float, precision: int = FP64, rho0: fl
     return d
 
 
+def max_level_bricks(desc: SolverDesc) -> int:
+    """Largest brick count (64 nodes each) one level may hold with these settings; no device call."""
+    out = C.c_int64()
+    rc = lib().zlh_max_level_bricks(C.byref(desc), C.byref(out))
+    if rc != 0:
+        raise ValueError("zlh_max_level_bricks: status %d" % rc)
+    return out.value
+
+
+def _checked(a, dtype, size, what):
+    """Contiguous copy of `a` as `dtype`, raising ValueError unless it holds `size` elements."""
+    a = np.ascontiguousarray(a, dtype).ravel()
+    if a.size != size:
+        raise ValueError("%s: expected %d values, got %d" % (what, size, a.size))
+    return a
+
+
 class Solver:
     """One device solver over a :class:`zllbm_amd.grid.Grid` (the SolverLbmInterface +
     GridInfoLbmInteface pair of the reference, lbm_interface.h:140-474)."""
@@ -145,8 +164,8 @@ class Solver:
         for face, prof in (bc_profile or {}).items():
             self._chk(L.zlh_set_boundary_profile(self.h, face, prof))
         for l, lev in enumerate(grid.levels):
-            self._chk(L.zlh_set_level_nodes(self.h, l, np.ascontiguousarray(lev.keys, np.uint64),
-                                            np.ascontiguousarray(lev.flags, np.int32), lev.n))
+            self._chk(L.zlh_set_level_nodes(self.h, l, _checked(lev.keys, np.uint64, lev.n, "level %d keys" % l),
+                                            _checked(lev.flags, np.int32, lev.n, "level %d flags" % l), lev.n))
         for l, lev in enumerate(grid.levels):
             for (kind, layer), keys in lev.layers.items():
                 self._chk(L.zlh_set_interface_layer(self.h, l, kind, layer,
@@ -187,11 +206,12 @@ class Solver:
         return self
 
     def init_macro(self, level, rho, u):
-        self._chk(lib().zlh_init_macros(self.h, level, np.ascontiguousarray(rho, np.float64),
-                                        np.ascontiguousarray(u, np.float64).ravel()))
+        n = self.n(level)
+        self._chk(lib().zlh_init_macros(self.h, level, _checked(rho, np.float64, n, "rho"),
+                                        _checked(u, np.float64, n * self.dims, "u")))
 
     def set_f(self, level, f):
-        self._chk(lib().zlh_upload_populations(self.h, level, np.ascontiguousarray(f, np.float64).ravel()))
+        self._chk(lib().zlh_upload_populations(self.h, level, _checked(f, np.float64, self.n(level) * self.q, "f")))
 
     def step_level(self, level, time_step_level):
         self._chk(lib().zlh_step_level(self.h, level, time_step_level))

@@ -152,7 +152,7 @@ def cylinder_amr(cells=(512, 256), diameter_fine=32, u_in=0.05, re=100.0):
 
 
 def sphere_amr(cells=(32, 16, 16), levels=5, diameter_fine=48, u_in=0.05, re=300.0, perturb=0.01,
-               band=8.0, margin=3, les=2, seed=20261202):
+               band=8.0, margin=3, les=2, seed=20261202, buffer=2):
     """Level k is refined where a level-k cell centre lies within r + band level-k cells of the
     sphere centre, at least ``margin`` level-k cells inside level k's region (refined_mask_grid;
     the 16-cell cross-section leaves room for 5 nested levels around the sphere with margin 3,
@@ -164,7 +164,7 @@ def sphere_amr(cells=(32, 16, 16), levels=5, diameter_fine=48, u_in=0.05, re=300
     def refine(k, c):
         return np.linalg.norm(c - ctr, axis=1) < r + band / (1 << k)
 
-    grid = g.refined_mask_grid(3, cells, levels, refine, margin=margin)
+    grid = g.refined_mask_grid(3, cells, levels, refine, margin=margin, buffer=buffer)
     nu = u_in * d_bg / re
     xyz, area = _geometry().fibonacci_sphere_markers(tuple(ctr + 1.0 + 0.013), r, 1.0 / (1 << (levels - 1)))
     rng = np.random.default_rng(seed)
