@@ -101,6 +101,19 @@ zlh_status zlh_ib_set_markers(zlh_ctx *ctx, int level, const double *xyz, const 
 #define ZLH_PROFILE_UNIFORM 0
 #define ZLH_PROFILE_PARABOLIC 1
 zlh_status zlh_set_boundary_profile(zlh_ctx *ctx, int dir, int profile);
+/* per-node velocities of a non-equilibrium-extrapolation face on one level, before zlh_build (no
+ * reference counterpart; the reference's BoundaryNonEqExtrapolation::SetValues takes one velocity
+ * per face, lbm_noneq_extrapolation.cpp:14-20): u is [n][3] for the level's nodes on that face in
+ * key order, n their count (zlh_build checks it); replaces the face velocity and profile there.
+ * A seeded perturbed inflow, for example. */
+zlh_status zlh_set_boundary_velocities(zlh_ctx *ctx, int level, int dir, const double *u, int64_t n);
+/* interpolation of the coarse-to-fine ghost nodes of fine level `level` (>= 1), before zlh_build
+ * (GridInfoInterface::interp_method_, grid_info_interface.h:317, and ChooseInterpolationMethod,
+ * grid/grid_interpolation.cpp:20-82): ZLH_INTERP_LAGRANGIAN (the default, 4-point Lagrange
+ * weights, InterpolationLagrangian{2D,3D}) or ZLH_INTERP_LINEAR (InterpolationLinear{2D,3D}) */
+#define ZLH_INTERP_LAGRANGIAN 0
+#define ZLH_INTERP_LINEAR 1
+zlh_status zlh_set_interpolation_method(zlh_ctx *ctx, int level, int method);
 /* largest number of 64-node bricks one level may hold with these settings (no reference
  * counterpart; no device call): slot indices are 32-bit, and the FP32 learned closure's fused
  * kernel addresses its pulls with 32-bit byte offsets (bricks x Q x 256 bytes <= 2^32, about

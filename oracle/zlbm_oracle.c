@@ -45,6 +45,8 @@ typedef struct zo_level {
     double *force;       /* [n][3] node (immersed boundary) force */
     int64_t nb[7];
     int32_t *bnd[7];     /* boundary node indices per direction */
+    double *bnd_u[7];    /* per-node velocities [nb][3] of a face (zo_set_bc_velocities), or NULL */
+    int64_t nbu[7];
     zo_layers layers;
     double dt, tau0;
     double tau_stream_c2f, tau_stream_f2c;
@@ -67,6 +69,7 @@ struct zo_sim {
     int mlp_set;
     int64_t mlp_fallbacks;
     int bc_profile[7];   /* per face: 0 uniform boundary velocity, 1 parabolic (zo_set_bc_profile) */
+    int interp_method[ZO_MAXL];   /* per fine level: 0 Lagrangian (the reference default), 1 linear */
 };
 
 static void zo_seterr(zo_sim *s, const char *fmt, ...) {
@@ -260,7 +263,7 @@ zo_sim *zo_create(const zo_params *p) {
 static void zo_free_level(zo_level *L) {
     free(L->key); free(L->flag); free(L->src); free(L->f); free(L->fc);
     free(L->rho); free(L->u); free(L->force);
-    for (int d = 0; d < 7; ++d) free(L->bnd[d]);
+    for (int d = 0; d < 7; ++d) { free(L->bnd[d]); free(L->bnd_u[d]); }
     for (int k = 0; k < 4; ++k) for (int l = 0; l < 3; ++l) free(L->layers.idx[k][l]);
     free(L->ib.xyz); free(L->ib.area); free(L->ib.vel); free(L->ib.force);
     memset(L, 0, sizeof(*L));
@@ -374,6 +377,12 @@ int zo_finalize(zo_sim *s) {
         zo_level_constants(s, l);
         zo_build_sources(s, &s->lv[l]);
         zo_build_boundaries(s, l);
+        for (int d = 1; d <= 2 * s->p.dims; ++d)
+            if (s->lv[l].bnd_u[d] && s->lv[l].nbu[d] != s->lv[l].nb[d]) {
+                zo_seterr(s, "level %d face %d: %lld boundary velocities for %lld boundary nodes", l, d,
+                          (long long)s->lv[l].nbu[d], (long long)s->lv[l].nb[d]);
+                return -1;
+            }
     }
     for (int d = 1; d <= 2 * s->p.dims; ++d)
         if (s->p.bc_scheme[d] == ZO_BC_NONE) {
@@ -727,6 +736,22 @@ int zo_set_bc_profile(zo_sim *s, int dir, int profile) {
     return 0;
 }
 
+/* per-node face velocities (no reference counterpart: the reference's faces take one velocity,
+ * lbm_noneq_extrapolation.cpp:14-20), [n][3] in key order of the level's nodes on the face */
+int zo_set_bc_velocities(zo_sim *s, int level, int dir, const double *u, int64_t n) {
+    if (level < 0 || level > s->p.max_level || dir < 1 || dir > 2 * s->p.dims || n < 0 || (n > 0 && !u) ||
+        s->p.bc_scheme[dir] != ZO_BC_NEE) {
+        zo_seterr(s, "bad boundary velocities (level %d, face %d)", level, dir);
+        return -1;
+    }
+    zo_level *L = &s->lv[level];
+    free(L->bnd_u[dir]);
+    L->bnd_u[dir] = (double *)malloc(sizeof(double) * 3 * (n ? n : 1));
+    if (n) memcpy(L->bnd_u[dir], u, sizeof(double) * 3 * n);
+    L->nbu[dir] = n;
+    return 0;
+}
+
 static void zo_bc_nee(zo_sim *s, int level, int dir) {
     zo_level *L = &s->lv[level];
     int q = s->q, dims = s->p.dims;
@@ -742,6 +767,8 @@ static void zo_bc_nee(zo_sim *s, int level, int dir) {
         if (j < 0) continue;
         double fac = s->bc_profile[dir] ? zo_profile_factor(s, level, axis, L->key[i]) : 1.0;
         double ub[3] = {bv[0] * fac, bv[1] * fac, bv[2] * fac};
+        if (L->bnd_u[dir])
+            for (int a = 0; a < 3; ++a) ub[a] = L->bnd_u[dir][b * 3 + a];
         if (s->p.has_force) {
             double F[3];
             zo_all_forces(s, L->force + j * 3, F);
@@ -846,12 +873,28 @@ static void zo_node_c2f(const zo_sim *s, const zo_level *C, int64_t j, double *r
         f[iq] = feq[iq] + C->tau_stream_c2f * (C->f[j * s->q + iq] - feq[iq]);
 }
 
-/* InterpolationLagrangian{2D,3D} (grid/grid_interpolation.cpp:224-304, 433-525) for one fine
- * ghost node; returns 0 or -1 when a contributor is missing in both levels */
+/* interpolation method of a fine level's ghost nodes (GridInfoInterface::interp_method_,
+ * grid_info_interface.h:317; ChooseInterpolationMethod, grid/grid_interpolation.cpp:20-82) */
+int zo_set_interp_method(zo_sim *s, int level, int method) {
+    if (level < 1 || level > s->p.max_level || (method != 0 && method != 1)) {
+        zo_seterr(s, "bad interpolation method %d at level %d", method, level);
+        return -1;
+    }
+    s->interp_method[level] = method;
+    return 0;
+}
+
+/* InterpolationLagrangian{2D,3D} (grid/grid_interpolation.cpp:224-304, 433-525) and
+ * InterpolationLinear{2D,3D} (:126-208, 319-417) for one fine ghost node; returns 0 or -1 when a
+ * contributor is missing in both levels.  Linear is the Lagrangian form with max_interp_length_
+ * 1 and the half-length-1 weights ((1, 0) at even, (1/2, 1/2) at odd fine coordinates) whatever
+ * the region's valid length. */
 static int zo_interp_node(zo_sim *s, int fine_level, int64_t i) {
     zo_level *F = &s->lv[fine_level], *C = &s->lv[fine_level - 1];
     int dims = s->p.dims, q = s->q, clevel = fine_level - 1;
-    const int Lmax = 2;   /* max_interp_length_ (grid_info_interface.h:317-318) */
+    const int linear = s->interp_method[fine_level] == 1;
+    /* max_interp_length_ (grid_info_interface.h:317-318; 1 for kLinear, grid_interpolation.cpp:23) */
+    const int Lmax = linear ? 1 : 2;
     uint64_t parent = F->key[i] >> dims;
     uint32_t pxyz[3];
     zo_decode(dims, parent, pxyz);
@@ -867,9 +910,10 @@ static int zo_interp_node(zo_sim *s, int fine_level, int64_t i) {
     F->rho[i] = 0.;
     F->u[i * 3] = F->u[i * 3 + 1] = F->u[i * 3 + 2] = 0.;
     for (int iq = 0; iq < q; ++iq) F->f[i * q + iq] = 0.;
-    int num = 2 * valid;
+    const int half = linear ? 1 : valid;
+    int num = 2 * half;
     double c0[4], c1[4], cf[3][4];
-    zo_lagrange_coeff(valid, c0, c1);
+    zo_lagrange_coeff(half, c0, c1);
     uint32_t fxyz[3];
     zo_decode(dims, F->key[i], fxyz);
     for (int a = 0; a < dims; ++a)
@@ -880,7 +924,8 @@ static int zo_interp_node(zo_sim *s, int fine_level, int64_t i) {
             for (int ix = 0; ix < num; ++ix) {
                 double coeff = dims == 3 ? cf[0][ix] * cf[1][iy] * cf[2][iz] : cf[0][ix] * cf[1][iy];
                 if (!(fabs(coeff) > ZO_KEPS)) continue;
-                int off = Lmax - valid;
+                int off = Lmax - half;
+                if (seq[0][ix + off] < 0 || seq[1][iy + off] < 0 || (dims == 3 && seq[2][iz + off] < 0)) return -1;
                 uint32_t c[3] = {(uint32_t)seq[0][ix + off], (uint32_t)seq[1][iy + off],
                                  dims == 3 ? (uint32_t)seq[2][iz + off] : 0};
                 uint64_t ck = zo_encode(dims, c[0], c[1], c[2]);

@@ -92,6 +92,11 @@ int zo_time_stepping_order(int max_level, int32_t *order, int cap);
 /* velocity profile of a non-equilibrium-extrapolation face: 0 uniform (the reference), 1 parabolic
  * across the transverse axes with the face velocity on the centre line */
 int zo_set_bc_profile(zo_sim *s, int dir, int profile);
+/* interpolation of the ghost nodes of fine level `level` (>= 1): 0 Lagrangian (default), 1 linear */
+int zo_set_interp_method(zo_sim *s, int level, int method);
+/* per-node velocities [n][3] of a non-equilibrium-extrapolation face on one level, in key order of
+ * the level's nodes on that face (no reference counterpart); checked against the count at finalize */
+int zo_set_bc_velocities(zo_sim *s, int level, int dir, const double *u, int64_t n);
 /* immersed boundary markers for one level (immersed_boundary.cpp:35-384) */
 int zo_set_ib(zo_sim *s, int level, int64_t n, const double *xyz, const double *area,
               const double *vel, int stencil_dis, double stiffness, double dx_background);

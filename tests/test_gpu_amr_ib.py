@@ -42,6 +42,21 @@ def test_amr3d_mrt_les_fp64():
         assert_close(s.f(l), o.f(l))
 
 
+@pytest.mark.parametrize("dims", [2, 3])
+def test_amr_linear_interpolation_fp64(dims):
+    """Linear coarse-to-fine interpolation (InterpolationLinear{2D,3D}) on every refined level:
+    populations of every level against the oracle after 10 background steps."""
+    grid, kw = sc.amr2d(levels=3) if dims == 2 else sc.amr3d()
+    kw.update(collision=(hip.MRT,), interp_method=hip.INTERP_LINEAR)
+    o, s = _pair(grid, kw)
+    assert s.transfer_info(1)["contributors"] < _pair(grid, dict(kw, interp_method=hip.INTERP_LAGRANGIAN))[1] \
+        .transfer_info(1)["contributors"]
+    o.time_marching(10)
+    s.time_marching(10)
+    for l in range(grid.max_level + 1):
+        assert_close(s.f(l), o.f(l))
+
+
 def test_amr_substep_by_substep_fp64():
     """Level substeps driven one at a time through the C ABI, including odd substeps."""
     grid, kw = sc.amr2d(levels=2)

@@ -398,6 +398,30 @@ def test_time_stepping_order():
         assert list(zo.time_stepping_order(ml)) == _reference_order_loop(ml)
 
 
+def test_inflow_node_velocities():
+    """Per-node face velocities (zo_set_bc_velocities): a channel at rest, after one step every
+    interior inlet node carries exactly the velocity given for it (feq(rho0, u_b) + f_in -
+    feq(f_in) with the inner node at rest), in key order of the face's nodes."""
+    grid, kw = sc.channel_nee(nx=24, ny=16, u_in=0.06)
+    xy = grid.coords(0).astype(np.int64)
+    lo, hi = grid.dom_min, grid.dom_max
+    face = xy[:, 0] == lo[0]
+    ub = np.zeros((int(face.sum()), 3))
+    rng = np.random.default_rng(7)
+    ub[:, 0] = 0.05 * (1 + 0.01 * rng.standard_normal(len(ub)))
+    ub[:, 1] = 1e-3 * rng.standard_normal(len(ub))
+    o = Oracle(grid, collision=(0,), bc_velocities={(0, g.XMIN): ub}, **kw)
+    o.advance_level(0, 1)
+    f = o.f(0)
+    cx, cy, _, _, _ = zo.lattice(2)
+    rho = f.sum(1)
+    u = np.stack([f @ np.asarray(cx, np.float64), f @ np.asarray(cy, np.float64)], 1) / rho[:, None]
+    inner = (xy[face, 1] > lo[1]) & (xy[face, 1] < hi[1])    # wall corners take bounce-back after
+    np.testing.assert_allclose(u[face][inner], ub[inner, :2], rtol=1e-12, atol=1e-15)
+    with pytest.raises(RuntimeError, match="boundary velocities"):
+        Oracle(grid, collision=(0,), bc_velocities={(0, g.XMIN): ub[:-1]}, **kw)
+
+
 def test_parabolic_inflow_profile():
     """A channel at rest: after one step the inlet nodes carry feq(rho0, u_b) + f_in - feq(f_in) with
     the inner node still at rest, so their velocity is the profile itself, u_in 4 t (1 - t) across

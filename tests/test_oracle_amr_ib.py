@@ -35,6 +35,44 @@ def test_c2f_reproduces_cubic_density():
         np.testing.assert_allclose(f1[idx], np.outer(_cubic(xyz1[idx]), w), rtol=1e-13)
 
 
+def _ghosts_at_rest(field, interp_method, dims=2):
+    """Interpolated ghost populations and w rho(x) at the ghost positions, for equilibrium at rest
+    with density field(x) on both levels (feq = w rho, unchanged by the tau rescaling)."""
+    grid, kw = sc.amr2d(levels=2) if dims == 2 else sc.amr3d()
+    o = Oracle(grid, interp_method=interp_method, **kw)
+    for l in range(2):
+        xyz = grid.coords(l).astype(float) / (1 << l)
+        o.init_macro(l, field(xyz), np.zeros((len(xyz), dims)))
+    o.debug_transfer(1)
+    w = zo.lattice(dims)[3]
+    f1 = o.f(1)
+    xyz1 = grid.coords(1).astype(float) / 2
+    lev = grid.levels[1]
+    idx = np.concatenate([np.searchsorted(lev.keys, lev.layers[(g.OUTER_F2C, layer)]) for layer in (1, 2)])
+    return f1[idx], np.outer(field(xyz1[idx]), w), xyz1[idx]
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_c2f_linear_reproduces_linear_density(dims):
+    """InterpolationLinear{2D,3D}: weights (1, 0) at even and (1/2, 1/2) at odd fine coordinates per
+    axis reproduce a linear density exactly at every ghost node, and a quadratic one with the
+    midpoint error -h^2/4 of each odd axis (h = 1 background cell), which the 4-point Lagrange
+    weights do not make."""
+    lin = lambda x: 1.0 + 0.01 * x[:, 0] - 0.003 * x[:, 1] + (0.002 * x[:, 2] if dims == 3 else 0.0)
+    got, want, _ = _ghosts_at_rest(lin, 1, dims)
+    np.testing.assert_allclose(got, want, rtol=1e-14)
+    quad = lambda x: 1.0 + 1e-4 * x[:, 0] ** 2
+    got, want, xyz = _ghosts_at_rest(quad, 1, dims)
+    odd_x = (np.round(xyz[:, 0] * 2).astype(int) % 2) == 1
+    w = zo.lattice(dims)[3]
+    # the midpoint of x0 and x0 + 1 gets (x0^2 + (x0 + 1)^2) / 2 = x^2 + 1/4
+    np.testing.assert_allclose(got[odd_x], want[odd_x] + np.outer(np.full(odd_x.sum(), 1e-4 * 0.25), w), rtol=1e-13)
+    np.testing.assert_allclose(got[~odd_x], want[~odd_x], rtol=1e-14)
+    assert odd_x.any() and (~odd_x).any()
+    got_l, want_l, _ = _ghosts_at_rest(quad, 0, dims)
+    np.testing.assert_allclose(got_l, want_l, rtol=1e-13)      # Lagrange weights: exact on quadratics
+
+
 def test_f2c_rescale_closed_form():
     """Coarse layer-1 nodes receive feq(rho_f, u_f) + tau_f2c (f_f - feq) of the coincident fine
     node with tau_f2c = 2 tau_c / tau_f (CalRelaxationTimeRatio)."""

@@ -381,7 +381,11 @@ inline zlh_status plan_transfer(zlh_ctx *ctx, int level) {
     P = TransferPlan();
     const int dims = ctx->d.dims;
     const int lc = level - 1;
-    const int Lmax = 2;   // max_interp_length_ (grid/grid_info_interface.h:317-318)
+    // InterpolationLagrangian{2D,3D} (grid/grid_interpolation.cpp:224-304, 433-525), or
+    // InterpolationLinear{2D,3D} (:126-208, 319-417): the Lagrangian form with max_interp_length_ 1
+    // (:23) and half-length-1 weights whatever the region's valid length
+    const bool linear = ctx->interp_method[level] == ZLH_INTERP_LINEAR;
+    const int Lmax = linear ? 1 : 2;   // max_interp_length_ (grid/grid_info_interface.h:317-318)
     bool per_lo[3], per_hi[3];
     int64_t lo[3], hi[3];
     for (int a = 0; a < dims; ++a) {
@@ -407,8 +411,9 @@ inline zlh_status plan_transfer(zlh_ctx *ctx, int level) {
                     valid = std::min(valid, region_axis(axis_coord(dims, parent, a), Lmax, lo[a], hi[a], per_lo[a],
                                                         per_hi[a], seq[a]));
                 double c0[4], c1[4], cf[3][4];
-                lagrange_coeff(valid, c0, c1);
-                const int num = 2 * valid, off = Lmax - valid;
+                const int half = linear ? 1 : valid;
+                lagrange_coeff(half, c0, c1);
+                const int num = 2 * half, off = Lmax - half;
                 for (int a = 0; a < dims; ++a)
                     for (int k = 0; k < num; ++k) cf[a][k] = (axis_coord(dims, key, a) & 1u) ? c1[k] : c0[k];
                 const int nz = dims == 3 ? num : 1;
@@ -419,6 +424,7 @@ inline zlh_status plan_transfer(zlh_ctx *ctx, int level) {
                             const double coeff = dims == 3 ? cf[0][ix] * cf[1][iy] * cf[2][iz] : cf[0][ix] * cf[1][iy];
                             if (!(std::fabs(coeff) > 1e-6)) continue;
                             int64_t c[3] = {seq[0][ix + off], seq[1][iy + off], dims == 3 ? seq[2][iz + off] : 0};
+                            if (c[0] < 0 || c[1] < 0 || c[2] < 0) { broken = true; ++P.incomplete; break; }
                             const uint64_t ck = encode_xyz(dims, c);
                             const int64_t j = find_key(C.keys, ck);
                             if (j >= 0 && !(C.flags[j] & (1 << 2))) {   // kNodeStatusCoarse2FineGhost_

@@ -399,6 +399,8 @@ struct zlh_ctx {
     // time marching runs each level on a stream of its own; events order a transfer after the
     // coarse step it pairs with, and the coarse level's next step after the transfer
     int bc_profile[7] = {};   // per face: ZLH_PROFILE_*
+    std::map<std::pair<int, int>, std::vector<double>> bc_node_u;   // (level, face) -> [n][3] per-node velocities
+    int interp_method[16] = {};   // per fine level: ZLH_INTERP_LAGRANGIAN or ZLH_INTERP_LINEAR
     bool level_streams = true;
     std::vector<hipStream_t> lstream;
     std::vector<hipEvent_t> ev_step, ev_xfer, ev_join;
@@ -887,7 +889,13 @@ zlh_status build_level_aux(zlh_ctx *ctx, int level) {
             slot.push_back(L.node_slot_h[L.bnode[dir][k]]);
         }
         L.bc_n[dir] = (int64_t)ai.size();
-        if (ctx->bc_profile[dir] == ZLH_PROFILE_PARABOLIC && ctx->d.bc_scheme[dir] == ZLH_BC_NEE) {
+        const auto given = ctx->bc_node_u.find({level, dir});
+        if (given != ctx->bc_node_u.end()) {
+            if ((int64_t)given->second.size() != 3 * L.bc_n[dir])
+                return ctx->fail(ZLH_EINVAL, "level %d face %d: %lld boundary velocities for %lld boundary nodes", level,
+                                 dir, (long long)(given->second.size() / 3), (long long)L.bc_n[dir]);
+            if ((s = upload(ctx, L.bc_u_d[dir], given->second))) return s;
+        } else if (ctx->bc_profile[dir] == ZLH_PROFILE_PARABOLIC && ctx->d.bc_scheme[dir] == ZLH_BC_NEE) {
             // the face velocity times prod over the transverse axes of 4 t (1 - t)
             const int axis = (dir - 1) / 2;
             std::vector<double> ub;
@@ -1290,6 +1298,28 @@ zlh_status zlh_set_boundary_profile(zlh_ctx *ctx, int dir, int profile) {
     if (profile == ZLH_PROFILE_PARABOLIC && ctx->d.bc_scheme[dir] != ZLH_BC_NEE)
         return ctx->fail(ZLH_EINVAL, "face %d: profiles apply to non-equilibrium extrapolation faces", dir);
     ctx->bc_profile[dir] = profile;
+    return ZLH_OK;
+}
+
+zlh_status zlh_set_boundary_velocities(zlh_ctx *ctx, int level, int dir, const double *u, int64_t n) {
+    if (!ctx) return ZLH_EINVAL;
+    if (ctx->built) return ctx->fail(ZLH_ESTATE, "boundary velocities after build");
+    if (level < 0 || level > ctx->d.max_level) return ctx->fail(ZLH_EINVAL, "bad level %d", level);
+    if (dir < 1 || dir > 2 * ctx->d.dims) return ctx->fail(ZLH_EINVAL, "bad face %d", dir);
+    if (ctx->d.bc_scheme[dir] != ZLH_BC_NEE)
+        return ctx->fail(ZLH_EINVAL, "face %d: node velocities apply to non-equilibrium extrapolation faces", dir);
+    if (n < 0 || (n > 0 && !u)) return ctx->fail(ZLH_EINVAL, "bad boundary velocities");
+    ctx->bc_node_u[{level, dir}] = std::vector<double>(u, u + 3 * n);
+    return ZLH_OK;
+}
+
+zlh_status zlh_set_interpolation_method(zlh_ctx *ctx, int level, int method) {
+    if (!ctx) return ZLH_EINVAL;
+    if (ctx->built) return ctx->fail(ZLH_ESTATE, "interpolation method after build");
+    if (level < 1 || level > ctx->d.max_level) return ctx->fail(ZLH_EINVAL, "bad fine level %d", level);
+    if (method != ZLH_INTERP_LAGRANGIAN && method != ZLH_INTERP_LINEAR)
+        return ctx->fail(ZLH_EINVAL, "bad interpolation method %d", method);
+    ctx->interp_method[level] = method;
     return ZLH_OK;
 }
 

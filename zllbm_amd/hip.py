@@ -24,6 +24,7 @@ LES_TAU_REFERENCE, LES_TAU_PHYSICAL = 0, 1
 PROFILE_UNIFORM, PROFILE_PARABOLIC = 0, 1
 MLP_FP32, MLP_BF16_MFMA, MLP_FP16_MFMA = 0, 1, 2
 MLP_OPERANDS_BF16, MLP_OPERANDS_FP16 = 0, 1
+INTERP_LAGRANGIAN, INTERP_LINEAR = 0, 1
 
 
 class SolverDesc(C.Structure):
@@ -44,6 +45,7 @@ EXPORTED = (
     "zlh_step_level", "zlh_transfer_levels", "zlh_time_marching", "zlh_download_populations", "zlh_download_macros",
     "zlh_download_source_table", "zlh_ib_download_forces", "zlh_conservation", "zlh_level_info", "zlh_transfer_info", "zlh_level_streams_enable", "zlh_set_boundary_profile",
     "zlh_synchronize", "zlh_timing_enable", "zlh_timing_read", "zlh_mlp_set", "zlh_mlp_gradient",
+    "zlh_set_boundary_velocities", "zlh_set_interpolation_method",
     "zlh_mlp_train", "zlh_mlp_training_form", "zlh_mlp_eval", "zlh_closure_features", "zlh_graphs_enable",
     "zlh_max_level_bricks",
 )
@@ -94,6 +96,8 @@ def lib():
         L.zlh_graphs_enable.argtypes = [P, C.c_int]
         L.zlh_level_streams_enable.argtypes = [P, C.c_int]
         L.zlh_set_boundary_profile.argtypes = [P, C.c_int, C.c_int]
+        L.zlh_set_boundary_velocities.argtypes = [P, C.c_int, C.c_int, D(np.float64), C.c_int64]
+        L.zlh_set_interpolation_method.argtypes = [P, C.c_int, C.c_int]
         L.zlh_max_level_bricks.argtypes = [C.POINTER(SolverDesc), C.POINTER(C.c_int64)]
         _lib = L
     return _lib
@@ -149,8 +153,12 @@ class Solver:
     """One device solver over a :class:`zllbm_amd.grid.Grid` (the SolverLbmInterface +
     GridInfoLbmInteface pair of the reference, lbm_interface.h:140-474)."""
 
-    def __init__(self, grid: "_grid.Grid", bc_profile=None, **kw):
-        """bc_profile: {face: PROFILE_PARABOLIC} for non-equilibrium-extrapolation faces."""
+    def __init__(self, grid: "_grid.Grid", bc_profile=None, bc_velocities=None, interp_method=INTERP_LAGRANGIAN,
+                 **kw):
+        """bc_profile: {face: PROFILE_PARABOLIC} for non-equilibrium-extrapolation faces;
+        bc_velocities: {(level, face): [n][3] velocities of the level's nodes on that face, in key
+        order} (zlh_set_boundary_velocities); interp_method: INTERP_LAGRANGIAN or INTERP_LINEAR
+        ghost-node interpolation of every refined level (zlh_set_interpolation_method)."""
         self.grid = grid
         self.dims = grid.dims
         self.q = 9 if grid.dims == 2 else 19
@@ -163,6 +171,11 @@ class Solver:
         self.h = h
         for face, prof in (bc_profile or {}).items():
             self._chk(L.zlh_set_boundary_profile(self.h, face, prof))
+        for level in range(1, grid.max_level + 1):
+            self._chk(L.zlh_set_interpolation_method(self.h, level, interp_method))
+        for (level, face), ub in (bc_velocities or {}).items():
+            ub = np.ascontiguousarray(ub, np.float64).reshape(-1, 3)
+            self._chk(L.zlh_set_boundary_velocities(self.h, level, face, ub.ravel(), len(ub)))
         for l, lev in enumerate(grid.levels):
             self._chk(L.zlh_set_level_nodes(self.h, l, _checked(lev.keys, np.uint64, lev.n, "level %d keys" % l),
                                             _checked(lev.flags, np.int32, lev.n, "level %d flags" % l), lev.n))

@@ -15,8 +15,8 @@
   profile everywhere.
 * ``sphere_amr()``: D3Q19 MRT-LES channel of 32 x 16 x 16 background cells (512 x 256 x 256 on
   the finest of 5 levels), each level refined within a band around a sphere of 48 finest cells
-  diameter (7,239 Fibonacci markers on the finest level), Re 300, inflow 0.05 with a seeded 1 %
-  velocity perturbation.
+  diameter (7,239 Fibonacci markers on the finest level), Re 300, started from uniform flow 0.05;
+  the inflow face carries 0.05 (1 + 0.01 xi) per node, xi seeded standard normal.
 """
 from __future__ import annotations
 
@@ -151,6 +151,21 @@ def cylinder_amr(cells=(512, 256), diameter_fine=32, u_in=0.05, re=100.0):
     return wl
 
 
+def perturbed_inflow(grid, u_in, perturb, seed):
+    """Inflow velocity u_in (1 + perturb xi), xi ~ N(0, 1) seeded, per node of the x-minimum face,
+    drawn level by level in key order; transverse components zero (zlh_set_boundary_velocities)."""
+    rng = np.random.default_rng(seed)
+    out = {}
+    for l, lev in enumerate(grid.levels):
+        on_face = grid.coords(l)[:, 0].astype(np.int64) == (int(grid.dom_min[0]) << l)
+        n = int(np.count_nonzero(on_face))
+        if n:
+            ub = np.zeros((n, 3))
+            ub[:, 0] = u_in * (1.0 + perturb * rng.standard_normal(n))
+            out[(l, g.XMIN)] = ub
+    return out
+
+
 def sphere_amr(cells=(32, 16, 16), levels=5, diameter_fine=48, u_in=0.05, re=300.0, perturb=0.01,
                band=8.0, margin=3, les=2, seed=20261202, buffer=2):
     """Level k is refined where a level-k cell centre lies within r + band level-k cells of the
@@ -167,14 +182,11 @@ def sphere_amr(cells=(32, 16, 16), levels=5, diameter_fine=48, u_in=0.05, re=300
     grid = g.refined_mask_grid(3, cells, levels, refine, margin=margin, buffer=buffer)
     nu = u_in * d_bg / re
     xyz, area = _geometry().fibonacci_sphere_markers(tuple(ctr + 1.0 + 0.013), r, 1.0 / (1 << (levels - 1)))
-    rng = np.random.default_rng(seed)
-    init = []
-    for lev in grid.levels:
-        u = np.tile([u_in, 0.0, 0.0], (lev.n, 1)) + rng.normal(0.0, perturb * u_in, (lev.n, 3))
-        init.append((np.ones(lev.n), u))
+    init = [(np.ones(lev.n), np.tile([u_in, 0.0, 0.0], (lev.n, 1))) for lev in grid.levels]
     wl = Workload("D3Q19 MRT-LES channel %dx%dx%d (%d levels), sphere D=%d finest cells, Re %g" % (
         cells + (levels, diameter_fine, re)), grid,
-        dict(nu0=nu, bc=_channel_bc(3, u_in), collision=(1,), les=les, les_tau_mode=1, cs=0.1, has_force=True), init)
+        dict(nu0=nu, bc=_channel_bc(3, u_in), collision=(1,), les=les, les_tau_mode=1, cs=0.1, has_force=True,
+             bc_velocities=perturbed_inflow(grid, u_in, perturb, seed)), init)
     wl.ib = IbSet(levels - 1, xyz, area)
     return wl
 
