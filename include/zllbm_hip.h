@@ -135,8 +135,14 @@ zlh_status zlh_upload_populations(zlh_ctx *ctx, int level, const double *f);
 zlh_status zlh_step_level(zlh_ctx *ctx, int level, int time_step_level);
 /* coarse-to-fine interpolation into the fine ghost layers, then fine-to-coarse copy into the
  * coarse interface layer (SolverLbmInterface::InformationFromGridOfDifferentLevel,
- * lbm_solver_interface.cpp:224-238; the caller applies it after even substeps of fine_level) */
+ * lbm_solver_interface.cpp:224-238; the caller applies it after even substeps of fine_level,
+ * ZLH_ESTATE after an odd one) */
 zlh_status zlh_transfer_levels(zlh_ctx *ctx, int fine_level);
+/* its two halves, in this order: the coarse-to-fine interpolation
+ * (GridInfoLbmInteface::TransferInfoFromCoarseGrid, lbm_grid_interface.cpp:408-515) and the
+ * fine-to-coarse copy (TransferInfoToCoarseGrid, lbm_grid_interface.cpp:524-564) */
+zlh_status zlh_transfer_from_coarse(zlh_ctx *ctx, int fine_level);
+zlh_status zlh_transfer_to_coarse(zlh_ctx *ctx, int fine_level);
 /* background steps in the coarse-to-fine multi time stepping order (AmrManager::TimeMarching,
  * amr_manager.cpp:279-300; MultiTimeSteppingC2F, grid/grid_manager.cpp:997-1024) */
 zlh_status zlh_time_marching(zlh_ctx *ctx, int nsteps);
@@ -155,6 +161,10 @@ zlh_status zlh_download_macros(zlh_ctx *ctx, int level, double *rho, double *u);
 /* pull-source table [n][q]: key-order index of the node x - c_q, -1 when absent; the device
  * replacement of the FindX/Y/Z{Pos,Neg} + map lookups (synchronising) */
 zlh_status zlh_download_source_table(zlh_ctx *ctx, int level, int32_t *src);
+/* node forces of a level in key order, [n][3] (GridNodeLbm::force_, lbm_interface.h:34-79, as
+ * the checkpoint writes them): the immersed-boundary force of the last forcing step on its
+ * supports, zero elsewhere (synchronising) */
+zlh_status zlh_download_node_forces(zlh_ctx *ctx, int level, double *force);
 /* Lagrangian marker forces of the last step [n][3] (synchronising) */
 zlh_status zlh_ib_download_forces(zlh_ctx *ctx, int level, double *force);
 /* FP64 mass and momentum sums of the post-stream populations of a level: out = {m, jx, jy, jz}

@@ -59,6 +59,88 @@ def reference_record_order(grid):
     return out
 
 
+# layer kinds in the order the reference's interface_nodes writer takes them, with their
+# indicators (kIndictor*, mpi/mpi_manager.h:615-616)
+KIND_ORDER = ((1, g.INNER_F2C), (2, g.OUTER_F2C), (3, g.INNER_C2F), (4, g.OUTER_C2F))
+
+
+def layer_vectors(lev):
+    """The driver's InterfaceLayerInfo of one level: per kind the list of layers (key sets), as
+    long as the largest layer index the grid names."""
+    per = {}
+    for (kind, layer), keys in lev.layers.items():
+        per.setdefault(kind, {})[layer] = set(int(k) for k in keys)
+    return {kind: [d.get(i, set()) for i in range(max(d) + 1)] for kind, d in per.items()}
+
+
+def _cells(grid):
+    dims = grid.dims
+    cells = {}
+    for l, lev in enumerate(grid.levels):
+        for k in lev.keys:
+            c = cells.setdefault(int(k) >> (dims * l), [0, 0, 0, 0])
+            c[0] |= 1 << l
+            c[1] += 1
+            c[3] += 1 << l
+        for kind, vec in layer_vectors(lev).items():
+            for layer in vec:
+                for k in layer:
+                    cells[k >> (dims * l)][2] += 1
+    return cells
+
+
+def reference_loadings(grid):
+    """Bytes of the reference's loadings file for one rank, by its own loops
+    (ComputeComputationalLoadOnEachRank + WriteCheckPointNodesAtWhichLevels): header {uint32 total
+    load, uint32 cells, int32 max level}, then per background cell by code {uint64 code, level
+    bits, int32 nodes, int32 interface entries, int32 load}; a level-l node costs 2^l."""
+    import struct
+    cells = _cells(grid)
+    nbytes = (grid.max_level + 1 + 7) // 8
+    out = struct.pack("<IIi", sum(c[3] for c in cells.values()), len(cells), grid.max_level)
+    for code in sorted(cells):
+        bits, nodes, iface, load = cells[code]
+        out += struct.pack("<Q", code) + bits.to_bytes(nbytes, "little") + struct.pack("<iii", nodes, iface, load)
+    return out
+
+
+def reference_interface_nodes(grid):
+    """Bytes of the reference's interface_nodes file for one rank (WriteCheckPointInterfaceNodes):
+    the index table of every layer of the one geometry interface (criterion 1, count 0) of every
+    level that has layers, then per background cell, level and node of the cell (z, y, x) one
+    {uint64 key, int32 index} per layer kind whose layers hold the node, naming the first."""
+    import struct
+    dims = grid.dims
+    vecs = [layer_vectors(lev) for lev in grid.levels]
+    index, table = {}, b""
+    for l, vec in enumerate(vecs):
+        if not grid.levels[l].layers:
+            continue
+        for ind, kind in KIND_ORDER:
+            for layer in range(len(vec.get(kind, []))):
+                index[(l, ind, layer)] = len(index)
+                table += struct.pack("<iiiii", l, 1, 0, ind, layer)
+    out = struct.pack("<i", len(index)) + table
+    cells = _cells(grid)
+    for code in sorted(cells):
+        b = g.morton_decode(dims, np.array([code], np.uint64))[0]
+        for l in range(grid.max_level + 1):
+            if not cells[code][0] >> l & 1:
+                continue
+            m = 1 << l
+            for z in (range(m) if dims == 3 else [0]):
+                for y in range(m):
+                    for x in range(m):
+                        c = [int(b[0]) * m + x, int(b[1]) * m + y] + ([int(b[2]) * m + z] if dims == 3 else [])
+                        k = int(g.morton_encode(dims, *[np.array([v], np.uint64) for v in c])[0])
+                        for ind, kind in KIND_ORDER:
+                            for layer, keys in enumerate(vecs[l].get(kind, [])):
+                                if k in keys:
+                                    out += struct.pack("<Qi", k, index[(l, ind, layer)])
+                                    break
+    return out
+
+
 def write_params(path, kw, dims, collision="srt", precision="fp64", closure_path=None, extra=()):
     lines = ["viscosity = %r" % kw["nu0"], "rho = 1.0",
              "velocity = " + " ".join(["0.0"] * dims), "collision = " + collision, "precision = " + precision]
@@ -74,7 +156,7 @@ def write_params(path, kw, dims, collision="srt", precision="fp64", closure_path
 
 
 def run(grid, kw, steps, workdir, collision="srt", precision="fp64", model=None, checkpoint_in=None,
-        checkpoint_out=None, vtu_out=None):
+        checkpoint_out=None, vtu_out=None, checkpoint_dir_in=None, checkpoint_dir_out=None):
     gpath = os.path.join(workdir, "grid.bin")
     ppath = os.path.join(workdir, "params.txt")
     opath = os.path.join(workdir, "out.bin")
@@ -89,6 +171,10 @@ def run(grid, kw, steps, workdir, collision="srt", precision="fp64", model=None,
         extra.append("checkpoint_out = " + checkpoint_out)
     if vtu_out:
         extra.append("vtu_out = " + vtu_out)
+    if checkpoint_dir_in:
+        extra.append("checkpoint_dir_in = " + checkpoint_dir_in)
+    if checkpoint_dir_out:
+        extra.append("checkpoint_dir_out = " + checkpoint_dir_out)
     write_params(ppath, kw, grid.dims, collision, precision, cpath, extra)
     proc = subprocess.run([DRIVER, gpath, ppath, str(steps), opath], capture_output=True, text=True, timeout=300)
     if proc.returncode != 0:

@@ -71,6 +71,29 @@ def test_amr_substep_by_substep_fp64():
             assert_close(s.f(l), o.f(l))
 
 
+def test_split_transfer_equals_whole_and_odd_substep_refused():
+    """The coarse-to-fine and fine-to-coarse halves, called in the reference's order, give the
+    whole transfer bit for bit; a transfer after an odd substep of the fine level is refused (its
+    ghost targets were not deferred)."""
+    grid, kw = sc.amr2d(levels=2)
+    kw["collision"] = (hip.MRT,)
+    _, a = _pair(grid, kw)
+    _, b = _pair(grid, kw)
+    for s in (a, b):
+        s.step_level(0, 1)
+        s.step_level(1, 1)
+        with pytest.raises(RuntimeError, match="odd substep"):
+            s.transfer_levels(1)
+        s.step_level(1, 2)
+    a.transfer_levels(1)
+    b.transfer_from_coarse(1)
+    b.transfer_to_coarse(1)
+    for s in (a, b):
+        s.step_level(0, 1)
+    for l in range(2):
+        np.testing.assert_array_equal(a.f(l), b.f(l))
+
+
 def test_amr_fp32_tracks_fp64():
     grid, kw = sc.amr3d()
     kw.update(collision=(hip.MRT,), les=1, cs=0.1)
@@ -98,6 +121,11 @@ def test_ib_cylinder_fp64(stencil):
     # FP64 atomics sum the spread in a different order than the reference's marker loop
     assert_close(s.f(0), o.f(0), rtol=1e-10)
     np.testing.assert_allclose(s.ib_force(0), o.ib_force(0), rtol=1e-8, atol=1e-12)
+    # node forces (GridNodeLbm::force_, as checkpoints write them): the spread on the supports,
+    # zero elsewhere
+    fs, fo = s.node_forces(0)[:, :2], o.node_force(0)[:, :2]
+    np.testing.assert_allclose(fs, fo, rtol=1e-8, atol=1e-13)
+    assert np.count_nonzero(np.abs(fo).sum(1)) > 0 and np.count_nonzero(np.abs(fs).sum(1)) == np.count_nonzero(np.abs(fo).sum(1))
 
 
 def test_ib_uniform_flow_force():

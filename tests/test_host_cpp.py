@@ -115,6 +115,43 @@ def test_driver_checkpoint_format_and_exact_restart(tmp_path, case):
         np.testing.assert_array_equal(restarted[l], straight[l])
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["amr2d", "amr3d"])
+def test_driver_checkpoint_folder_and_exact_restart(tmp_path, case):
+    """The checkpoint folder as the reference writes it (loadings, grid_nodes, interface_nodes):
+    loadings and interface_nodes byte for byte as the reference's own loops produce them from this
+    grid, grid_nodes as the single file; a run restarted from the folder continues bit for bit;
+    a folder whose loadings describe another grid is refused."""
+    grid, kw = sc.amr2d(levels=3) if case == "amr2d" else sc.amr3d()
+    q = 9 if grid.dims == 2 else 19
+    dirs = []
+    for name in ("first", "restart", "straight", "bad"):
+        (tmp_path / name).mkdir()
+        dirs.append(str(tmp_path / name))
+    ck = str(tmp_path / "ck")
+    proc, _ = host_driver.run(grid, kw, 10, dirs[0], collision="mrt", checkpoint_dir_out=ck)
+    assert proc.returncode == 0, proc.stderr
+    with open(ck + "/loadings", "rb") as fh:
+        assert fh.read() == host_driver.reference_loadings(grid)
+    with open(ck + "/interface_nodes", "rb") as fh:
+        got = fh.read()
+    want = host_driver.reference_interface_nodes(grid)
+    assert len(got) == len(want) and got == want
+    count, payload, data, rest = host_driver.read_checkpoint(ck + "/grid_nodes", q)
+    assert count == sum(lev.n for lev in grid.levels) and rest == 0
+    proc, restarted = host_driver.run(grid, kw, 10, dirs[1], collision="mrt", checkpoint_dir_in=ck)
+    assert proc.returncode == 0, proc.stderr
+    proc, straight = host_driver.run(grid, kw, 20, dirs[2], collision="mrt")
+    assert proc.returncode == 0, proc.stderr
+    for l in range(grid.max_level + 1):
+        np.testing.assert_array_equal(restarted[l], straight[l])
+    raw = bytearray(open(ck + "/loadings", "rb").read())
+    raw[-4] ^= 1                                            # the last cell's load
+    open(ck + "/loadings", "wb").write(bytes(raw))
+    proc, _ = host_driver.run(grid, kw, 1, dirs[3], collision="mrt", checkpoint_dir_in=ck)
+    assert proc.returncode != 0 and "loadings do not describe" in proc.stderr
+
+
 @pytest.mark.parametrize("case", ["amr2d", "amr3d"])
 def test_checkpoint_order_rule_matches_reference_loops(case):
     """The host's record order (sort by background cell, level, then z, y, x inside the cell)

@@ -429,17 +429,17 @@ inline zlh_status plan_transfer(zlh_ctx *ctx, int level) {
                             const int64_t j = find_key(C.keys, ck);
                             if (j >= 0 && !(C.flags[j] & (1 << 2))) {   // kNodeStatusCoarse2FineGhost_
                                 P.contrib.push_back(j);
-                                C.aux_mark[j] = 1;
+                                C.aux_mark[j] |= kAuxRecorded;   // read by the transfer
                             } else {
                                 const int64_t k = find_key(F.keys, ck << dims);
                                 if (k < 0) { broken = true; ++P.incomplete; break; }
                                 P.contrib.push_back(-k - 1);
-                                F.aux_mark[k] = 1;
+                                F.aux_mark[k] |= kAuxRecorded;
                             }
                             P.coeff.push_back(coeff);
                         }
                 P.begin.push_back((int64_t)P.contrib.size());
-                F.aux_mark[ti] = 1;
+                F.aux_mark[ti] |= kAuxRecorded | kAuxDeferOnTransfer;   // written by the transfer
             }
         }
     }
@@ -457,8 +457,8 @@ inline zlh_status plan_transfer(zlh_ctx *ctx, int level) {
             if (i < 0) continue;
             P.f2c_coarse.push_back(j);
             P.f2c_fine.push_back(i);
-            C.aux_mark[j] = 1;
-            F.aux_mark[i] = 1;
+            C.aux_mark[j] |= kAuxRecorded | kAuxDeferAlways;   // written after every coarse substep
+            F.aux_mark[i] |= kAuxRecorded | kAuxDeferAlways;   // its collision's rho, u are read
         }
     }
     return ZLH_OK;
@@ -522,7 +522,7 @@ inline zlh_status plan_ib(zlh_ctx *ctx, int level) {
                     if (j < 0) return ctx->fail(ZLH_EINVAL, "marker %lld stencil node absent at level %d", (long long)m, level);
                     P.node[m * P.support + e] = j;
                     P.weight[m * P.support + e] = dims == 3 ? wx * wy * wz : wx * wy;
-                    L.aux_mark[j] = 1;
+                    L.aux_mark[j] |= kAuxRecorded | kAuxDeferAlways;   // forced before its collision
                 }
             }
         }
@@ -752,8 +752,11 @@ inline zlh_status upload_ib(zlh_ctx *ctx, int level) {
 // ----------------------------------------------------------------------------------------------
 // launches
 // ----------------------------------------------------------------------------------------------
+// parts: kTransferC2F (TransferInfoFromCoarseGrid) and / or kTransferF2C (TransferInfoToCoarseGrid),
+// in that order (InformationFromGridOfDifferentLevel, lbm_solver_interface.cpp:224-238)
+constexpr int kTransferC2F = 1, kTransferF2C = 2;
 template <int D, typename T, bool SHIFT>
-zlh_status launch_transfers_t(zlh_ctx *ctx, int level) {
+zlh_status launch_transfers_t(zlh_ctx *ctx, int level, int parts) {
     LevelState &F = ctx->lv[level], &C = ctx->lv[level - 1];
     TransferTables &X = F.xfer;
     const int comp = ctx->d.compressible;
@@ -763,7 +766,7 @@ zlh_status launch_transfers_t(zlh_ctx *ctx, int level) {
     const double dtc = 1. / (double)(1u << (level - 1)), dtf = 1. / (double)(1u << level);
     const double c2f = 0.5 * (nu * 3. / dtc * 2. + 0.5) / (nu * 3. / dtc + 0.5);
     const double f2c = 2. * (nu * 3. / dtf / 2 + 0.5) / (nu * 3. / dtf + 0.5);
-    if (X.n_c2f > 0) {
+    if ((parts & kTransferC2F) && X.n_c2f > 0) {
         if (X.n_src + X.n_fb > 0)
             hipLaunchKernelGGL((k_c2f_sources<D, T, SHIFT>), dim3(blocks(X.n_src + X.n_fb)), dim3(256), 0, ctx->stream,
                                X.n_src, X.n_fb, X.src_coarse_ai.as<int32_t>(), C.aux_f_d.as<T>(), C.naux,
@@ -773,7 +776,7 @@ zlh_status launch_transfers_t(zlh_ctx *ctx, int level) {
                            X.c2f_begin.as<int32_t>(), X.c2f_contrib.as<uint32_t>(), X.group_rows.as<int32_t>(),
                            X.src_f.as<T>(), F.aux_f_d.as<T>(), F.naux);
     }
-    if (X.n_f2c > 0)
+    if ((parts & kTransferF2C) && X.n_f2c > 0)
         hipLaunchKernelGGL((k_f2c<D, T, SHIFT>), dim3(blocks(X.n_f2c)), dim3(256), 0, ctx->stream, X.n_f2c,
                            X.f2c_coarse_ai.as<int32_t>(), X.f2c_fine_ai.as<int32_t>(), F.aux_f_d.as<T>(), F.naux,
                            F.aux_macro_d.as<double>(), f2c, comp, C.aux_f_d.as<T>(), C.naux);
@@ -781,14 +784,14 @@ zlh_status launch_transfers_t(zlh_ctx *ctx, int level) {
     return ZLH_OK;
 }
 
-inline zlh_status launch_transfers(zlh_ctx *ctx, int level) {
+inline zlh_status launch_transfers(zlh_ctx *ctx, int level, int parts = kTransferC2F | kTransferF2C) {
     if (ctx->d.dims == 2)
-        return ctx->fp64() ? launch_transfers_t<2, double, false>(ctx, level)
-             : ctx->bf16() ? launch_transfers_t<2, bf16_t, true>(ctx, level)
-                           : launch_transfers_t<2, float, true>(ctx, level);
-    return ctx->fp64() ? launch_transfers_t<3, double, false>(ctx, level)
-         : ctx->bf16() ? launch_transfers_t<3, bf16_t, true>(ctx, level)
-                       : launch_transfers_t<3, float, true>(ctx, level);
+        return ctx->fp64() ? launch_transfers_t<2, double, false>(ctx, level, parts)
+             : ctx->bf16() ? launch_transfers_t<2, bf16_t, true>(ctx, level, parts)
+                           : launch_transfers_t<2, float, true>(ctx, level, parts);
+    return ctx->fp64() ? launch_transfers_t<3, double, false>(ctx, level, parts)
+         : ctx->bf16() ? launch_transfers_t<3, bf16_t, true>(ctx, level, parts)
+                       : launch_transfers_t<3, float, true>(ctx, level, parts);
 }
 
 template <int D, typename T, typename Cc, bool SHIFT>

@@ -65,7 +65,8 @@ struct BrickLevel {
     int64_t naux;
     const uint64_t *occ;      // [nbrick]
     const int32_t *nbr;       // [nbrick][NN]
-    const uint64_t *auxmask;  // [nbrick]
+    const uint64_t *auxmask;  // [nbrick] nodes whose post-stream values go to the aux array
+    const uint64_t *defmask;  // [nbrick] of those, the nodes whose collision waits for the aux collision
     const int32_t *auxbase;   // [nbrick]
     const uint32_t *aux_missing;  // [naux]
     const int32_t *aux_slot;  // [naux]
@@ -139,7 +140,7 @@ __device__ __forceinline__ C relaxation_time(int les, const C *pi, C rho, C drho
     if (!les) return t0;
     if (les == 1 || les == 3) {
         const C pi2 = pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2] + C(2) * (pi[3] * pi[3] + pi[4] * pi[4] + pi[5] * pi[5]);
-        const C tau_t = C(0.5) * (sqrt(t0 * t0 + C(p.cs18) * sqrt(C(2) * pi2)) - t0);
+        const C tau_t = C(0.5) * (root(t0 * t0 + C(p.cs18) * root(C(2) * pi2)) - t0);
         return les == 1 ? t0 + tau_t : tau_t;
     }
     if constexpr (std::is_same<C, float>::value) {
@@ -427,6 +428,7 @@ __device__ __forceinline__ void stream_collide_brick(const BrickLevel<T> &lv, co
     if (!kWave && !has_node) return;
     const uint64_t am = lv.auxmask[b];
     const bool is_aux = (am >> lane) & 1ull;
+    const bool deferred = (lv.defmask[b] >> lane) & 1ull;
     int64_t ai = 0;
     uint32_t missing = 0;
     if (is_aux) {
@@ -463,12 +465,13 @@ __device__ __forceinline__ void stream_collide_brick(const BrickLevel<T> &lv, co
     if (is_aux) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) aux_f[q * lv.naux + ai] = T(f[q]);
-        if (!kWave) return;
+        // a recorded node that nothing changes before its next collision collides here
+        if (!kWave && deferred) return;
     }
     C F[3] = {C(p.const_force[0]), C(p.const_force[1]), C(p.const_force[2])};
     C rho, u[3];
     collide<D, SHIFT, COLL, C, VAR>(f, F, p, rho, u, ops);
-    if (kWave && (!has_node || is_aux)) return;
+    if (kWave && (!has_node || deferred)) return;
     // streaming stores: the next substep reads these values through other CUs' caches
     T *__restrict__ dst = fout + b * Q * 64 + lane;
 #pragma unroll
@@ -676,16 +679,19 @@ void k_stream_collide_mlp(int64_t nbrick, int64_t naux, const uint64_t *__restri
 // LDS first.
 template <int D, typename T, typename C, bool SHIFT, int COLL, int VAR>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(mlp_on_mfma<C, VAR>() ? ((VAR & 1) ? 3 : 4) : 1)))
-void k_aux_collide(BrickLevel<T> lv, const T *__restrict__ aux_f,
+void k_aux_collide(BrickLevel<T> lv, const int32_t *__restrict__ list, int64_t nlist,
+                                                     const T *__restrict__ aux_f,
                                                      const double *__restrict__ aux_force,
                                                      T *__restrict__ fa, double *__restrict__ aux_macro,
                                                      const uint8_t *__restrict__ macro_flag, StepParams p) {
     constexpr int Q = Lattice<D>::Q;
     constexpr bool kWave = mlp_on_mfma<C, VAR>();
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     // lanes past the end run the collision on the rest state (FP32 learned closure is
     // wavefront-wide) and store nothing
-    const bool active = i < lv.naux;
+    const bool active = t < nlist;
+    // the aux index: entry t of the list of deferred nodes, or every recorded node in turn
+    const int64_t i = active ? (list ? (int64_t)list[t] : t) : 0;
     if (!kWave && !active) return;
     C f[Q];
 #pragma unroll

@@ -65,6 +65,14 @@ __device__ __forceinline__ C recip(C x) {
     else return C(1) / x;
 }
 
+// square root: correctly rounded in FP64; in FP32 the hardware square root (about 1 ulp, one
+// instruction where the correctly rounded sqrtf takes about fifteen)
+template <typename C>
+__device__ __forceinline__ C root(C x) {
+    if constexpr (std::is_same<C, float>::value) return __builtin_amdgcn_sqrtf(x);
+    else return sqrt(x);
+}
+
 // raw closure inputs from the non-equilibrium stress Pi (xx, yy, zz, xy, xz, yz), density and
 // velocity: P = Pi / rho, a = sqrt(2 P:P), I1 = tr P, I2 = (I1^2 - P:P) / 2, I3 = det P, |u|^2,
 // rho - 1, tau0, a tau0, a^2
@@ -73,7 +81,7 @@ __device__ __forceinline__ void closure_features(const C *pi, C rho, C drho, con
     const C ir = recip(rho);
     const C pxx = pi[0] * ir, pyy = pi[1] * ir, pzz = pi[2] * ir, pxy = pi[3] * ir, pxz = pi[4] * ir, pyz = pi[5] * ir;
     const C pp = pxx * pxx + pyy * pyy + pzz * pzz + C(2) * (pxy * pxy + pxz * pxz + pyz * pyz);
-    const C a = sqrt(C(2) * pp);
+    const C a = root(C(2) * pp);
     const C i1 = pxx + pyy + pzz;
     const C i2 = C(0.5) * (i1 * i1 - pp);
     const C i3 = pxx * (pyy * pzz - pyz * pyz) - pxy * (pxy * pzz - pyz * pxz) + pxz * (pxy * pyz - pyy * pxz);
@@ -123,9 +131,9 @@ __device__ __forceinline__ C mlp_tau_t(const MlpDev &m, const C *pi, C rho, C dr
     for (int i = 0; i < kMlpF; ++i) inside = inside && x[i] >= m.fmin[i] && x[i] <= m.fmax[i];
     if (!inside) {
         *fallback = true;
-        const C q = sqrt(C(2) * (pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2] +
+        const C q = root(C(2) * (pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2] +
                                  C(2) * (pi[3] * pi[3] + pi[4] * pi[4] + pi[5] * pi[5])));
-        return C(0.5) * (sqrt(tau0 * tau0 + cs18 * q) - tau0);
+        return C(0.5) * (root(tau0 * tau0 + cs18 * q) - tau0);
     }
     *fallback = false;
     float xn[kMlpF];
@@ -186,6 +194,14 @@ __device__ __forceinline__ float silu_scaled(float t) {
     return t * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(t));
 }
 
+// softplus of the matrix-core form: max(z, 0) + ln 2 log2(1 + 2^(-|z| log2 e)) with the hardware
+// exp2 and log2 (about 1 ulp each).  Rounding 1 + e to FP32 costs at most 2^-24 absolute where
+// e is small, far below the operands' rounding; the library log1pf takes about 110 instructions.
+__device__ __forceinline__ float softplus_fast(float z) {
+    const float e = __builtin_amdgcn_exp2f(-1.4426950408889634f * __builtin_fabsf(z));
+    return __builtin_fmaxf(z, 0.f) + 0.6931471805599453f * __builtin_amdgcn_logf(1.f + e);
+}
+
 // value held by lane l ^ 32 (v_permlane32_swap exchanges the halves of a register pair; unlike
 // ds_bpermute it is no LDS instruction, so it never waits on direct-to-LDS loads in flight)
 __device__ __forceinline__ float swap_halves(float v) {
@@ -200,19 +216,18 @@ __device__ __forceinline__ float mlp_forward_wave(const MlpOps &o, const float *
     typedef typename OT::elem elem;
     const int lane = __lane_id();
     const int h = lane >> 5;
-    float px[kMlpF];
-#pragma unroll
-    for (int k = 0; k < kMlpF; ++k) px[k] = swap_halves(xn[k]);
     // layer-1 B operands: column block 0 = nodes 0..31, block 1 = nodes 32..63; lane l carries
-    // K slots 8 h .. 8 h + 7 of node l % 32 (block 0) or 32 + l % 32 (block 1); slot 9 = 1 (bias)
+    // K slots 8 h .. 8 h + 7 of node l % 32 (block 0) or 32 + l % 32 (block 1); slot 9 = 1 (bias).
+    // Slot e takes feature e in the lower half and t_e (feature 8, the bias 1, zeros) in the upper
+    // half: v_permlane32_swap(xn[e], t_e) leaves exactly block 0's operand in the first register
+    // and block 1's in the second.
     vec x0, x1;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-        const float hi = e == 1 ? 1.f : 0.f;
-        const float v0 = e == 0 ? (h ? px[8] : xn[0]) : (h ? hi : xn[e]);
-        const float v1 = e == 0 ? (h ? xn[8] : px[0]) : (h ? hi : px[e]);
-        x0[e] = (elem)v0;
-        x1[e] = (elem)v1;
+        const float t = e == 0 ? xn[8] : (e == 1 ? 1.f : 0.f);
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(xn[e]), __float_as_uint(t), false, false);
+        x0[e] = (elem)__uint_as_float(r[0]);
+        x1[e] = (elem)__uint_as_float(r[1]);
     }
     const f32x16_t zero = {};
     f32x16_t z0, z1;
@@ -245,9 +260,10 @@ __device__ __forceinline__ float mlp_forward_wave(const MlpOps &o, const float *
         p0 += ww * silu_scaled(y0[r]);
         p1 += ww * silu_scaled(y1[r]);
     }
-    // node l % 32 + 32 h: own half of its block plus the other half's, held by lane l ^ 32
-    const float recv = swap_halves(h ? p0 : p1);
-    return softplusf_((h ? p1 : p0) + recv + b3);
+    // node l % 32 + 32 h sums the two halves of its block: after v_permlane32_swap(p0, p1) the
+    // lower lanes hold block 0's halves (p0 of lanes l and l + 32) and the upper lanes block 1's
+    const auto pr = __builtin_amdgcn_permlane32_swap(__float_as_uint(p0), __float_as_uint(p1), false, false);
+    return softplus_fast(__uint_as_float(pr[0]) + __uint_as_float(pr[1]) + b3);
 }
 
 // mlp_tau_t for the FP32 solver, evaluated by the whole wavefront
@@ -263,9 +279,13 @@ __device__ __forceinline__ float mlp_tau_t_wave(const MlpOps &o, const float *pi
         xn[i] = (x[i] - o.mean[i]) * o.inv_std[i];
     }
     const float net = mlp_forward_wave<OT>(o, xn, o.b3);
-    const float q = sqrtf(2.f * (pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2] +
-                                 2.f * (pi[3] * pi[3] + pi[4] * pi[4] + pi[5] * pi[5])));
-    const float smag = 0.5f * (sqrtf(tau0 * tau0 + cs18 * q) - tau0);
+    // the Smagorinsky fallback only where some lane of the wavefront needs it
+    float smag = 0.f;
+    if (__ballot(!inside)) {
+        const float q = root(2.f * (pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2] +
+                                    2.f * (pi[3] * pi[3] + pi[4] * pi[4] + pi[5] * pi[5])));
+        smag = 0.5f * (root(tau0 * tau0 + cs18 * q) - tau0);
+    }
     return inside ? 3.f * o.nu_scale * net : smag;
 }
 

@@ -103,6 +103,15 @@ struct IbPlan {
     int support = 0;
 };
 
+// Roles of a node on the deferred ("aux") path.  Its post-stream values are kept in the aux array
+// (kAuxRecorded) for the boundary, transfer and forcing kernels to read or change; its collision
+// waits for the next substep's aux collision where they change after the fused kernel: after
+// every substep (kAuxDeferAlways: domain-boundary nodes, immersed-boundary supports, fine-to-
+// coarse targets and sources, nodes with an absent pull source) or only after the substeps that
+// the transfer into the level follows (kAuxDeferOnTransfer: coarse-to-fine ghost targets).  A
+// recorded node that is not deferred collides in the fused kernel like a bulk node.
+constexpr uint8_t kAuxRecorded = 1, kAuxDeferOnTransfer = 2, kAuxDeferAlways = 4;
+
 struct LevelState {
     // host copies
     std::vector<uint64_t> keys;
@@ -115,7 +124,7 @@ struct LevelState {
     std::vector<uint64_t> bkeys;
     std::vector<int32_t> node_slot_h, node_aux_h;
     std::vector<uint32_t> missing_h;
-    std::vector<uint8_t> aux_mark;   // nodes that must take the deferred path
+    std::vector<uint8_t> aux_mark;   // kAux* roles per node (0: bulk)
     // aux nodes of this level that the transfer from the coarser level writes (ghost targets) and
     // reads (fallback rows, fine-to-coarse sources), and whether the transfer into the next finer
     // level conflicts with them (writes a node the first touches, or reads a node it writes)
@@ -134,9 +143,20 @@ struct LevelState {
     IbTables ibt;
     TransferPlan xplan;
     IbPlan ibplan;
+    // deferred nodes per kind of substep (0: no transfer into the level follows, 1: one does): a
+    // per-brick lane mask for the fused kernel and the list of their aux indices for the aux
+    // collision that follows
+    DevBuf defmask_d[2], deflist_d[2];
+    int64_t ndef[2] = {0, 0};
     int cur = 0;        // buffer read by the next fused kernel
     bool fresh = true;  // no step since upload
+    // nodes the next aux collision takes: kPendingAll (every recorded node: after an upload), or
+    // the deferred nodes of the last substep's kind (kPendingPlain, kPendingTransfer)
+    int pending = 0;
+    int last_tsl = 0;   // substep index of the last step (a transfer into the level needs it even)
     StepParams sp{};
 };
+
+constexpr int kPendingAll = 0, kPendingPlain = 1, kPendingTransfer = 2;
 
 }  // namespace zlh

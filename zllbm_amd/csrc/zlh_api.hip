@@ -395,7 +395,7 @@ struct zlh_ctx {
     // two background steps captured as one graph, per combination of the levels' buffer
     // indices (two steps flip every level's buffers an even number of times, so replays chain)
     bool use_graphs = true;
-    std::map<uint32_t, hipGraphExec_t> graph_exec;
+    std::map<uint64_t, hipGraphExec_t> graph_exec;
     // time marching runs each level on a stream of its own; events order a transfer after the
     // coarse step it pairs with, and the coarse level's next step after the transfer
     int bc_profile[7] = {};   // per face: ZLH_PROFILE_*
@@ -506,6 +506,7 @@ BrickLevel<T> view(const LevelState &L) {
     v.occ = L.occ_d.as<uint64_t>();
     v.nbr = L.nbr_d.as<int32_t>();
     v.auxmask = L.auxmask_d.as<uint64_t>();
+    v.defmask = L.defmask_d[0].as<uint64_t>();
     v.auxbase = L.auxbase_d.as<int32_t>();
     v.aux_missing = L.aux_missing_d.as<uint32_t>();
     v.aux_slot = L.aux_slot_d.as<int32_t>();
@@ -594,12 +595,12 @@ void launch_stream_collide_v(zlh_ctx *ctx, int var, unsigned grid, const BrickLe
 }
 
 template <int D, typename T, typename C, bool SHIFT, int COLL>
-void launch_aux_collide_v(zlh_ctx *ctx, int var, LevelState &L, const BrickLevel<T> &v, T *aux_f,
-                          const double *aux_force, T *fa) {
+void launch_aux_collide_v(zlh_ctx *ctx, int var, LevelState &L, const BrickLevel<T> &v, const int32_t *list,
+                          int64_t nlist, T *aux_f, const double *aux_force, T *fa) {
 #define ZLH_AC(V)                                                                                             \
     case V:                                                                                                   \
-        hipLaunchKernelGGL((k_aux_collide<D, T, C, SHIFT, COLL, V>), dim3(blocks(L.naux)), dim3(256), 0,       \
-                           ctx->stream, v, aux_f, aux_force, fa, L.aux_macro_d.as<double>(),                  \
+        hipLaunchKernelGGL((k_aux_collide<D, T, C, SHIFT, COLL, V>), dim3(blocks(nlist)), dim3(256), 0,        \
+                           ctx->stream, v, list, nlist, aux_f, aux_force, fa, L.aux_macro_d.as<double>(),     \
                            L.aux_macro_flag_d.as<uint8_t>(), L.sp);                                           \
         break;
     switch (var) {
@@ -624,10 +625,22 @@ zlh_status launch_step_kernels(zlh_ctx *ctx, int level, int coll) {
     T *aux_f = L.aux_f_d.as<T>();
     const double *aux_force = L.aux_force_d.as<double>();
     const int var = 2 * les_code(L.sp.les, L.sp.les_tau_mode, L.sp.mlp_operands) + (L.sp.has_force ? 1 : 0);
-    if (L.naux > 0) {
-        if (coll == ZLH_SRT) launch_aux_collide_v<D, T, C, SHIFT, 0>(ctx, var, L, v, aux_f, aux_force, fa);
-        else launch_aux_collide_v<D, T, C, SHIFT, 1>(ctx, var, L, v, aux_f, aux_force, fa);
+    // the aux collision takes the nodes deferred by the last substep (every recorded node after
+    // an upload)
+    const int32_t *list = nullptr;
+    int64_t nlist = L.naux;
+    if (L.pending != kPendingAll) {
+        const int pk = L.pending == kPendingTransfer ? 1 : 0;
+        list = L.deflist_d[pk].as<int32_t>();
+        nlist = L.ndef[pk];
     }
+    if (nlist > 0) {
+        if (coll == ZLH_SRT) launch_aux_collide_v<D, T, C, SHIFT, 0>(ctx, var, L, v, list, nlist, aux_f, aux_force, fa);
+        else launch_aux_collide_v<D, T, C, SHIFT, 1>(ctx, var, L, v, list, nlist, aux_f, aux_force, fa);
+    }
+    // this substep defers the ghost targets too when the transfer into the level follows it
+    const int kind = (level > 0 && L.last_tsl % 2 == 0) ? 1 : 0;
+    v.defmask = L.defmask_d[kind].as<uint64_t>();
     const int64_t groups = (L.nbrick + 3) / 4;
     const unsigned grid = (unsigned)(((groups + 7) / 8) * 8);
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -675,6 +688,7 @@ zlh_status launch_step_kernels(zlh_ctx *ctx, int level, int coll) {
     ZLH_TRY(hipGetLastError());
     L.cur ^= 1;
     L.fresh = false;
+    L.pending = kind ? kPendingTransfer : kPendingPlain;
     return ZLH_OK;
 }
 
@@ -692,6 +706,8 @@ zlh_status dispatch_upload(zlh_ctx *ctx, int level, const double *f_dev) {
                            L.node_slot_d.as<int32_t>(), L.node_aux_d.as<int32_t>(), fa, fb, L.aux_f_d.as<T>(), L.sp);
     ZLH_TRY(hipGetLastError());
     L.fresh = true;
+    L.pending = kPendingAll;
+    L.last_tsl = 0;
     return ZLH_OK;
 }
 
@@ -815,7 +831,7 @@ zlh_status build_level_slots(zlh_ctx *ctx, int level) {
     // nodes with an absent pull source keep that population from the previous substep
     L.aux_mark.assign(L.n, 0);
     for (int64_t i = 0; i < L.n; ++i)
-        if (L.missing_h[i]) L.aux_mark[i] = 1;
+        if (L.missing_h[i]) L.aux_mark[i] = kAuxRecorded | kAuxDeferAlways;
     // domain boundaries (CheckIfNodeOutsideCubicDomain, grid/grid_info.cpp:84-142) and the
     // nodes their conditions read
     for (int d = 0; d < 7; ++d) { L.bnode[d].clear(); L.bpart[d].clear(); }
@@ -838,8 +854,8 @@ zlh_status build_level_slots(zlh_ctx *ctx, int level) {
                 }
                 L.bnode[dir].push_back(i);
                 L.bpart[dir].push_back(partner);
-                L.aux_mark[i] = 1;
-                if (partner >= 0) L.aux_mark[partner] = 1;
+                L.aux_mark[i] |= kAuxRecorded | kAuxDeferAlways;
+                if (partner >= 0) L.aux_mark[partner] |= kAuxRecorded;   // read by the condition
             }
         }
     }
@@ -876,6 +892,27 @@ zlh_status build_level_aux(zlh_ctx *ctx, int level) {
         auxbase[b] = acc;
         acc += __builtin_popcountll(auxmask[b]);
     }
+    // deferred nodes per kind of substep: always-deferred roles on both, ghost targets only on the
+    // substeps that the transfer into this level follows
+    std::vector<uint64_t> defmask[2] = {std::vector<uint64_t>(L.nbrick, 0), std::vector<uint64_t>(L.nbrick, 0)};
+    std::vector<int32_t> deflist[2];
+    for (int64_t k = 0; k < L.naux; ++k) {
+        const uint8_t m = L.aux_mark[aux_nodes[k]];
+        const int32_t slot = L.node_slot_h[aux_nodes[k]];
+        for (int kind = 0; kind < 2; ++kind) {
+            const uint8_t deferred = kind ? (kAuxDeferAlways | kAuxDeferOnTransfer) : kAuxDeferAlways;
+            if (m & deferred) {
+                defmask[kind][slot >> 6] |= 1ull << (slot & 63);
+                deflist[kind].push_back((int32_t)k);
+            }
+        }
+    }
+    for (int kind = 0; kind < 2; ++kind) {
+        L.ndef[kind] = (int64_t)deflist[kind].size();
+        if ((s = upload(ctx, L.defmask_d[kind], defmask[kind]))) return s;
+        if ((s = upload(ctx, L.deflist_d[kind], deflist[kind]))) return s;
+    }
+    L.pending = kPendingAll;
     if ((s = upload(ctx, L.auxmask_d, auxmask))) return s;
     if ((s = upload(ctx, L.auxbase_d, auxbase))) return s;
     if ((s = upload(ctx, L.aux_slot_d, aux_slot))) return s;
@@ -1144,15 +1181,39 @@ zlh_status zlh_step_level(zlh_ctx *ctx, int level, int time_step_level) {
     // immersed-boundary forcing (UpdateCriterion, lbm_grid_interface.cpp:61-95)
     if ((s = launch_ib(ctx, level))) return s;
     // collision, streaming and domain boundaries (RunSolverOnGivenGrid + ComputeDomainBoundaryCondition)
+    ctx->lv[level].last_tsl = time_step_level;
     if ((s = do_step_kernels(ctx, level))) return s;
     return ZLH_OK;
 }
 
-zlh_status zlh_transfer_levels(zlh_ctx *ctx, int fine_level) {
+}  // extern "C"
+
+namespace {
+zlh_status transfer_checked(zlh_ctx *ctx, int fine_level, int parts) {
     if (!ctx) return ZLH_EINVAL;
     if (!ctx->built) return ctx->fail(ZLH_ESTATE, "build before transfer");
     if (fine_level < 1 || fine_level > ctx->d.max_level) return ctx->fail(ZLH_EINVAL, "bad fine level %d", fine_level);
-    return launch_transfers(ctx, fine_level);
+    // InformationFromGridOfDifferentLevel (lbm_solver_interface.cpp:224-238) exchanges after the
+    // fine level's even substeps, whose fused kernel defers the ghost targets this writes
+    if (ctx->lv[fine_level].last_tsl % 2 != 0)
+        return ctx->fail(ZLH_ESTATE, "transfer into level %d after its odd substep %d", fine_level,
+                         ctx->lv[fine_level].last_tsl);
+    return launch_transfers(ctx, fine_level, parts);
+}
+}  // namespace
+
+extern "C" {
+
+zlh_status zlh_transfer_levels(zlh_ctx *ctx, int fine_level) {
+    return transfer_checked(ctx, fine_level, kTransferC2F | kTransferF2C);
+}
+
+zlh_status zlh_transfer_from_coarse(zlh_ctx *ctx, int fine_level) {
+    return transfer_checked(ctx, fine_level, kTransferC2F);
+}
+
+zlh_status zlh_transfer_to_coarse(zlh_ctx *ctx, int fine_level) {
+    return transfer_checked(ctx, fine_level, kTransferF2C);
 }
 
 }  // extern "C"
@@ -1264,8 +1325,10 @@ zlh_status zlh_time_marching(zlh_ctx *ctx, int nsteps) {
     if (!ctx->use_graphs || ctx->timing) return march_launches(ctx, nsteps);
     int t = 0;
     while (nsteps - t >= 2) {
-        uint32_t key = 0;
-        for (int l = 0; l <= ctx->d.max_level; ++l) key |= (uint32_t)(ctx->lv[l].cur & 1) << l;
+        // buffer index and the aux collision's pending node set of every level
+        uint64_t key = 0;
+        for (int l = 0; l <= ctx->d.max_level; ++l)
+            key |= (uint64_t)(ctx->lv[l].cur & 1) << l | (uint64_t)(ctx->lv[l].pending & 3) << (16 + 2 * l);
         auto it = ctx->graph_exec.find(key);
         if (it == ctx->graph_exec.end()) {
             hipGraphExec_t g = nullptr;
@@ -1275,8 +1338,14 @@ zlh_status zlh_time_marching(zlh_ctx *ctx, int nsteps) {
             ZLH_TRY(hipGraphLaunch(g, ctx->stream));
         } else {
             ZLH_TRY(hipGraphLaunch(it->second, ctx->stream));
-            // host state after two steps: same buffer indices, post-stream data current
-            for (auto &L : ctx->lv) L.fresh = false;
+            // host state after two steps: same buffer indices, post-stream data current, and each
+            // level's last substep the last of a background step (even on refined levels)
+            for (int l = 0; l <= ctx->d.max_level; ++l) {
+                LevelState &L = ctx->lv[l];
+                L.fresh = false;
+                L.last_tsl = 1 << l;
+                L.pending = l > 0 ? kPendingTransfer : kPendingPlain;
+            }
         }
         t += 2;
     }
@@ -1390,6 +1459,22 @@ zlh_status zlh_download_source_table(zlh_ctx *ctx, int level, int32_t *src) {
     ZLH_TRY(hipGetLastError());
     ZLH_TRY(hipMemcpyAsync(src, tmp.p, sizeof(int32_t) * L.n * ctx->q, hipMemcpyDeviceToHost, ctx->stream));
     ZLH_TRY(hipStreamSynchronize(ctx->stream));
+    return ZLH_OK;
+}
+
+zlh_status zlh_download_node_forces(zlh_ctx *ctx, int level, double *force) {
+    if (!ctx) return ZLH_EINVAL;
+    if (!ctx->built) return ctx->fail(ZLH_ESTATE, "not built");
+    if (level < 0 || level > ctx->d.max_level || !force) return ctx->fail(ZLH_EINVAL, "bad arguments");
+    LevelState &L = ctx->lv[level];
+    std::vector<double> aux((size_t)3 * std::max<int64_t>(L.naux, 1));
+    ZLH_TRY(hipMemcpyAsync(aux.data(), L.aux_force_d.p, sizeof(double) * aux.size(), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    ZLH_TRY(hipStreamSynchronize(ctx->stream));
+    for (int64_t i = 0; i < L.n; ++i) {
+        const int32_t ai = L.node_aux_h[i];
+        for (int a = 0; a < 3; ++a) force[i * 3 + a] = ai >= 0 ? aux[(size_t)ai * 3 + a] : 0.;
+    }
     return ZLH_OK;
 }
 

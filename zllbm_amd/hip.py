@@ -46,6 +46,7 @@ EXPORTED = (
     "zlh_download_source_table", "zlh_ib_download_forces", "zlh_conservation", "zlh_level_info", "zlh_transfer_info", "zlh_level_streams_enable", "zlh_set_boundary_profile",
     "zlh_synchronize", "zlh_timing_enable", "zlh_timing_read", "zlh_mlp_set", "zlh_mlp_gradient",
     "zlh_set_boundary_velocities", "zlh_set_interpolation_method",
+    "zlh_download_node_forces", "zlh_transfer_from_coarse", "zlh_transfer_to_coarse",
     "zlh_mlp_train", "zlh_mlp_training_form", "zlh_mlp_eval", "zlh_closure_features", "zlh_graphs_enable",
     "zlh_max_level_bricks",
 )
@@ -75,11 +76,14 @@ def lib():
         L.zlh_upload_populations.argtypes = [P, C.c_int, D(np.float64)]
         L.zlh_step_level.argtypes = [P, C.c_int, C.c_int]
         L.zlh_transfer_levels.argtypes = [P, C.c_int]
+        L.zlh_transfer_from_coarse.argtypes = [P, C.c_int]
+        L.zlh_transfer_to_coarse.argtypes = [P, C.c_int]
         L.zlh_time_marching.argtypes = [P, C.c_int]
         L.zlh_download_populations.argtypes = [P, C.c_int, D(np.float64)]
         L.zlh_download_macros.argtypes = [P, C.c_int, D(np.float64), D(np.float64)]
         L.zlh_download_source_table.argtypes = [P, C.c_int, D(np.int32)]
         L.zlh_ib_download_forces.argtypes = [P, C.c_int, D(np.float64)]
+        L.zlh_download_node_forces.argtypes = [P, C.c_int, D(np.float64)]
         L.zlh_conservation.argtypes = [P, C.c_int, D(np.float64)]
         L.zlh_level_info.argtypes = [P, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.zlh_transfer_info.argtypes = [P, C.c_int, C.POINTER(C.c_int64)]
@@ -229,6 +233,14 @@ class Solver:
     def step_level(self, level, time_step_level):
         self._chk(lib().zlh_step_level(self.h, level, time_step_level))
 
+    def transfer_from_coarse(self, fine_level):
+        """Coarse-to-fine half of transfer_levels (TransferInfoFromCoarseGrid)."""
+        self._chk(lib().zlh_transfer_from_coarse(self.h, fine_level))
+
+    def transfer_to_coarse(self, fine_level):
+        """Fine-to-coarse half of transfer_levels (TransferInfoToCoarseGrid)."""
+        self._chk(lib().zlh_transfer_to_coarse(self.h, fine_level))
+
     def transfer_levels(self, fine_level):
         self._chk(lib().zlh_transfer_levels(self.h, fine_level))
 
@@ -271,6 +283,12 @@ class Solver:
         out = np.zeros(self._ib_n[level] * 3)
         self._chk(lib().zlh_ib_download_forces(self.h, level, out))
         return out.reshape(-1, 3)[:, :self.dims]
+
+    def node_forces(self, level=0) -> np.ndarray:
+        """Node forces [n][3] in key order (the immersed-boundary force on its supports)."""
+        out = np.zeros(self.n(level) * 3)
+        self._chk(lib().zlh_download_node_forces(self.h, level, out))
+        return out.reshape(-1, 3)
 
     def conservation(self, level=0) -> np.ndarray:
         out = np.zeros(4)

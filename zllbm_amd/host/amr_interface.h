@@ -41,7 +41,8 @@ enum class EDomainBoundaryDirection {
     kUndefined = 0, kBoundaryXMin = 1, kBoundaryXMax = 2, kBoundaryYMin = 3, kBoundaryYMax = 4,
     kBoundaryZMin = 5, kBoundaryZMax = 6
 };
-enum class ECriterionType { kGeometry = 0 };
+// ECriterionType (criterion/criterion_numerates.h:15-18); its value is written into checkpoints
+enum class ECriterionType : int { kUndefined = 0, kGeometry = 1 };
 
 struct NodeBitStatus {
     inline static DefInt kNodeStatus0_ = 0;
@@ -68,6 +69,15 @@ class SolverInterface;
 struct GridNode {
     DefInt flag_status_ = 0;
     virtual ~GridNode() {}
+};
+
+// one output variable of a grid's nodes (io/vtk_writer_manager.h, OutputNodeVariableInfoInterface):
+// its name and component count
+class OutputNodeVariableInfoInterface {
+ public:
+    std::string output_name_;
+    DefInt variable_dims_ = 1;
+    virtual ~OutputNodeVariableInfoInterface() {}
 };
 
 class InterfaceLayerInfo {
@@ -102,6 +112,21 @@ class GridInfoInterface {
     virtual void SetNodeVariablesAsZeros(GridNode *const ptr_node) {}
     virtual void InitialNotComputeNodeFlag() {}
     virtual void SetUpGridAtBeginningOfTimeStep(const DefInt time_step) {}
+    // communication between grids of different refinement levels (grid_info_interface.h:296-299)
+    virtual int TransferInfoFromCoarseGrid(const SFBitsetAuxInterface &sfbitset_aux, const DefInt node_flag,
+                                           const GridInfoInterface &grid_info_coarse) { return -1; }
+    virtual int TransferInfoToCoarseGrid(const SFBitsetAuxInterface &sfbitset_aux, const DefInt node_flag,
+                                         GridInfoInterface *const ptr_grid_info_coarse) { return -1; }
+    // node information across levels (grid_info_interface.h:325-326)
+    virtual void NodeInfoCoarse2fine(const GridNode &coarse_node, GridNode *const ptr_fine_node) const {}
+    virtual void NodeInfoFine2Coarse(const GridNode &fine_node, GridNode *const ptr_coarse_node) const {}
+    // node record sizes (grid_info_interface.h:376-377)
+    virtual int GetSizeOfGridNodeInfoForMpiCommunication() const { return sizeof(DefInt); }
+    virtual int GetSizeOfGridNodeInfoForCheckPoint() const { return GetSizeOfGridNodeInfoForMpiCommunication(); }
+    // output variables (grid_info_interface.h:278-281)
+    std::vector<std::string> output_variable_name_;
+    std::vector<std::unique_ptr<OutputNodeVariableInfoInterface>> output_variables_;
+    virtual void SetupOutputVariables() {}
 
     std::map<std::pair<ECriterionType, DefInt>, std::shared_ptr<InterfaceLayerInfo>> map_ptr_interface_layer_info_;
     DefMap<std::unique_ptr<GridNode>> map_grid_node_{};

@@ -9,6 +9,8 @@
 //   parameters: "key = value" lines (viscosity, rho, velocity, collision, les, cs, les_tau_mode,
 //               precision = <fp64|fp32|bf16>, closure = <file>, mlp_operands = <fp16|bf16>,
 //               checkpoint_in = <file> (start from it), checkpoint_out = <file> (write it at the end),
+//               checkpoint_dir_in / checkpoint_dir_out = <folder> (the same with the reference's
+//               checkpoint folder: loadings, grid_nodes, interface_nodes),
 //               vtu_out = <prefix> (VTU files of every level at the end),
 //               bc.xmin ... bc.zmax = <bounce_back|periodic|nee> [u v w])
 //   closure file (les = learned): whitespace-separated numbers, the 1409 network parameters,
@@ -158,7 +160,7 @@ int main(int argc, char **argv) {
                 lbm->SetDomainBoundaryCondition(dir_of(key.substr(3)), s, v);
             }
         }
-        std::string checkpoint_in, checkpoint_out, vtu_out;
+        std::string checkpoint_in, checkpoint_out, checkpoint_dir_in, checkpoint_dir_out, vtu_out;
         {
             std::ifstream pin2(argv[2]);
             while (std::getline(pin2, line)) {
@@ -167,16 +169,20 @@ int main(int argc, char **argv) {
                 if (ls >> key >> eq >> val && eq == "=") {
                     if (key == "checkpoint_in") checkpoint_in = val;
                     if (key == "checkpoint_out") checkpoint_out = val;
+                    if (key == "checkpoint_dir_in") checkpoint_dir_in = val;
+                    if (key == "checkpoint_dir_out") checkpoint_dir_out = val;
                     if (key == "vtu_out") vtu_out = val;
                 }
             }
         }
         solver->SolverInitial();
         if (!checkpoint_in.empty()) lbm->ReadCheckPointGridNodes(checkpoint_in);
+        if (!checkpoint_dir_in.empty()) lbm->ReadCheckPoint(checkpoint_dir_in);
         std::cout << solver->GetSolverMethod() << std::endl;
         const int steps = std::atoi(argv[3]);
         for (int t = 0; t < steps; ++t) TimeMarching(&gm, (DefAmrLUint)t);
         if (!checkpoint_out.empty()) lbm->WriteCheckPointGridNodes(checkpoint_out);
+        if (!checkpoint_dir_out.empty()) lbm->WriteCheckPoint(checkpoint_dir_out);
         if (!vtu_out.empty()) lbm->WriteVtu(vtu_out);
         std::ofstream out(argv[4], std::ios::binary);
         for (int l = 0; l < nlev; ++l) {

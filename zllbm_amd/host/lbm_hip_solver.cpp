@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
 #include <tuple>
 
@@ -69,6 +70,140 @@ std::unique_ptr<amrproject::GridNode> GridInfoLbmHip::GridNodeCreator() const {
         if (s) n->rho_ = s->GetDefaultDensity();
     }
     return n;
+}
+
+namespace {
+// D2Q9 / D3Q19 directions in the reference's order (lbm_d2q9.h:53-61, lbm_d3q19.h:73-85) and
+// their weights
+struct HostLattice {
+    int q;
+    int c[19][3];
+    DefReal w[19];
+};
+HostLattice host_lattice(int dims) {
+    HostLattice L{};
+    if (dims == 2) {
+        const int c[9][2] = {{0, 0}, {1, 0}, {0, 1}, {-1, 0}, {0, -1}, {1, 1}, {-1, 1}, {-1, -1}, {1, -1}};
+        L.q = 9;
+        for (int i = 0; i < 9; ++i) {
+            L.c[i][0] = c[i][0];
+            L.c[i][1] = c[i][1];
+            const int c2 = c[i][0] * c[i][0] + c[i][1] * c[i][1];
+            L.w[i] = c2 == 0 ? 4. / 9. : (c2 == 1 ? 1. / 9. : 1. / 36.);
+        }
+    } else {
+        const int c[19][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1},
+                              {1, 1, 0}, {-1, 1, 0}, {1, -1, 0}, {-1, -1, 0}, {1, 0, 1}, {-1, 0, 1}, {1, 0, -1},
+                              {-1, 0, -1}, {0, 1, 1}, {0, -1, 1}, {0, 1, -1}, {0, -1, -1}};
+        L.q = 19;
+        for (int i = 0; i < 19; ++i) {
+            for (int a = 0; a < 3; ++a) L.c[i][a] = c[i][a];
+            const int c2 = c[i][0] * c[i][0] + c[i][1] * c[i][1] + c[i][2] * c[i][2];
+            L.w[i] = c2 == 0 ? 1. / 3. : (c2 == 1 ? 1. / 18. : 1. / 36.);
+        }
+    }
+    return L;
+}
+
+// compressible equilibrium (CalFeq2D/3DCompressible, lbm_solver_interface.cpp:374-435)
+void host_feq(const HostLattice &L, DefReal rho, const std::vector<DefReal> &u, std::vector<DefReal> *feq) {
+    feq->resize(L.q);
+    const DefReal usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    for (int q = 0; q < L.q; ++q) {
+        const DefReal cu = L.c[q][0] * u[0] + L.c[q][1] * u[1] + L.c[q][2] * u[2];
+        (*feq)[q] = L.w[q] * rho * (1. + 3. * cu + 4.5 * cu * cu - 1.5 * usq);
+    }
+}
+
+// equilibrium part plus the scaled non-equilibrium part of a source node; the macros from the
+// source's populations without force (CalMacro2D/3DCompressible, lbm_solver_interface.cpp:477-530)
+// or its own rho_, velocity_ (those of its last collision)
+void rescaled_populations(int dims, const GridNodeLbmHip &src, DefReal ratio, GridNodeLbmHip *dst, bool macros_from_f) {
+    const HostLattice L = host_lattice(dims);
+    if ((int)src.f_.size() != L.q) throw ZlhError("node information: the source node has no populations");
+    std::vector<DefReal> feq, u(3, 0.);
+    DefReal rho = 0.;
+    if (macros_from_f) {
+        for (int q = 0; q < L.q; ++q) {
+            rho += src.f_[q];
+            for (int a = 0; a < 3; ++a) u[a] += L.c[q][a] * src.f_[q];
+        }
+        for (int a = 0; a < 3; ++a) u[a] /= rho;
+    } else {
+        rho = src.rho_;
+        for (int a = 0; a < dims && a < (int)src.velocity_.size(); ++a) u[a] = src.velocity_[a];
+    }
+    host_feq(L, rho, u, &feq);
+    u.resize(dims);
+    dst->rho_ = rho;
+    dst->velocity_ = u;
+    dst->f_.resize(feq.size());
+    for (size_t q = 0; q < feq.size(); ++q) dst->f_[q] = feq[q] + ratio * (src.f_[q] - feq[q]);
+}
+
+// relaxation time of a level (CalRelaxationTime, lbm_collision_models.cpp:105-120): the level's
+// viscosity nu0 / dt in lattice units of that level, tau = 3 nu / dt + 1/2
+DefReal level_tau(DefReal nu0, DefInt level) { return 3. * nu0 * (DefReal)(1u << level) + 0.5; }
+}  // namespace
+
+int GridInfoLbmHip::TransferInfoFromCoarseGrid(const amrproject::SFBitsetAuxInterface &, const DefInt,
+                                               const amrproject::GridInfoInterface &) {
+    SolverLbmHip *s = solver_of(*this);
+    if (!s->Context()) s->BuildDevice();
+    const zlh_status st = zlh_transfer_from_coarse(s->Context(), i_level_);
+    if (st != ZLH_OK) throw ZlhError(std::string("zlh_transfer_from_coarse: ") + zlh_last_error(s->Context()));
+    return 0;
+}
+
+int GridInfoLbmHip::TransferInfoToCoarseGrid(const amrproject::SFBitsetAuxInterface &, const DefInt,
+                                             amrproject::GridInfoInterface *const) {
+    SolverLbmHip *s = solver_of(*this);
+    if (!s->Context()) s->BuildDevice();
+    const zlh_status st = zlh_transfer_to_coarse(s->Context(), i_level_);
+    if (st != ZLH_OK) throw ZlhError(std::string("zlh_transfer_to_coarse: ") + zlh_last_error(s->Context()));
+    return 0;
+}
+
+// called on the coarse grid's information (the reference's coarse_grid_info.NodeInfoCoarse2fine):
+// macros from the coarse populations, ratio tau_f / (2 tau_c) of this (coarse) level
+void GridInfoLbmHip::NodeInfoCoarse2fine(const amrproject::GridNode &coarse_node, amrproject::GridNode *const ptr_fine_node) const {
+    const auto &c = dynamic_cast<const GridNodeLbmHip &>(coarse_node);
+    auto *f = dynamic_cast<GridNodeLbmHip *>(ptr_fine_node);
+    if (!f) throw ZlhError("NodeInfoCoarse2fine: the fine node is not an LBM node");
+    const DefReal nu = solver_of(*this)->GetDefaultViscosity();
+    rescaled_populations(dims_, c, 0.5 * level_tau(nu, i_level_ + 1) / level_tau(nu, i_level_), f, true);
+}
+
+// called on the fine grid's information: rho_, velocity_ of the fine node's last collision, ratio
+// 2 tau_c / tau_f of this (fine) level
+void GridInfoLbmHip::NodeInfoFine2Coarse(const amrproject::GridNode &fine_node, amrproject::GridNode *const ptr_coarse_node) const {
+    const auto &f = dynamic_cast<const GridNodeLbmHip &>(fine_node);
+    auto *c = dynamic_cast<GridNodeLbmHip *>(ptr_coarse_node);
+    if (!c) throw ZlhError("NodeInfoFine2Coarse: the coarse node is not an LBM node");
+    const DefReal nu = solver_of(*this)->GetDefaultViscosity();
+    rescaled_populations(dims_, f, 2. * level_tau(nu, i_level_ - 1) / level_tau(nu, i_level_), c, false);
+}
+
+int GridInfoLbmHip::GetSizeOfGridNodeInfoForCheckPoint() const {
+    const SolverLbmHip *s = solver_of(*this);
+    return (int)(sizeof(DefInt) + sizeof(DefReal) * (s->k0NumQ_ + s->GetNumForces()));
+}
+
+int GridInfoLbmHip::GetSizeOfGridNodeInfoForMpiCommunication() const {
+    const SolverLbmHip *s = solver_of(*this);
+    return (int)(sizeof(DefReal) * (s->k0NumQ_ + 1 + s->GetSolverDim() + s->GetNumForces()));
+}
+
+void GridInfoLbmHip::SetupOutputVariables() {
+    const SolverLbmHip *s = solver_of(*this);
+    output_variables_.clear();
+    for (const std::string &name : output_variable_name_) {
+        if (name != "rho" && name != "velocity") continue;
+        auto v = std::make_unique<amrproject::OutputNodeVariableInfoInterface>();
+        v->output_name_ = name;
+        v->variable_dims_ = name == "rho" ? 1 : s->GetSolverDim();
+        output_variables_.push_back(std::move(v));
+    }
 }
 
 void GridInfoLbmHip::SetNodeVariablesAsZeros(amrproject::GridNode *const ptr_node) {
@@ -278,11 +413,20 @@ void SolverLbmHip::RunSolverOnGivenGrid(const amrproject::ETimeSteppingScheme, c
 }
 
 // SolverLbmInterface::InformationFromGridOfDifferentLevel (lbm_solver_interface.cpp:224-238)
+// SolverLbmInterface::InformationFromGridOfDifferentLevel (lbm_solver_interface.cpp:224-238): after
+// the fine level's even substeps, the fine grid's TransferInfoFromCoarseGrid, then its
+// TransferInfoToCoarseGrid, with the coarse grid's information
 void SolverLbmHip::InformationFromGridOfDifferentLevel(const DefInt time_step_level,
-                                                       const amrproject::SFBitsetAuxInterface &,
+                                                       const amrproject::SFBitsetAuxInterface &sfbitset_aux,
                                                        amrproject::GridInfoInterface *const ptr_grid_info) {
     const DefInt i_level = ptr_grid_info->GetGridLevel();
-    if (i_level > 0 && time_step_level % 2 == 0) Check(zlh_transfer_levels(ctx_, i_level), "zlh_transfer_levels");
+    if (i_level > 0 && time_step_level % 2 == 0) {
+        amrproject::GridInfoInterface *coarse = ptr_grid_manager_->vec_ptr_grid_info_.at(i_level - 1).get();
+        ptr_grid_info->TransferInfoFromCoarseGrid(sfbitset_aux, amrproject::NodeBitStatus::kNodeStatusCoarse2FineGhost_,
+                                                  *coarse);
+        ptr_grid_info->TransferInfoToCoarseGrid(sfbitset_aux, amrproject::NodeBitStatus::kNodeStatusCoarse2FineGhost_,
+                                                coarse);
+    }
 }
 
 void SolverLbmHip::CopyPopulationsToHost(DefInt level) {
@@ -328,7 +472,18 @@ void SolverLbmHip::WriteCheckPointGridNodes(const std::string &file_name) {
     const uint32_t count = (uint32_t)order.size();   // DefAmrLUint
     out.write(reinterpret_cast<const char *>(&count), sizeof(count));
     out.write(reinterpret_cast<const char *>(&payload), sizeof(payload));
-    const std::vector<double> zeros(nforce, 0.);
+    // node forces of every level in key order (GridNodeLbm::force_)
+    std::vector<std::vector<uint64_t>> level_keys(gm.k0MaxLevel_ + 1);
+    std::vector<std::vector<double>> level_force(gm.k0MaxLevel_ + 1);
+    if (nforce) {
+        if (nforce > 3) throw ZlhError("checkpoint: more than 3 node forces");
+        for (DefInt l = 0; l <= gm.k0MaxLevel_; ++l) {
+            for (auto &kv : gm.vec_ptr_grid_info_.at(l)->map_grid_node_) level_keys[l].push_back(kv.first.to_ullong());
+            std::sort(level_keys[l].begin(), level_keys[l].end());
+            level_force[l].resize(level_keys[l].size() * 3);
+            Check(zlh_download_node_forces(ctx_, l, level_force[l].data()), "zlh_download_node_forces");
+        }
+    }
     for (const CheckpointOrder &r : order) {
         const auto *ptr = dynamic_cast<const GridNodeLbmHip *>(
             gm.vec_ptr_grid_info_.at(r.level)->map_grid_node_.at(DefSFBitset(r.key)).get());
@@ -339,9 +494,163 @@ void SolverLbmHip::WriteCheckPointGridNodes(const std::string &file_name) {
         out.write(reinterpret_cast<const char *>(&r.level), sizeof(int32_t));
         out.write(reinterpret_cast<const char *>(&flag), sizeof(int32_t));
         out.write(reinterpret_cast<const char *>(node.f_.data()), sizeof(double) * k0NumQ_);
-        if (nforce) out.write(reinterpret_cast<const char *>(zeros.data()), sizeof(double) * nforce);
+        if (nforce) {
+            const auto &ks = level_keys[r.level];
+            const size_t i = (size_t)(std::lower_bound(ks.begin(), ks.end(), r.key) - ks.begin());
+            out.write(reinterpret_cast<const char *>(level_force[r.level].data() + 3 * i), sizeof(double) * nforce);
+        }
     }
     if (!out) throw ZlhError("checkpoint " + file_name + ": write failed");
+}
+
+namespace {
+template <typename V>
+void put(std::vector<char> &b, const V &v) {
+    const char *p = reinterpret_cast<const char *>(&v);
+    b.insert(b.end(), p, p + sizeof(V));
+}
+
+// the nodes of a level inside one background cell in the grid_nodes record order (z, y, x)
+std::vector<uint64_t> cell_keys(int dims, uint64_t cell, int level) {
+    const uint32_t n = 1u << level;
+    const uint32_t base[3] = {morton_axis(cell, dims, 0) << level, morton_axis(cell, dims, 1) << level,
+                              dims == 3 ? morton_axis(cell, dims, 2) << level : 0u};
+    std::vector<uint64_t> out;
+    for (uint32_t z = 0; z < (dims == 3 ? n : 1u); ++z)
+        for (uint32_t y = 0; y < n; ++y)
+            for (uint32_t x = 0; x < n; ++x) out.push_back(morton_key(dims, base[0] + x, base[1] + y, base[2] + z));
+    return out;
+}
+
+// the four layer kinds of an interface in the order the reference writes them, with their
+// indicators (kIndictor*, mpi/mpi_manager.h:615-616)
+std::vector<std::pair<int32_t, const std::vector<DefMap<DefInt>> *>> layer_kinds(const amrproject::InterfaceLayerInfo &li) {
+    return {{1, &li.vec_inner_fine2coarse_}, {2, &li.vec_outer_fine2coarse_},
+            {3, &li.vec_inner_coarse2fine_}, {4, &li.vec_outer_coarse2fine_}};
+}
+
+struct CellLoad {
+    std::vector<uint8_t> levels;
+    int64_t nodes = 0, interface_nodes = 0, load = 0;
+};
+
+std::map<uint64_t, CellLoad> cell_loads(const amrproject::GridManagerInterface &gm, int dims) {
+    std::map<uint64_t, CellLoad> cells;
+    const size_t nbytes = (size_t)(gm.k0MaxLevel_ + 1 + 7) / 8;
+    for (DefInt l = 0; l <= gm.k0MaxLevel_; ++l) {
+        const auto &gi = *gm.vec_ptr_grid_info_.at(l);
+        for (const auto &kv : gi.map_grid_node_) {
+            CellLoad &c = cells[kv.first.to_ullong() >> (dims * l)];
+            if (c.levels.empty()) c.levels.assign(nbytes, 0);
+            c.levels[l / 8] |= (uint8_t)(1u << (l % 8));
+            ++c.nodes;
+            c.load += (int64_t)1 << l;
+        }
+        for (const auto &it : gi.map_ptr_interface_layer_info_)
+            for (const auto &kind : layer_kinds(*it.second))
+                for (const auto &layer : *kind.second)
+                    for (const auto &kv : layer) {
+                        if (!gi.map_grid_node_.count(kv.first))
+                            throw ZlhError("checkpoint: an interface node is not a grid node");
+                        ++cells.at(kv.first.to_ullong() >> (dims * l)).interface_nodes;
+                    }
+    }
+    return cells;
+}
+
+std::vector<char> read_file(const std::string &name) {
+    std::ifstream in(name, std::ios::binary);
+    if (!in) throw ZlhError("cannot read checkpoint " + name);
+    return std::vector<char>((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+}
+
+void write_file(const std::string &name, const std::vector<char> &b) {
+    std::ofstream out(name, std::ios::binary);
+    if (!out) throw ZlhError("cannot write checkpoint " + name);
+    out.write(b.data(), (std::streamsize)b.size());
+    if (!out) throw ZlhError("checkpoint " + name + ": write failed");
+}
+}  // namespace
+
+std::vector<char> SolverLbmHip::CheckPointLoadings() const {
+    const amrproject::GridManagerInterface &gm = *ptr_grid_manager_;
+    const auto cells = cell_loads(gm, k0SolverDims_);
+    int64_t total = 0;
+    for (const auto &c : cells) total += c.second.load;
+    if (total > 0xffffffffll || cells.size() > 0xffffffffull)
+        throw ZlhError("checkpoint: load or cell count beyond the loadings header's uint32");
+    std::vector<char> b;
+    put(b, (uint32_t)total);
+    put(b, (uint32_t)cells.size());
+    put(b, (int32_t)gm.k0MaxLevel_);
+    for (const auto &c : cells) {
+        if (c.second.nodes > INT32_MAX || c.second.interface_nodes > INT32_MAX || c.second.load > INT32_MAX)
+            throw ZlhError("checkpoint: a cell's counts exceed int32");
+        put(b, c.first);
+        b.insert(b.end(), c.second.levels.begin(), c.second.levels.end());
+        put(b, (int32_t)c.second.nodes);
+        put(b, (int32_t)c.second.interface_nodes);
+        put(b, (int32_t)c.second.load);
+    }
+    return b;
+}
+
+std::vector<char> SolverLbmHip::CheckPointInterfaceNodes() const {
+    const amrproject::GridManagerInterface &gm = *ptr_grid_manager_;
+    const int dims = k0SolverDims_;
+    // index table: every layer of every interface of every level, in order
+    std::map<std::tuple<int32_t, int32_t, int32_t, int32_t, int32_t>, int32_t> index;
+    std::vector<char> table;
+    int32_t nidx = 0;
+    for (DefInt l = 0; l <= gm.k0MaxLevel_; ++l)
+        for (const auto &it : gm.vec_ptr_grid_info_.at(l)->map_ptr_interface_layer_info_)
+            for (const auto &kind : layer_kinds(*it.second))
+                for (int32_t layer = 0; layer < (int32_t)kind.second->size(); ++layer) {
+                    const auto entry = std::make_tuple((int32_t)l, (int32_t)it.first.first, (int32_t)it.first.second,
+                                                       kind.first, layer);
+                    index[entry] = nidx++;
+                    put(table, std::get<0>(entry));
+                    put(table, std::get<1>(entry));
+                    put(table, std::get<2>(entry));
+                    put(table, std::get<3>(entry));
+                    put(table, std::get<4>(entry));
+                }
+    std::vector<char> b;
+    put(b, nidx);
+    b.insert(b.end(), table.begin(), table.end());
+    for (const auto &c : cell_loads(gm, dims))
+        for (DefInt l = 0; l <= gm.k0MaxLevel_; ++l) {
+            if (!(c.second.levels[l / 8] >> (l % 8) & 1u)) continue;
+            const auto &gi = *gm.vec_ptr_grid_info_.at(l);
+            for (uint64_t key : cell_keys(dims, c.first, l))
+                for (const auto &it : gi.map_ptr_interface_layer_info_)
+                    for (const auto &kind : layer_kinds(*it.second))
+                        for (int32_t layer = 0; layer < (int32_t)kind.second->size(); ++layer)
+                            if (kind.second->at(layer).count(DefSFBitset(key))) {
+                                put(b, key);
+                                put(b, index.at(std::make_tuple((int32_t)l, (int32_t)it.first.first,
+                                                                (int32_t)it.first.second, kind.first, layer)));
+                                break;
+                            }
+        }
+    return b;
+}
+
+void SolverLbmHip::WriteCheckPoint(const std::string &folder) {
+    std::error_code ec;
+    std::filesystem::create_directories(folder, ec);
+    if (ec) throw ZlhError("cannot create checkpoint folder " + folder + ": " + ec.message());
+    write_file(folder + "/loadings", CheckPointLoadings());
+    WriteCheckPointGridNodes(folder + "/grid_nodes");
+    write_file(folder + "/interface_nodes", CheckPointInterfaceNodes());
+}
+
+void SolverLbmHip::ReadCheckPoint(const std::string &folder) {
+    if (read_file(folder + "/loadings") != CheckPointLoadings())
+        throw ZlhError("checkpoint " + folder + ": loadings do not describe this grid");
+    if (read_file(folder + "/interface_nodes") != CheckPointInterfaceNodes())
+        throw ZlhError("checkpoint " + folder + ": interface_nodes do not describe this grid's interfaces");
+    ReadCheckPointGridNodes(folder + "/grid_nodes");
 }
 
 void SolverLbmHip::ReadCheckPointGridNodes(const std::string &file_name) {

@@ -53,6 +53,25 @@ class GridInfoLbmHip : public amrproject::GridInfoInterface {
     std::unique_ptr<amrproject::GridNode> GridNodeCreator() const override;
     void SetNodeVariablesAsZeros(amrproject::GridNode *const ptr_node) override;
     void InitialGridInfo(const DefInt dims) override { dims_ = dims; }
+    // TransferInfoFromCoarseGrid / TransferInfoToCoarseGrid (lbm_grid_interface.cpp:408-564) on
+    // the device: zlh_transfer_from_coarse / zlh_transfer_to_coarse of this (fine) level; 0 on
+    // success as in the reference
+    int TransferInfoFromCoarseGrid(const amrproject::SFBitsetAuxInterface &sfbitset_aux, const DefInt node_flag,
+                                   const amrproject::GridInfoInterface &grid_info_coarse) override;
+    int TransferInfoToCoarseGrid(const amrproject::SFBitsetAuxInterface &sfbitset_aux, const DefInt node_flag,
+                                 amrproject::GridInfoInterface *const ptr_grid_info_coarse) override;
+    // NodeInfoCoarse2fine / NodeInfoFine2Coarse (lbm_grid_interface.cpp:570-610) on the host
+    // mirrors (GridNodeLbmHip): feq of the source's macros plus the tau-ratio-scaled
+    // non-equilibrium part (PostStreamCoarse2Fine / PostStreamFine2Coarse,
+    // lbm_collision_models.cpp:244-267); the device kernels do the same for the time step
+    void NodeInfoCoarse2fine(const amrproject::GridNode &coarse_node, amrproject::GridNode *const ptr_fine_node) const override;
+    void NodeInfoFine2Coarse(const amrproject::GridNode &fine_node, amrproject::GridNode *const ptr_coarse_node) const override;
+    // node record sizes (lbm_grid_interface.cpp:737-749, lbm_grid_interface_mpi.cpp:77-90)
+    int GetSizeOfGridNodeInfoForMpiCommunication() const override;
+    int GetSizeOfGridNodeInfoForCheckPoint() const override;
+    // the names in output_variable_name_ ("rho", "velocity") as output variables
+    // (SetupOutputVariables, lbm_grid_interface.cpp:614-660); WriteVtu writes them
+    void SetupOutputVariables() override;
 
  private:
     DefInt dims_ = 0;
@@ -77,6 +96,8 @@ class SolverLbmHip : public amrproject::SolverInterface {
     void SetConstantForce(const std::vector<DefReal> &force) { bool_forces_ = true; k0ConstForce_ = force; }
     void SetNumForces(const DefInt n) { bool_forces_ = true; k0NumForces_ = n; }
     DefReal GetDefaultViscosity() const { return k0LbmViscosity_; }
+    // GetNumForces (lbm_interface.h:308-314): the node force count when forces are on
+    DefInt GetNumForces() const { return bool_forces_ ? k0NumForces_ : 0; }
     DefReal GetDefaultDensity() const { return k0Rho_; }
     // LES (SetLesModel, les_models/lbm_les_models.cpp:17-41) with the Smagorinsky constant and the
     // relaxation-time mode exposed
@@ -133,9 +154,31 @@ class SolverLbmHip : public amrproject::SolverInterface {
     // (lbm_interface.h:57-67): int32 flag, Q doubles of post-stream populations, and, with forces,
     // GetNumForces() doubles.  Records are ordered as the reference writes them: by background
     // cell (Morton code ascending), then level, then z, y, x inside the cell
-    // (SFBitsetHigherLevelInACell, grid/sfbitset_aux.cpp:140-156, 1958-1985).  The force block is
-    // written as zeros: this solver recomputes node forces from the populations every step.
+    // (SFBitsetHigherLevelInACell, grid/sfbitset_aux.cpp:140-156, 1958-1985).  The force block
+    // holds the node forces (the immersed-boundary force of the last forcing step on its
+    // supports, zero elsewhere).
     void WriteCheckPointGridNodes(const std::string &file_name);
+    // checkpoint folder for one rank, as IoManager::WriteCheckPointData writes it
+    // (io/io_checkpoint.cpp:27-51): <folder>/loadings, <folder>/grid_nodes and
+    // <folder>/interface_nodes
+    void WriteCheckPoint(const std::string &folder);
+    // restart from such a folder (IoManager::ReadCheckPointData, io/io_checkpoint.cpp:59-166, on
+    // a grid already set): its loadings and interface_nodes must be those of this grid, byte for
+    // byte, and grid_nodes restores the populations (ReadCheckPointGridNodes)
+    void ReadCheckPoint(const std::string &folder);
+    // the loadings file (ComputeComputationalLoadOnEachRank + WriteCheckPointNodesAtWhichLevels,
+    // mpi/mpi_checkpoint.cpp:39-190): header {uint32 total load, uint32 background cells, int32
+    // max level}, then per background cell that holds a node of any level, by Morton code:
+    // {uint64 code, level bit field of (max level + 8) / 8 bytes, int32 nodes, int32 interface-
+    // layer entries, int32 load}; a level-l node costs 2^l (grid_manager.cpp:146)
+    std::vector<char> CheckPointLoadings() const;
+    // the interface_nodes file (WriteCheckPointInterfaceNodes, mpi/mpi_checkpoint.cpp:1214-1410):
+    // int32 index count, the index table {int32 level, int32 criterion, int32 criterion count,
+    // int32 layer kind (1 inner fine-to-coarse, 2 outer fine-to-coarse, 3 inner coarse-to-fine,
+    // 4 outer coarse-to-fine), int32 layer} of every layer of every interface of every level,
+    // then per interface node {uint64 key, int32 index} in the grid_nodes order, one record per
+    // interface and layer kind naming the first layer that holds the node
+    std::vector<char> CheckPointInterfaceNodes() const;
     // ReadCheckPointGridNodes (mpi/mpi_checkpoint.cpp:392-520) for one rank: every record's flag
     // and populations go to the node mirrors (ReadANodeFromBufferForCheckpoint), and then to the
     // device (uploaded now if it is built, else by BuildDevice).  Every node of the grid must
