@@ -199,6 +199,18 @@ zlh_status zlh_mlp_set(zlh_ctx *ctx, const float *params, const float *mean, con
  * (normalised inputs [n][9]); grad is [1409], loss may be null */
 zlh_status zlh_mlp_gradient(zlh_ctx *ctx, const float *params, const float *x, const float *t, int64_t n,
                             float *grad, float *loss);
+/* zlh_mlp_gradient without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight training
+ * loops, harnesses that forbid hipMalloc).  The parameters, samples, targets and the gradient
+ * and loss accumulators are staged in one static device array of the library, 2.6 MiB per GPU,
+ * held until the process ends; it holds at most ZLH_MLP_NOALLOC_MAX_SAMPLES samples, and a
+ * larger n returns ZLH_EINVAL.  Same arguments, kernel (zlh_mlp_training_form) and results as
+ * zlh_mlp_gradient.  Every call on one GPU uses that GPU's one array, which keeps the last
+ * call's contents (each call clears its accumulators first): calls that overlap on one GPU,
+ * from several threads or contexts, race. */
+#define ZLH_MLP_NOALLOC_MAX_SAMPLES 65536
+zlh_status zlh_mlp_gradient_noalloc(zlh_ctx *ctx, const float *params, const float *x, const float *t,
+                                    int64_t n, float *grad, float *loss);
 /* Adam training on the device: `steps` minibatches of `batch` samples drawn with a seeded hash
  * (batch >= n: every step uses all n samples in order); params is updated in place;
  * loss_history (may be null) receives each minibatch loss */

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "zllbm_hip.h"
@@ -361,6 +362,34 @@ __global__ void k_closure_features(const double *fq, int64_t n, double tau0, dou
 }  // namespace zlh
 
 // ==============================================================================================
+// static device array of zlh_mlp_gradient_noalloc: the parameters, the samples and targets,
+// and the gradient and loss accumulators, at fixed float offsets (each a multiple of 64 floats,
+// so every region starts 256-byte aligned).  One copy per GPU, at its own address, for the life
+// of the process.  Host code never uses the array's name as an address: the per-device address
+// comes from hipGetSymbolAddress (scratch_setup).
+// ==============================================================================================
+namespace {
+constexpr int64_t kNoAllocMaxSamples = ZLH_MLP_NOALLOC_MAX_SAMPLES;
+constexpr int64_t kScratchParams = 0;                                     // [kMlpParams]
+constexpr int64_t kScratchGrad = 1536;                                    // [kMlpParams]
+constexpr int64_t kScratchLoss = 3072;                                    // [1]
+constexpr int64_t kScratchT = 3136;                                       // [max samples]
+constexpr int64_t kScratchX = kScratchT + kNoAllocMaxSamples;             // [max samples][kMlpF]
+constexpr int64_t kScratchFloats = kScratchX + kNoAllocMaxSamples * zlh::kMlpF;
+static_assert(zlh::kMlpParams <= kScratchGrad - kScratchParams && zlh::kMlpParams <= kScratchLoss - kScratchGrad,
+              "network parameters fit their regions");
+static_assert(kScratchT % 64 == 0 && kScratchX % 64 == 0, "regions are 256-byte aligned");
+}  // namespace
+
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+// clears the accumulator regions (a kernel, not a memset, so that the call stays capturable)
+__global__ void k_scratch_clear(float *p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+// ==============================================================================================
 // host context
 // ==============================================================================================
 using namespace zlh;
@@ -392,6 +421,8 @@ struct zlh_ctx {
     bool mlp_set = false;
     int mlp_train_form = ZLH_MLP_FP32;   // kernel of zlh_mlp_gradient / zlh_mlp_train
     int n_cu = 0;           // compute units of the context's device
+    int device = 0;         // the device the context was created on
+    float *scratch_addr = nullptr;   // that device's address of the static no-allocation array
     // two background steps captured as one graph, per combination of the levels' buffer
     // indices (two steps flip every level's buffers an even number of times, so replays chain)
     bool use_graphs = true;
@@ -425,6 +456,32 @@ struct zlh_ctx {
 };
 
 namespace {
+
+// device address of the static no-allocation array, looked up once per device with that device
+// current (each GPU holds its own copy of the array at its own address)
+constexpr int kScratchMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float *g_scratch_addr[kScratchMaxDevices] = {};
+
+zlh_status scratch_setup(zlh_ctx *ctx) {
+    if (ctx->device < 0 || ctx->device >= kScratchMaxDevices)
+        return ctx->fail(ZLH_EDEVICE, "device %d: no-allocation array supports devices 0..%d", ctx->device,
+                         kScratchMaxDevices - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_addr[ctx->device]) {
+        int cur = -1;
+        ZLH_TRY(hipGetDevice(&cur));
+        if (cur != ctx->device) ZLH_TRY(hipSetDevice(ctx->device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != ctx->device) ZLH_TRY(hipSetDevice(cur));
+        if (e != hipSuccess || !p)
+            return ctx->fail(ZLH_EDEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_addr[ctx->device] = static_cast<float *>(p);
+    }
+    ctx->scratch_addr = g_scratch_addr[ctx->device];
+    return ZLH_OK;
+}
 
 zlh_status dalloc(zlh_ctx *ctx, DevBuf &b, size_t bytes, bool zero = true) {
     if (b.p) { (void)hipFree(b.p); b.p = nullptr; }
@@ -1015,6 +1072,8 @@ zlh_status zlh_create(const zlh_solver_desc *desc, zlh_ctx **out) {
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ctx->n_cu < 1)
         return ZLH_EDEVICE;
+    ctx->device = dev;
+    if (scratch_setup(ctx.get())) return ZLH_EDEVICE;
     if (dalloc(ctx.get(), ctx->mlp_d, sizeof(MlpDev))) return ZLH_ENOMEM;
     *out = ctx.release();
     return ZLH_OK;
@@ -1667,6 +1726,31 @@ zlh_status zlh_mlp_gradient(zlh_ctx *ctx, const float *params, const float *x, c
     ZLH_TRY(hipGetLastError());
     ZLH_TRY(hipMemcpyAsync(grad, dg.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, ctx->stream));
     if (loss) ZLH_TRY(hipMemcpyAsync(loss, dl.p, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    ZLH_TRY(hipStreamSynchronize(ctx->stream));
+    return ZLH_OK;
+}
+
+zlh_status zlh_mlp_gradient_noalloc(zlh_ctx *ctx, const float *params, const float *x, const float *t, int64_t n,
+                                    float *grad, float *loss) {
+    if (!ctx) return ZLH_EINVAL;
+    if (!params || !x || !t || !grad || n < 1) return ctx->fail(ZLH_EINVAL, "bad gradient arguments");
+    if (n > kNoAllocMaxSamples)
+        return ctx->fail(ZLH_EINVAL, "zlh_mlp_gradient_noalloc holds at most %lld samples, got %lld",
+                         (long long)kNoAllocMaxSamples, (long long)n);
+    if (!ctx->scratch_addr) return ctx->fail(ZLH_ESTATE, "no-allocation array not set up");
+    float *dp = ctx->scratch_addr + kScratchParams, *dg = ctx->scratch_addr + kScratchGrad;
+    float *dl = ctx->scratch_addr + kScratchLoss, *dt = ctx->scratch_addr + kScratchT, *dx = ctx->scratch_addr + kScratchX;
+    // gradient and loss regions (and the gap between them) start at zero
+    const int64_t nclear = kScratchT - kScratchGrad;
+    hipLaunchKernelGGL(k_scratch_clear, dim3(blocks(nclear)), dim3(256), 0, ctx->stream, dg, nclear);
+    ZLH_TRY(hipGetLastError());
+    ZLH_TRY(hipMemcpyAsync(dp, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, ctx->stream));
+    ZLH_TRY(hipMemcpyAsync(dx, x, sizeof(float) * kMlpF * n, hipMemcpyHostToDevice, ctx->stream));
+    ZLH_TRY(hipMemcpyAsync(dt, t, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    launch_mlp_grad(ctx, dp, dx, dt, n, (int)n, 0u, 0, dg, dl);
+    ZLH_TRY(hipGetLastError());
+    ZLH_TRY(hipMemcpyAsync(grad, dg, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, ctx->stream));
+    if (loss) ZLH_TRY(hipMemcpyAsync(loss, dl, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     ZLH_TRY(hipStreamSynchronize(ctx->stream));
     return ZLH_OK;
 }

@@ -24,6 +24,7 @@ LES_TAU_REFERENCE, LES_TAU_PHYSICAL = 0, 1
 PROFILE_UNIFORM, PROFILE_PARABOLIC = 0, 1
 MLP_FP32, MLP_BF16_MFMA, MLP_FP16_MFMA = 0, 1, 2
 MLP_OPERANDS_BF16, MLP_OPERANDS_FP16 = 0, 1
+MLP_NOALLOC_MAX_SAMPLES = 65536   # ZLH_MLP_NOALLOC_MAX_SAMPLES
 INTERP_LAGRANGIAN, INTERP_LINEAR = 0, 1
 
 
@@ -45,7 +46,7 @@ EXPORTED = (
     "zlh_step_level", "zlh_transfer_levels", "zlh_time_marching", "zlh_download_populations", "zlh_download_macros",
     "zlh_download_source_table", "zlh_ib_download_forces", "zlh_conservation", "zlh_level_info", "zlh_transfer_info", "zlh_level_streams_enable", "zlh_set_boundary_profile",
     "zlh_synchronize", "zlh_timing_enable", "zlh_timing_read", "zlh_mlp_set", "zlh_mlp_gradient",
-    "zlh_set_boundary_velocities", "zlh_set_interpolation_method",
+    "zlh_mlp_gradient_noalloc", "zlh_set_boundary_velocities", "zlh_set_interpolation_method",
     "zlh_download_node_forces", "zlh_transfer_from_coarse", "zlh_transfer_to_coarse",
     "zlh_mlp_train", "zlh_mlp_training_form", "zlh_mlp_eval", "zlh_closure_features", "zlh_graphs_enable",
     "zlh_max_level_bricks",
@@ -93,6 +94,7 @@ def lib():
         F32 = D(np.float32)
         L.zlh_mlp_set.argtypes = [P, F32, F32, F32, F32, F32, C.c_double]
         L.zlh_mlp_gradient.argtypes = [P, F32, F32, F32, C.c_int64, F32, F32]
+        L.zlh_mlp_gradient_noalloc.argtypes = [P, F32, F32, F32, C.c_int64, F32, F32]
         L.zlh_mlp_train.argtypes = [P, F32, F32, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_uint64, F32, F32]
         L.zlh_mlp_training_form.argtypes = [P, C.c_int]
         L.zlh_mlp_eval.argtypes = [P, F32, C.c_int64, C.c_int, F32]
@@ -309,6 +311,18 @@ class Solver:
         loss = np.zeros(1, np.float32)
         self._chk(lib().zlh_mlp_training_form(self.h, form))
         self._chk(lib().zlh_mlp_gradient(self.h, f(params), f(x), f(t), len(t), g, loss))
+        return float(loss[0]), g
+
+    def mlp_gradient_noalloc(self, params, x, t, form=MLP_FP32):
+        """mlp_gradient through zlh_mlp_gradient_noalloc: no device allocation, at most
+        MLP_NOALLOC_MAX_SAMPLES samples."""
+        t = np.ascontiguousarray(t, np.float32).ravel()
+        params = _checked(params, np.float32, 1409, "params")
+        x = _checked(x, np.float32, 9 * len(t), "x")
+        g = np.zeros(len(params), np.float32)
+        loss = np.zeros(1, np.float32)
+        self._chk(lib().zlh_mlp_training_form(self.h, form))
+        self._chk(lib().zlh_mlp_gradient_noalloc(self.h, params, x, t, len(t), g, loss))
         return float(loss[0]), g
 
     def mlp_train(self, params, x, t, steps, batch, lr, seed=20261202, form=MLP_FP32):
