@@ -53,9 +53,9 @@ PROFILED_TRAFFIC = {
 }
 
 
-def make_solver(hip, W, precision, les, model=None):
+def make_solver(hip, W, precision, les, model=None, **extra):
     kw = dict(W.solver)
-    kw.update(les=les)
+    kw.update(les=les, **extra)
     if les == hip.LES_MLP:
         kw.update(mlp_operands=hip.MLP_OPERANDS_FP16 if MLP_OPERANDS == "fp16" else hip.MLP_OPERANDS_BF16)
     s = hip.Solver(W.grid, precision=precision, **kw)
@@ -253,6 +253,12 @@ def main():
         wall32, _, _, _ = timed(s32, W, 5, c_steps, kernel_steps=0)
         del s32
         log("FP32 Smagorinsky: %.3f ms per step" % (wall32 / c_steps * 1e3))
+        # the same learned-closure step with the MRT transform on the matrix cores
+        sx = make_solver(hip, W, hip.FP32_SHIFTED, les, model, moment_transform=hip.TRANSFORM_MFMA)
+        wallx, kx_ms, kx_n, _ = timed(sx, W, 5, c_steps)
+        del sx
+        log("FP32 learned, MRT transform on the matrix cores: %.3f ms per step, finest-level kernel %.4f ms" % (
+            wallx / c_steps * 1e3, kx_ms / max(kx_n, 1)))
         # accuracy: FP32 against FP64, same closure, in lockstep (the learned closure, and for
         # reference the plain Smagorinsky pair, whose difference is FP32 rounding alone)
         curve = l2_curve(hip, W, hip.LES_MLP, model, args.l2_steps, args.l2_every)
@@ -263,6 +269,11 @@ def main():
                              "fp64_smagorinsky_ms_per_step": round(wall64 / c_steps * 1e3, 4),
                              "speedup_fp32_learned_vs_fp64_smagorinsky": round(mlups / mlups64, 3),
                              "fp32_smagorinsky_mlups": round(W.updates_per_step * c_steps / wall32 / 1e6, 2),
+                             "mrt_transform_matrix_cores": {
+                                 "ms_per_step": round(wallx / c_steps * 1e3, 4),
+                                 "finest_kernel_ms": round(kx_ms / max(kx_n, 1), 4),
+                                 "vector_units_finest_kernel_ms": round(avg_s * 1e3, 4),
+                                 "kernel": "k_stream_collide_mlp (level %d), v_mfma_f32_32x32x2f32 transform" % finest},
                              "l2_u_vs_fp64_mean": float(np.mean([c[1] for c in curve])),
                              "l2_u_vs_fp64_final": curve[-1][1], "l2_rho_vs_fp64_final": curve[-1][2],
                              "l2_u_curve": [[t, round(u, 8)] for t, u, _ in curve],

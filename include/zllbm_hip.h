@@ -114,6 +114,25 @@ zlh_status zlh_set_boundary_velocities(zlh_ctx *ctx, int level, int dir, const d
 #define ZLH_INTERP_LAGRANGIAN 0
 #define ZLH_INTERP_LINEAR 1
 zlh_status zlh_set_interpolation_method(zlh_ctx *ctx, int level, int method);
+/* time interpolation at coarse-fine interfaces (no reference counterpart; 0, the default, is the
+ * reference's schedule of transfers after even fine substeps only, lbm_solver_interface.cpp:
+ * 224-238).  1: the ghost layers of every refined level are also refilled after its odd
+ * substeps, each coarse contributor giving the mean of its transferred values before and after
+ * the coarse level's last step (a linear interpolation in time), each fine fallback contributor
+ * its current values; the even-substep transfers are unchanged.  zlh_transfer_levels and
+ * zlh_transfer_from_coarse after an odd substep then do that refill (zlh_transfer_to_coarse there
+ * stays ZLH_ESTATE).  Any time, taking effect from the next substep; set it between background
+ * steps, since the earlier coarse state is taken from the state at the call (and at each upload). */
+zlh_status zlh_set_time_interpolation(zlh_ctx *ctx, int enable);
+/* where the FP32 learned-closure kernels run the MRT moment transform m = M f and its inverse
+ * (no reference counterpart; the reference's MRT collision multiplies by dense matrices,
+ * lbm_collision_model_mrt.cpp:114-143): ZLH_TRANSFORM_VALU (the default, pair sums and
+ * differences of opposite populations on the vector units) or ZLH_TRANSFORM_MFMA (19 x 19
+ * contractions on v_mfma_f32_32x32x2f32, FP32 operands and sums).  Other solver forms always use
+ * the vector units. */
+#define ZLH_TRANSFORM_VALU 0
+#define ZLH_TRANSFORM_MFMA 1
+zlh_status zlh_set_moment_transform(zlh_ctx *ctx, int where);
 /* largest number of 64-node bricks one level may hold with these settings (no reference
  * counterpart; no device call): slot indices are 32-bit, and the FP32 learned closure's fused
  * kernel addresses its pulls with 32-bit byte offsets (bricks x Q x 256 bytes <= 2^32, about
@@ -136,7 +155,7 @@ zlh_status zlh_step_level(zlh_ctx *ctx, int level, int time_step_level);
 /* coarse-to-fine interpolation into the fine ghost layers, then fine-to-coarse copy into the
  * coarse interface layer (SolverLbmInterface::InformationFromGridOfDifferentLevel,
  * lbm_solver_interface.cpp:224-238; the caller applies it after even substeps of fine_level,
- * ZLH_ESTATE after an odd one) */
+ * ZLH_ESTATE after an odd one unless time interpolation is on, zlh_set_time_interpolation) */
 zlh_status zlh_transfer_levels(zlh_ctx *ctx, int fine_level);
 /* its two halves, in this order: the coarse-to-fine interpolation
  * (GridInfoLbmInteface::TransferInfoFromCoarseGrid, lbm_grid_interface.cpp:408-515) and the

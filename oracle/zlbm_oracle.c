@@ -43,6 +43,7 @@ typedef struct zo_level {
     double *f, *fc;      /* [n][q] */
     double *rho, *u;     /* [n], [n][3] */
     double *force;       /* [n][3] node (immersed boundary) force */
+    double *f_prev;      /* [n][q] f at the start of the level's last substep (time interpolation) */
     int64_t nb[7];
     int32_t *bnd[7];     /* boundary node indices per direction */
     double *bnd_u[7];    /* per-node velocities [nb][3] of a face (zo_set_bc_velocities), or NULL */
@@ -70,6 +71,7 @@ struct zo_sim {
     int64_t mlp_fallbacks;
     int bc_profile[7];   /* per face: 0 uniform boundary velocity, 1 parabolic (zo_set_bc_profile) */
     int interp_method[ZO_MAXL];   /* per fine level: 0 Lagrangian (the reference default), 1 linear */
+    int time_interp;     /* 1: refill fine ghost layers after odd substeps too (zo_set_time_interp) */
 };
 
 static void zo_seterr(zo_sim *s, const char *fmt, ...) {
@@ -262,7 +264,7 @@ zo_sim *zo_create(const zo_params *p) {
 
 static void zo_free_level(zo_level *L) {
     free(L->key); free(L->flag); free(L->src); free(L->f); free(L->fc);
-    free(L->rho); free(L->u); free(L->force);
+    free(L->rho); free(L->u); free(L->force); free(L->f_prev);
     for (int d = 0; d < 7; ++d) { free(L->bnd[d]); free(L->bnd_u[d]); }
     for (int k = 0; k < 4; ++k) for (int l = 0; l < 3; ++l) free(L->layers.idx[k][l]);
     free(L->ib.xyz); free(L->ib.area); free(L->ib.vel); free(L->ib.force);
@@ -865,12 +867,24 @@ static void zo_lagrange_coeff(int half, double *c0, double *c1) {
 
 /* NodeInfoCoarse2fine (lbm_grid_interface.cpp:570-588) with PostStreamCoarse2Fine
  * (lbm_collision_models.cpp:244-251) using the coarse level's tau ratio */
-static void zo_node_c2f(const zo_sim *s, const zo_level *C, int64_t j, double *rho, double *u, double *f) {
+static void zo_node_c2f(const zo_sim *s, const zo_level *C, const double *fj, double *rho, double *u, double *f) {
     double feq[ZO_MAXQ];
-    zo_macro(s, C->f + j * s->q, rho, u);
+    zo_macro(s, fj, rho, u);
     zo_feq(s, *rho, u, feq);
     for (int iq = 0; iq < s->q; ++iq)
-        f[iq] = feq[iq] + C->tau_stream_c2f * (C->f[j * s->q + iq] - feq[iq]);
+        f[iq] = feq[iq] + C->tau_stream_c2f * (fj[iq] - feq[iq]);
+}
+
+/* Time interpolation (no reference counterpart: the reference refills fine ghost layers only
+ * after even substeps, when both levels have reached the same time, lbm_solver_interface.cpp:
+ * 224-238).  When enabled, the ghost layers are also refilled after odd substeps, at the fine
+ * level's half-way time between two coarse steps: every coarse contributor then gives the mean of
+ * its transferred values (zo_node_c2f) before and after the coarse level's last step, i.e. a
+ * linear interpolation in time; fine fallback contributors give their current values.  The
+ * even-substep transfers are unchanged. */
+int zo_set_time_interp(zo_sim *s, int on) {
+    s->time_interp = on != 0;
+    return 0;
 }
 
 /* interpolation method of a fine level's ghost nodes (GridInfoInterface::interp_method_,
@@ -889,7 +903,7 @@ int zo_set_interp_method(zo_sim *s, int level, int method) {
  * contributor is missing in both levels.  Linear is the Lagrangian form with max_interp_length_
  * 1 and the half-length-1 weights ((1, 0) at even, (1/2, 1/2) at odd fine coordinates) whatever
  * the region's valid length. */
-static int zo_interp_node(zo_sim *s, int fine_level, int64_t i) {
+static int zo_interp_node(zo_sim *s, int fine_level, int64_t i, int tmean) {
     zo_level *F = &s->lv[fine_level], *C = &s->lv[fine_level - 1];
     int dims = s->p.dims, q = s->q, clevel = fine_level - 1;
     const int linear = s->interp_method[fine_level] == 1;
@@ -933,7 +947,15 @@ static int zo_interp_node(zo_sim *s, int fine_level, int64_t i) {
                 double r, uu[3], ff[ZO_MAXQ];
                 const double *fr;
                 if (j >= 0 && !(C->flag[j] & ZO_FLAG_C2F_GHOST)) {
-                    zo_node_c2f(s, C, j, &r, uu, ff);
+                    zo_node_c2f(s, C, C->f + j * q, &r, uu, ff);
+                    if (tmean) {
+                        /* mean of the contributor's values before and after the coarse step */
+                        double r0, u0[3] = {0., 0., 0.}, f0[ZO_MAXQ];
+                        zo_node_c2f(s, C, C->f_prev + j * q, &r0, u0, f0);
+                        r = 0.5 * (r0 + r);
+                        for (int a = 0; a < dims; ++a) uu[a] = 0.5 * (u0[a] + uu[a]);
+                        for (int iq = 0; iq < q; ++iq) ff[iq] = 0.5 * (f0[iq] + ff[iq]);
+                    }
                     fr = ff;
                 } else {
                     int64_t k = zo_find(F, ck << dims);
@@ -954,13 +976,17 @@ static int zo_interp_node(zo_sim *s, int fine_level, int64_t i) {
  * interface, inner then outer per layer.  List order is ascending key (the reference's is the
  * unordered_map order; the results do not depend on it for grids whose ghost nodes never
  * serve as fallback contributors of each other). */
-static int zo_transfer_from_coarse(zo_sim *s, int fine_level) {
+static int zo_transfer_from_coarse(zo_sim *s, int fine_level, int tmean) {
     zo_level *F = &s->lv[fine_level];
     int errs = 0;
+    if (tmean && !s->lv[fine_level - 1].f_prev) {
+        zo_seterr(s, "time interpolation into level %d before a step of level %d", fine_level, fine_level - 1);
+        return -1;
+    }
     for (int layer = 1; layer < 3; ++layer) {
         for (int kind = ZO_INNER_F2C; kind >= ZO_OUTER_F2C; kind -= 2) {
             for (int64_t k = 0; k < F->layers.n[kind][layer]; ++k)
-                if (zo_interp_node(s, fine_level, F->layers.idx[kind][layer][k])) ++errs;
+                if (zo_interp_node(s, fine_level, F->layers.idx[kind][layer][k], tmean)) ++errs;
         }
     }
     if (errs) zo_seterr(s, "%d interpolation stencils incomplete at level %d", errs, fine_level);
@@ -1106,7 +1132,7 @@ int zo_get_ib_force(const zo_sim *s, int level, double *force) {
 /* test hook: the coarse/fine transfer of one refined level on the current populations */
 int zo_debug_transfer(zo_sim *s, int fine_level) {
     if (fine_level < 1 || fine_level > s->p.max_level) { zo_seterr(s, "bad level"); return -1; }
-    int rc = zo_transfer_from_coarse(s, fine_level);
+    int rc = zo_transfer_from_coarse(s, fine_level, 0);
     zo_transfer_to_coarse(s, fine_level);
     return rc;
 }
@@ -1123,13 +1149,23 @@ int zo_debug_set_s_base(zo_sim *s, const double *s_base) {
 int zo_advance_level(zo_sim *s, int level, int time_step_level) {
     if (!s->finalized) { zo_seterr(s, "not finalized"); return -1; }
     int rc = 0;
+    zo_level *L = &s->lv[level];
+    if (s->time_interp && level < s->p.max_level) {     /* the state the next finer level interpolates from */
+        if (!L->f_prev && !(L->f_prev = (double *)malloc(sizeof(double) * (size_t)L->n * s->q + 1))) {
+            zo_seterr(s, "out of memory");
+            return -1;
+        }
+        memcpy(L->f_prev, L->f, sizeof(double) * (size_t)L->n * s->q);
+    }
     if (zo_ib_level(s, level)) rc = -1;                  /* UpdateCriterion (:61-95) */
     zo_collision(s, level);                              /* RunSolverOnGivenGrid (:246-262) */
     zo_stream(s, level);
     zo_boundaries(s, level);                             /* ComputeDomainBoundaryCondition */
     if (level > 0 && time_step_level % 2 == 0) {         /* InformationFromGridOfDifferentLevel (:224-238) */
-        if (zo_transfer_from_coarse(s, level)) rc = -1;
+        if (zo_transfer_from_coarse(s, level, 0)) rc = -1;
         zo_transfer_to_coarse(s, level);
+    } else if (level > 0 && s->time_interp) {            /* time interpolation (zo_set_time_interp) */
+        if (zo_transfer_from_coarse(s, level, 1)) rc = -1;
     }
     return rc;
 }

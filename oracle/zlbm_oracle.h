@@ -94,6 +94,10 @@ int zo_time_stepping_order(int max_level, int32_t *order, int cap);
 int zo_set_bc_profile(zo_sim *s, int dir, int profile);
 /* interpolation of the ghost nodes of fine level `level` (>= 1): 0 Lagrangian (default), 1 linear */
 int zo_set_interp_method(zo_sim *s, int level, int method);
+/* 1: also refill fine ghost layers after odd substeps, each coarse contributor giving the mean of
+ * its values before and after the coarse level's last step (no reference counterpart; 0, the
+ * default, is the reference's behaviour) */
+int zo_set_time_interp(zo_sim *s, int on);
 /* per-node velocities [n][3] of a non-equilibrium-extrapolation face on one level, in key order of
  * the level's nodes on that face (no reference counterpart); checked against the count at finalize */
 int zo_set_bc_velocities(zo_sim *s, int level, int dir, const double *u, int64_t n);

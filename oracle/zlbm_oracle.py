@@ -65,6 +65,7 @@ def lib():
         L.zo_time_stepping_order.argtypes = [C.c_int, D(np.int32), C.c_int]
         L.zo_set_bc_profile.argtypes = [P, C.c_int, C.c_int]
         L.zo_set_interp_method.argtypes = [P, C.c_int, C.c_int]
+        L.zo_set_time_interp.argtypes = [P, C.c_int]
         L.zo_set_bc_velocities.argtypes = [P, C.c_int, C.c_int, np.ctypeslib.ndpointer(np.float64), C.c_int64]
         L.zo_set_ib.argtypes = [P, C.c_int, C.c_int64, D(np.float64), D(np.float64), D(np.float64),
                                 C.c_int, C.c_double, C.c_double]
@@ -115,7 +116,8 @@ class Oracle:
                  collision: Sequence[int] = (0,), les: int = 0, les_tau_mode: int = 1,
                  cs: float = 0.1, has_force: bool = False, const_force=(0., 0., 0.),
                  bc: Optional[Dict[int, tuple]] = None, compressible: bool = True,
-                 bc_profile: Optional[Dict[int, int]] = None, bc_velocities=None, interp_method: int = 0):
+                 bc_profile: Optional[Dict[int, int]] = None, bc_velocities=None, interp_method: int = 0,
+                 time_interp: bool = False):
         self.grid = grid
         self.dims = grid.dims
         self.q = 9 if grid.dims == 2 else 19
@@ -155,6 +157,7 @@ class Oracle:
             self._chk(L.zo_set_bc_profile(self.h, d, prof))
         for level in range(1, grid.max_level + 1):
             self._chk(L.zo_set_interp_method(self.h, level, interp_method))
+        self._chk(L.zo_set_time_interp(self.h, 1 if time_interp else 0))
         for (level, d), ub in (bc_velocities or {}).items():
             ub = np.ascontiguousarray(ub, np.float64).reshape(-1, 3)
             self._chk(L.zo_set_bc_velocities(self.h, level, d, ub.ravel(), len(ub)))

@@ -156,7 +156,7 @@ def write_params(path, kw, dims, collision="srt", precision="fp64", closure_path
 
 
 def run(grid, kw, steps, workdir, collision="srt", precision="fp64", model=None, checkpoint_in=None,
-        checkpoint_out=None, vtu_out=None, checkpoint_dir_in=None, checkpoint_dir_out=None):
+        checkpoint_out=None, vtu_out=None, checkpoint_dir_in=None, checkpoint_dir_out=None, time_interp=False):
     gpath = os.path.join(workdir, "grid.bin")
     ppath = os.path.join(workdir, "params.txt")
     opath = os.path.join(workdir, "out.bin")
@@ -175,6 +175,8 @@ def run(grid, kw, steps, workdir, collision="srt", precision="fp64", model=None,
         extra.append("checkpoint_dir_in = " + checkpoint_dir_in)
     if checkpoint_dir_out:
         extra.append("checkpoint_dir_out = " + checkpoint_dir_out)
+    if time_interp:
+        extra.append("time_interpolation = 1")
     write_params(ppath, kw, grid.dims, collision, precision, cpath, extra)
     proc = subprocess.run([DRIVER, gpath, ppath, str(steps), opath], capture_output=True, text=True, timeout=300)
     if proc.returncode != 0:

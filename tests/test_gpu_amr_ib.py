@@ -224,3 +224,57 @@ def test_graph_replay_with_immersed_boundary():
             s.time_marching(n)
         # FP64 atomics in the force spread may sum in another order
         assert_close(runs[0].f(0), runs[1].f(0), rtol=1e-12)
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_time_interpolation_fp64(dims):
+    """Time-interpolated ghost refills after odd substeps (zlh_set_time_interpolation): every
+    level against the oracle with the same option after 8 background steps, level streams and
+    graph replay on; the run must differ from the reference schedule's."""
+    grid, kw = sc.amr2d(levels=3) if dims == 2 else sc.amr3d()
+    kw.update(collision=(hip.MRT,), time_interp=True)
+    o, s = _pair(grid, kw)
+    o.time_marching(8)
+    s.time_marching(8)
+    for l in range(grid.max_level + 1):
+        assert_close(s.f(l), o.f(l))
+    o2, _ = _pair(grid, dict(kw, time_interp=False))
+    o2.time_marching(8)
+    assert np.abs(o2.f(1) - o.f(1)).max() > 1e-9
+
+
+def test_time_interpolation_substeps_and_restart():
+    """The time-interpolated refill through the C ABI (transfer_levels after an odd substep)
+    equals time marching bit for bit, with level streams and graphs (a) or on one stream (c); the
+    fine-to-coarse half stays refused after an odd substep; a solver restarted from the downloaded
+    populations (the upload takes the earlier coarse rows from the state it receives) continues
+    bit-exactly."""
+    grid, kw = sc.amr2d(levels=2)
+    kw.update(collision=(hip.MRT,), time_interp=True)
+    _, a = _pair(grid, kw)
+    _, b = _pair(grid, kw)
+    _, c = _pair(grid, kw)
+    c.level_streams_enable(False)
+    c.graphs_enable(False)
+    a.time_marching(3)
+    c.time_marching(3)
+    for _ in range(3):
+        for lvl, tsl in [(0, 1), (1, 1), (1, 2)]:
+            b.step_level(lvl, tsl)
+            if lvl > 0:
+                b.transfer_levels(lvl)
+    for l in range(2):
+        np.testing.assert_array_equal(c.f(l), a.f(l))
+        np.testing.assert_array_equal(b.f(l), a.f(l))
+    b.step_level(0, 1)
+    b.step_level(1, 1)
+    with pytest.raises(RuntimeError, match="odd substep"):
+        b.transfer_to_coarse(1)
+    # restart: a fresh solver from a's populations, two more steps each
+    r = hip.Solver(grid, precision=hip.FP64, **kw).build()
+    for l in range(2):
+        r.set_f(l, a.f(l))
+    a.time_marching(2)
+    r.time_marching(2)
+    for l in range(2):
+        np.testing.assert_array_equal(r.f(l), a.f(l))

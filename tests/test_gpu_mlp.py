@@ -185,17 +185,19 @@ def test_fused_closure_matches_oracle(coll):
     assert_close(s.f(0), o.f(0), rtol=2e-7)
 
 
-@pytest.mark.parametrize("operands", [hip.MLP_OPERANDS_BF16, hip.MLP_OPERANDS_FP16])
-def test_fp32_learned_closure_tracks_fp64(operands):
-    """FP32 solver (shifted populations, matrix-core network) against the FP64 oracle with the
-    same network in FP64."""
+@pytest.mark.parametrize("operands,transform", [(hip.MLP_OPERANDS_BF16, hip.TRANSFORM_VALU),
+                                                (hip.MLP_OPERANDS_FP16, hip.TRANSFORM_VALU),
+                                                (hip.MLP_OPERANDS_FP16, hip.TRANSFORM_MFMA)])
+def test_fp32_learned_closure_tracks_fp64(operands, transform):
+    """FP32 solver (shifted populations, matrix-core network, MRT transform on the vector units or
+    the matrix cores) against the FP64 oracle with the same network in FP64."""
     nu = 0.002
     grid, kw = sc.periodic_tgv(3, 16, nu=nu)
     m = _random_model(3 * nu + 0.5)
     o = Oracle(grid, collision=(hip.MRT,), les=2, cs=0.1, **kw)
     o.set_mlp(m.params.astype(np.float64), m.mean, m.std, m.fmin, m.fmax, m.nu_scale)
     s = hip.Solver(grid, precision=hip.FP32_SHIFTED, collision=(hip.MRT,), les=hip.LES_MLP, cs=0.1,
-                   mlp_operands=operands, **kw).build()
+                   mlp_operands=operands, moment_transform=transform, **kw).build()
     s.mlp_set(m)
     init = sc.tgv_fields(grid, 0.05, noise=1e-3)
     o.init_macro(0, *init)
@@ -206,24 +208,28 @@ def test_fp32_learned_closure_tracks_fp64(operands):
     rho_s, u_s = s.macros(0)
     l2_u = np.sqrt(np.sum((u_s - u_o) ** 2) / np.sum(u_o ** 2))
     l2_rho = np.sqrt(np.mean((rho_s - rho_o) ** 2))
-    print("FP32 + closure (operands %d) vs FP64: L2(u) %.2e, L2(rho) %.2e" % (operands, l2_u, l2_rho))
+    print("FP32 + closure (operands %d, transform %d) vs FP64: L2(u) %.2e, L2(rho) %.2e" % (
+        operands, transform, l2_u, l2_rho))
     assert l2_u < 1e-3, l2_u
     assert l2_rho < 1e-5, l2_rho
 
 
-def test_fp32_learned_closure_two_rounds_per_workgroup():
+@pytest.mark.parametrize("transform", [hip.TRANSFORM_VALU, hip.TRANSFORM_MFMA])
+def test_fp32_learned_closure_two_rounds_per_workgroup(transform):
     """A level large enough (97^3 nodes in 15,625 bricks: more than two waves of resident
     workgroups on a 256-CU device) that the learned-closure kernel takes two rounds of bricks per
     workgroup, each wave pulling its second brick into LDS while it collides the first. FP32
     against the FP64 solver with the same network, in lockstep: a round that read its brick before
-    the pull had landed would leave the FP32 run far from the FP64 one."""
+    the pull had landed would leave the FP32 run far from the FP64 one.  The MRT transform on the
+    matrix cores (exact integer operands, FP32 sums) tracks as closely as the vector-unit one."""
     nu = 0.002
     grid, kw = sc.periodic_tgv(3, 96, nu=nu)
     m = _random_model(3 * nu + 0.5)
     init = sc.tgv_fields(grid, 0.05, noise=1e-3)
     out = []
     for prec in (hip.FP64, hip.FP32_SHIFTED):
-        s = hip.Solver(grid, precision=prec, collision=(hip.MRT,), les=hip.LES_MLP, cs=0.1, **kw).build()
+        s = hip.Solver(grid, precision=prec, collision=(hip.MRT,), les=hip.LES_MLP, cs=0.1,
+                       moment_transform=transform, **kw).build()
         assert s.info(0)[2] == 25 ** 3
         s.mlp_set(m)
         s.init_macro(0, *init)
@@ -362,3 +368,26 @@ def test_fp32_learned_closure_on_refined_grid(operands):
     l2 = np.sqrt(num / den)
     print("FP32 + closure (operands %d) vs FP64 on 3 levels: L2(u) %.2e" % (operands, l2))
     assert l2 < 1e-4
+
+
+def test_moment_transform_switch_takes_effect():
+    """zlh_set_moment_transform selects other kernels: the two transforms round differently (FP32
+    sums in another order), so their runs differ in the last bits and stay within FP32 rounding
+    of each other; switching back mid-run is allowed."""
+    nu = 0.002
+    grid, kw = sc.periodic_tgv(3, 16, nu=nu)
+    m = _random_model(3 * nu + 0.5)
+    init = sc.tgv_fields(grid, 0.05, noise=1e-3)
+    out = []
+    for transform in (hip.TRANSFORM_VALU, hip.TRANSFORM_MFMA):
+        s = hip.Solver(grid, precision=hip.FP32_SHIFTED, collision=(hip.MRT,), les=hip.LES_MLP, cs=0.1,
+                       moment_transform=transform, **kw).build()
+        s.mlp_set(m)
+        s.init_macro(0, *init)
+        s.time_marching(10)
+        out.append(s.f(0))
+        s.set_moment_transform(hip.TRANSFORM_VALU)
+        s.time_marching(1)
+    d = np.abs(out[0] - out[1]).max()
+    print("VALU vs matrix-core transform after 10 steps: max |df| %.2e" % d)
+    assert 0 < d < 1e-5

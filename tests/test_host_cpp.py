@@ -35,6 +35,21 @@ def test_driver_matches_oracle(tmp_path, case):
         assert np.max(np.abs(out[l] - ref)) < 1e-11 * np.max(np.abs(ref))
 
 
+@pytest.mark.gpu
+def test_driver_time_interpolation_matches_oracle(tmp_path):
+    """SetTimeInterpolation(true) on the host: InformationFromGridOfDifferentLevel then also
+    refills the ghost layers after odd substeps (TransferInfoFromCoarseGrid over the C ABI), like
+    the oracle with the same option."""
+    grid, kw = sc.amr2d(levels=3)
+    proc, out = host_driver.run(grid, kw, 10, str(tmp_path), collision="mrt", time_interp=True)
+    assert proc.returncode == 0, proc.stderr
+    o = Oracle(grid, collision=(1,), time_interp=True, **kw)
+    o.time_marching(10)
+    for l in range(grid.max_level + 1):
+        ref = o.f(l)
+        assert np.max(np.abs(out[l] - ref)) < 1e-11 * np.max(np.abs(ref))
+
+
 def _cavity3d(n=10, lid=0.05):
     from zllbm_amd import grid as g
     grid = g.uniform_grid(3, (n, n, n))

@@ -37,11 +37,14 @@ __device__ __forceinline__ void macro_feq_double(const double *f, int compressib
 // w), so that each stencil contributor is one contiguous row; both row sets are sorted by aux
 // index, so consecutive threads read neighbouring entries of each direction's plane.  A
 // workgroup's 256 rows go out through LDS as one contiguous, coalesced run.
-template <int D, typename T, bool SHIFT>
+// MID (time interpolation after an odd fine substep): each coarse row is the mean of the row
+// computed now and the row `prev` of the last synchronised transfer, i.e. of the coarse state
+// before and after the coarse level's last step; fallback rows stay current copies.
+template <int D, typename T, bool SHIFT, bool MID>
 __global__ __launch_bounds__(256) void k_c2f_sources(int64_t n_src, int64_t n_fb, const int32_t *src_ai,
                                                      const T *aux_c, int64_t naux_c, const int32_t *fb_ai,
                                                      const T *aux_f, int64_t naux_f, double tau_ratio,
-                                                     int compressible, T *src_f) {
+                                                     int compressible, const T *__restrict__ prev, T *src_f) {
     constexpr int Q = Lattice<D>::Q;
     __shared__ T tile[256 * Q];   // row-major; the odd stride Q keeps the row writes conflict-free
     const int64_t r0 = (int64_t)blockIdx.x * 256;
@@ -66,7 +69,14 @@ __global__ __launch_bounds__(256) void k_c2f_sources(int64_t n_src, int64_t n_fb
     }
     __syncthreads();
     const int rows = (int)(n - r0 < 256 ? n - r0 : 256);
-    for (int e = threadIdx.x; e < rows * Q; e += 256) src_f[r0 * Q + e] = tile[e];
+    if constexpr (MID) {
+        // coarse rows [0, n_src) of this workgroup: elements below (n_src - r0) Q
+        const int64_t nc = (n_src - r0) * Q;
+        for (int e = threadIdx.x; e < rows * Q; e += 256)
+            src_f[r0 * Q + e] = e < nc ? T(0.5 * (double(prev[r0 * Q + e]) + double(tile[e]))) : tile[e];
+    } else {
+        for (int e = threadIdx.x; e < rows * Q; e += 256) src_f[r0 * Q + e] = tile[e];
+    }
 }
 
 // InterpolationLagrangian{2D,3D} (grid/grid_interpolation.cpp:224-304, 433-525): each ghost node
@@ -675,6 +685,7 @@ inline zlh_status upload_transfer(zlh_ctx *ctx, int level) {
     if ((s = upload(ctx, X.src_coarse_ai, src_ai))) return s;
     if ((s = upload(ctx, X.fb_fine_ai, fb_ai))) return s;
     if ((s = dalloc(ctx, X.src_f, ctx->tsize() * ctx->q * std::max<int64_t>(X.n_src + X.n_fb, 1)))) return s;
+    if ((s = dalloc(ctx, X.src_mid, ctx->tsize() * ctx->q * std::max<int64_t>(X.n_src + X.n_fb, 1)))) return s;
     if ((s = upload(ctx, X.f2c_coarse_ai, f2c_c))) return s;
     if ((s = upload(ctx, X.f2c_fine_ai, f2c_f))) return s;
     F.xfer_writes = tgt;
@@ -753,8 +764,11 @@ inline zlh_status upload_ib(zlh_ctx *ctx, int level) {
 // launches
 // ----------------------------------------------------------------------------------------------
 // parts: kTransferC2F (TransferInfoFromCoarseGrid) and / or kTransferF2C (TransferInfoToCoarseGrid),
-// in that order (InformationFromGridOfDifferentLevel, lbm_solver_interface.cpp:224-238)
-constexpr int kTransferC2F = 1, kTransferF2C = 2;
+// in that order (InformationFromGridOfDifferentLevel, lbm_solver_interface.cpp:224-238);
+// kTransferMid makes the coarse-to-fine part the time-interpolated refill after an odd substep
+// (rows into src_mid, src_f kept as the synchronised rows), kTransferRows only computes the
+// synchronised rows (the earlier state of the next time-interpolated refill)
+constexpr int kTransferC2F = 1, kTransferF2C = 2, kTransferMid = 4, kTransferRows = 8;
 template <int D, typename T, bool SHIFT>
 zlh_status launch_transfers_t(zlh_ctx *ctx, int level, int parts) {
     LevelState &F = ctx->lv[level], &C = ctx->lv[level - 1];
@@ -766,15 +780,26 @@ zlh_status launch_transfers_t(zlh_ctx *ctx, int level, int parts) {
     const double dtc = 1. / (double)(1u << (level - 1)), dtf = 1. / (double)(1u << level);
     const double c2f = 0.5 * (nu * 3. / dtc * 2. + 0.5) / (nu * 3. / dtc + 0.5);
     const double f2c = 2. * (nu * 3. / dtf / 2 + 0.5) / (nu * 3. / dtf + 0.5);
-    if ((parts & kTransferC2F) && X.n_c2f > 0) {
-        if (X.n_src + X.n_fb > 0)
-            hipLaunchKernelGGL((k_c2f_sources<D, T, SHIFT>), dim3(blocks(X.n_src + X.n_fb)), dim3(256), 0, ctx->stream,
-                               X.n_src, X.n_fb, X.src_coarse_ai.as<int32_t>(), C.aux_f_d.as<T>(), C.naux,
-                               X.fb_fine_ai.as<int32_t>(), F.aux_f_d.as<T>(), F.naux, c2f, comp, X.src_f.as<T>());
-        hipLaunchKernelGGL((k_c2f_gather<D, T, SHIFT>), dim3((unsigned)((X.n_groups + 7) / 8 * 8)), dim3(256),
-                           X.lds_bytes, ctx->stream, X.n_groups, X.groups.as<GatherGroup>(), X.c2f_target.as<int32_t>(),
-                           X.c2f_begin.as<int32_t>(), X.c2f_contrib.as<uint32_t>(), X.group_rows.as<int32_t>(),
-                           X.src_f.as<T>(), F.aux_f_d.as<T>(), F.naux);
+    if ((parts & (kTransferC2F | kTransferRows)) && X.n_c2f > 0) {
+        const bool mid = (parts & kTransferMid) != 0;
+        T *rows = mid ? X.src_mid.as<T>() : X.src_f.as<T>();
+        if (X.n_src + X.n_fb > 0) {
+            if (mid)
+                hipLaunchKernelGGL((k_c2f_sources<D, T, SHIFT, true>), dim3(blocks(X.n_src + X.n_fb)), dim3(256), 0,
+                                   ctx->stream, X.n_src, X.n_fb, X.src_coarse_ai.as<int32_t>(), C.aux_f_d.as<T>(),
+                                   C.naux, X.fb_fine_ai.as<int32_t>(), F.aux_f_d.as<T>(), F.naux, c2f, comp,
+                                   X.src_f.as<T>(), rows);
+            else
+                hipLaunchKernelGGL((k_c2f_sources<D, T, SHIFT, false>), dim3(blocks(X.n_src + X.n_fb)), dim3(256), 0,
+                                   ctx->stream, X.n_src, X.n_fb, X.src_coarse_ai.as<int32_t>(), C.aux_f_d.as<T>(),
+                                   C.naux, X.fb_fine_ai.as<int32_t>(), F.aux_f_d.as<T>(), F.naux, c2f, comp,
+                                   nullptr, rows);
+        }
+        if (parts & kTransferC2F)
+            hipLaunchKernelGGL((k_c2f_gather<D, T, SHIFT>), dim3((unsigned)((X.n_groups + 7) / 8 * 8)), dim3(256),
+                               X.lds_bytes, ctx->stream, X.n_groups, X.groups.as<GatherGroup>(),
+                               X.c2f_target.as<int32_t>(), X.c2f_begin.as<int32_t>(), X.c2f_contrib.as<uint32_t>(),
+                               X.group_rows.as<int32_t>(), rows, F.aux_f_d.as<T>(), F.naux);
     }
     if ((parts & kTransferF2C) && X.n_f2c > 0)
         hipLaunchKernelGGL((k_f2c<D, T, SHIFT>), dim3(blocks(X.n_f2c)), dim3(256), 0, ctx->stream, X.n_f2c,

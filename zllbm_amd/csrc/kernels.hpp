@@ -205,28 +205,34 @@ __device__ __forceinline__ void collide_mrt(C (&f)[Lattice<D>::Q], const C *F, c
     constexpr int Q = L::Q;
     constexpr MomentSpace<D> S{};
     constexpr int NP = S.NP;
-    const int les = VAR >= 0 ? (VAR >> 1) : les_code(p.les, p.les_tau_mode, p.mlp_operands);
+    const int les = VAR >= 0 ? ((VAR & 15) >> 1) : les_code(p.les, p.les_tau_mode, p.mlp_operands);
     const bool forced = VAR >= 0 ? (VAR & 1) != 0 : p.has_force != 0;
     const C dt = C(p.dt);
-    C ps[NP], pd[NP];
-#pragma unroll
-    for (int j = 0; j < NP; ++j) {
-        ps[j] = f[S.rep[j]] + f[S.opp[j]];
-        pd[j] = f[S.rep[j]] - f[S.opp[j]];
-    }
+    // the transforms run on the matrix cores in the kVarMrtMfma variants of the FP32 learned-
+    // closure kernels (every lane of the wavefront collides there), on the vector units otherwise
     C mm[Q];
-#pragma unroll
-    for (int k = 0; k < Q; ++k) {
-        C acc = S.even[k] ? C(S.m[k][0]) * f[0] : C(0);
+    if constexpr (mrt_on_mfma<C, VAR>()) {
+        moments_mfma<Q>(*ops, f, mm);
+    } else {
+        C ps[NP], pd[NP];
 #pragma unroll
         for (int j = 0; j < NP; ++j) {
-            const int c = S.m[k][S.rep[j]];
-            const C v = S.even[k] ? ps[j] : pd[j];
-            if (c == 1) acc += v;
-            else if (c == -1) acc -= v;
-            else if (c != 0) acc += C(c) * v;
+            ps[j] = f[S.rep[j]] + f[S.opp[j]];
+            pd[j] = f[S.rep[j]] - f[S.opp[j]];
         }
-        mm[k] = acc;
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+            C acc = S.even[k] ? C(S.m[k][0]) * f[0] : C(0);
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                const int c = S.m[k][S.rep[j]];
+                const C v = S.even[k] ? ps[j] : pd[j];
+                if (c == 1) acc += v;
+                else if (c == -1) acc -= v;
+                else if (c != 0) acc += C(c) * v;
+            }
+            mm[k] = acc;
+        }
     }
     // macroscopic variables (CalMacro*/CalMacroForce*, lbm_solver_interface.cpp:477-592): rows 0
     // and 3, 5 (, 7) are the density and the momentum
@@ -316,6 +322,10 @@ __device__ __forceinline__ void collide_mrt(C (&f)[Lattice<D>::Q], const C *F, c
 #pragma unroll
     for (int k = 0; k < Q; ++k)
         if (L::viscous_row[k]) post(k, s_nu, true);
+    if constexpr (mrt_on_mfma<C, VAR>()) {
+        populations_mfma<Q>(*ops, y, f);
+        return;
+    }
     // back to populations: even rows give the pair mean, odd rows the pair difference
     {
         C acc = C(0);
@@ -354,7 +364,7 @@ __device__ __forceinline__ void collide(C (&f)[Lattice<D>::Q], const C *F, const
     }
     using L = Lattice<D>;
     constexpr int Q = L::Q;
-    const int les = VAR >= 0 ? (VAR >> 1) : les_code(p.les, p.les_tau_mode, p.mlp_operands);
+    const int les = VAR >= 0 ? ((VAR & 15) >> 1) : les_code(p.les, p.les_tau_mode, p.mlp_operands);
     const bool forced = VAR >= 0 ? (VAR & 1) != 0 : p.has_force != 0;
     const C dt = C(p.dt);
     C rho, drho;

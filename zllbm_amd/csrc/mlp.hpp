@@ -26,12 +26,18 @@ __host__ __device__ constexpr int mfma_row(int h, int r) { return (r >> 2) * 8 +
 //                  holds b2[l % 32] in slot 0 (its B slot holds 1)
 //   w3r[h][r]    = w3[mfma_row(h, r)] (FP32)
 // plus the input normalisation, training range, output scale and b3, so that the fused kernel
-// reads nothing of the network from global memory after its first store
+// reads nothing of the network from global memory after its first store; and the operand images
+// of the MRT moment transform on the matrix cores (moments_mfma, populations_mfma):
+//   mfwd[s][l] = M[l % 32][2 s + l / 32]   (moment row, population column; zero outside Q x Q)
+//   mbwd[s][l] = M[2 s + l / 32][l % 32]   (M^T)
+constexpr int kMrtSteps = 10;   // K steps of two: (19 + 1) / 2 for D3Q19, 5 used for D2Q9
 struct alignas(16) MlpOps {
     uint16_t w1a[64][8];
     uint16_t w2a[3][64][8];
     uint16_t h1a[64][8];
     uint16_t h2a[3][64][8];
+    float mfwd[kMrtSteps][64];
+    float mbwd[kMrtSteps][64];
     float w3r[2][16];
     float mean[kMlpF];
     float inv_std[kMlpF];
@@ -153,10 +159,16 @@ typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
-// sub-grid codes of the learned closure: 2 with BF16 operands, 4 with FP16 (VAR = 2 code + forced)
+// kernel variants: VAR = 2 sub-grid code + forced (+ kVarMrtMfma); sub-grid codes of the learned
+// closure: 2 with BF16 operands, 4 with FP16
+constexpr int kVarMrtMfma = 16;   // the FP32 learned-closure kernels' MRT transform on the matrix cores
 template <typename C, int VAR>
 __host__ __device__ constexpr bool mlp_on_mfma() {
-    return std::is_same<C, float>::value && (VAR < 0 || (VAR >> 1) == 2 || (VAR >> 1) == 4);
+    return std::is_same<C, float>::value && (VAR < 0 || ((VAR & 15) >> 1) == 2 || ((VAR & 15) >> 1) == 4);
+}
+template <typename C, int VAR>
+__host__ __device__ constexpr bool mrt_on_mfma() {
+    return VAR >= 0 && (VAR & kVarMrtMfma) != 0 && mlp_on_mfma<C, VAR>();
 }
 
 struct OperandsBf16 {
@@ -200,6 +212,63 @@ __device__ __forceinline__ float silu_scaled(float t) {
 __device__ __forceinline__ float softplus_fast(float z) {
     const float e = __builtin_amdgcn_exp2f(-1.4426950408889634f * __builtin_fabsf(z));
     return __builtin_fmaxf(z, 0.f) + 0.6931471805599453f * __builtin_amdgcn_logf(1.f + e);
+}
+
+// The MRT moment transform m = M f and its inverse f = M^T y as batched 19 x 19 contractions on
+// the matrix cores (v_mfma_f32_32x32x2f32, FP32 operands, products exact, FP32 sums): the 64 nodes
+// of the wavefront are two 32-node column blocks, a K step takes populations 2s and 2s + 1.  The
+// B operand of block 0 holds population 2s of node l (lanes 0..31) and 2s + 1 of node l - 32
+// (lanes 32..63); block 1 the same for nodes 32..63: v_permlane32_swap(f[2s], f[2s + 1]) gives
+// both.  The results (rows in the accumulator order of a 32 x 32 product) return to one node per
+// lane by v_permlane32_swap(block 0, block 1) of each accumulator register: then register r of
+// the first result holds row 8 (r / 4) + r % 4 and of the second row 8 (r / 4) + 4 + r % 4.
+// Every lane of the wavefront must call them (EXEC all ones).
+__device__ __forceinline__ constexpr int mrt_reg(int k) { return 4 * (k >> 3) + (k & 3); }
+__device__ __forceinline__ constexpr bool mrt_upper(int k) { return (k & 4) != 0; }
+
+template <int Q>
+__device__ __forceinline__ void rows_to_lanes(const f32x16_t &d0, const f32x16_t &d1, float (&out)[Q]) {
+    float lo[16], hi[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        // only the registers that carry rows below Q
+        const bool need = 8 * (r >> 2) + (r & 3) < Q;
+        if (!need) { lo[r] = hi[r] = 0.f; continue; }
+        const auto s = __builtin_amdgcn_permlane32_swap(__float_as_uint(d0[r]), __float_as_uint(d1[r]), false, false);
+        lo[r] = __uint_as_float(s[0]);
+        hi[r] = __uint_as_float(s[1]);
+    }
+#pragma unroll
+    for (int k = 0; k < Q; ++k) out[k] = mrt_upper(k) ? hi[mrt_reg(k)] : lo[mrt_reg(k)];
+}
+
+template <int Q>
+__device__ __forceinline__ void contract_mfma(const float (&x)[Q], const float (*a)[64], float (&out)[Q]) {
+    constexpr int NK = (Q + 1) / 2;
+    static_assert(NK <= kMrtSteps, "transform operand images cover the lattice");
+    const int lane = __lane_id();
+    f32x16_t d0 = {}, d1 = {};
+#pragma unroll
+    for (int s = 0; s < NK; ++s) {
+        const float lo = x[2 * s];
+        const float hi = 2 * s + 1 < Q ? x[2 * s + 1] : 0.f;
+        const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(lo), __float_as_uint(hi), false, false);
+        const float av = a[s][lane];
+        d0 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, __uint_as_float(b[0]), d0, 0, 0, 0);
+        d1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av, __uint_as_float(b[1]), d1, 0, 0, 0);
+    }
+    rows_to_lanes<Q>(d0, d1, out);
+}
+
+// m = M f of this lane's node (shifted populations give shifted moments)
+template <int Q>
+__device__ __forceinline__ void moments_mfma(const MlpOps &o, const float (&f)[Q], float (&m)[Q]) {
+    contract_mfma<Q>(f, o.mfwd, m);
+}
+// f = M^T y of this lane's node
+template <int Q>
+__device__ __forceinline__ void populations_mfma(const MlpOps &o, const float (&y)[Q], float (&f)[Q]) {
+    contract_mfma<Q>(y, o.mbwd, f);
 }
 
 // value held by lane l ^ 32 (v_permlane32_swap exchanges the halves of a register pair; unlike

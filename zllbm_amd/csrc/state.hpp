@@ -56,7 +56,9 @@ struct TransferTables {
     DevBuf src_coarse_ai;       // [n_src] coarse aux index of each source row (ascending)
     DevBuf fb_fine_ai;          // [n_fb] fine aux index of each fallback row (ascending)
     DevBuf src_f;               // [n_src + n_fb][Q] rescaled coarse rows, then fallback copies
-                                // (storage form)
+                                // (storage form), as of the last synchronised transfer
+    DevBuf src_mid;             // the same rows of a time-interpolated refill (coarse rows: mean
+                                // of src_f and the current ones)
     DevBuf f2c_coarse_ai;       // [n_f2c]
     DevBuf f2c_fine_ai;         // [n_f2c] fine aux index of the coincident node
 };

@@ -432,6 +432,8 @@ struct zlh_ctx {
     int bc_profile[7] = {};   // per face: ZLH_PROFILE_*
     std::map<std::pair<int, int>, std::vector<double>> bc_node_u;   // (level, face) -> [n][3] per-node velocities
     int interp_method[16] = {};   // per fine level: ZLH_INTERP_LAGRANGIAN or ZLH_INTERP_LINEAR
+    bool time_interp = false;     // refill ghost layers after odd substeps too (time-interpolated)
+    int mrt_transform = ZLH_TRANSFORM_VALU;   // FP32 learned-closure kernels: ZLH_TRANSFORM_*
     bool level_streams = true;
     std::vector<hipStream_t> lstream;
     std::vector<hipEvent_t> ev_step, ev_xfer, ev_join;
@@ -637,8 +639,9 @@ void launch_stream_collide_v(zlh_ctx *ctx, int var, unsigned grid, const BrickLe
     case V:                                                                                                   \
         if constexpr (mlp_on_mfma<C, V>() && sizeof(T) == 4) {                                                \
             launch_mlp<D, T, C, SHIFT, COLL, V>(ctx, v, fa, fb, aux_f, sp);                                   \
-        } else if constexpr (mlp_on_mfma<C, V>()) {                                                           \
-            /* BF16 storage with the learned closure: refused by zlh_create */                                \
+        } else if constexpr (mlp_on_mfma<C, V>() || V >= kVarMrtMfma) {                                       \
+            /* BF16 storage with the learned closure: refused by zlh_create; kVarMrtMfma is only */           \
+            /* selected for the FP32 learned closure */                                                       \
         } else {                                                                                              \
             hipLaunchKernelGGL((k_stream_collide<D, T, C, SHIFT, COLL, V>), dim3(grid), dim3(256), 0, ctx->stream, \
                                v, fa, fb, aux_f, sp);                                                         \
@@ -646,6 +649,7 @@ void launch_stream_collide_v(zlh_ctx *ctx, int var, unsigned grid, const BrickLe
         break;
     switch (var) {
         ZLH_SC(0) ZLH_SC(1) ZLH_SC(2) ZLH_SC(3) ZLH_SC(4) ZLH_SC(5) ZLH_SC(6) ZLH_SC(7) ZLH_SC(8) ZLH_SC(9)
+        ZLH_SC(20) ZLH_SC(21) ZLH_SC(24) ZLH_SC(25)
         default: break;
     }
 #undef ZLH_SC
@@ -662,7 +666,11 @@ void launch_aux_collide_v(zlh_ctx *ctx, int var, LevelState &L, const BrickLevel
         break;
     switch (var) {
         ZLH_AC(0) ZLH_AC(1) ZLH_AC(2) ZLH_AC(3) ZLH_AC(4) ZLH_AC(5) ZLH_AC(6) ZLH_AC(7) ZLH_AC(8) ZLH_AC(9)
-        default: break;
+        default:
+            if constexpr (std::is_same<C, float>::value && sizeof(T) == 4) {
+                switch (var) { ZLH_AC(20) ZLH_AC(21) ZLH_AC(24) ZLH_AC(25) default: break; }
+            }
+            break;
     }
 #undef ZLH_AC
 }
@@ -681,7 +689,10 @@ zlh_status launch_step_kernels(zlh_ctx *ctx, int level, int coll) {
     T *fa = L.f_d[L.cur].as<T>(), *fb = L.f_d[L.cur ^ 1].as<T>();
     T *aux_f = L.aux_f_d.as<T>();
     const double *aux_force = L.aux_force_d.as<double>();
-    const int var = 2 * les_code(L.sp.les, L.sp.les_tau_mode, L.sp.mlp_operands) + (L.sp.has_force ? 1 : 0);
+    int var = 2 * les_code(L.sp.les, L.sp.les_tau_mode, L.sp.mlp_operands) + (L.sp.has_force ? 1 : 0);
+    if (std::is_same<C, float>::value && sizeof(T) == 4 && (var >> 1 == 2 || var >> 1 == 4) &&
+        ctx->mrt_transform == ZLH_TRANSFORM_MFMA)
+        var += kVarMrtMfma;
     // the aux collision takes the nodes deferred by the last substep (every recorded node after
     // an upload)
     const int32_t *list = nullptr;
@@ -696,7 +707,8 @@ zlh_status launch_step_kernels(zlh_ctx *ctx, int level, int coll) {
         else launch_aux_collide_v<D, T, C, SHIFT, 1>(ctx, var, L, v, list, nlist, aux_f, aux_force, fa);
     }
     // this substep defers the ghost targets too when the transfer into the level follows it
-    const int kind = (level > 0 && L.last_tsl % 2 == 0) ? 1 : 0;
+    // (after every substep with time interpolation)
+    const int kind = (level > 0 && (L.last_tsl % 2 == 0 || ctx->time_interp)) ? 1 : 0;
     v.defmask = L.defmask_d[kind].as<uint64_t>();
     const int64_t groups = (L.nbrick + 3) / 4;
     const unsigned grid = (unsigned)(((groups + 7) / 8) * 8);
@@ -1030,6 +1042,10 @@ zlh_status build_level_aux(zlh_ctx *ctx, int level) {
 // ==============================================================================================
 // C ABI
 // ==============================================================================================
+namespace {
+zlh_status refresh_transfer_rows(zlh_ctx *ctx, int coarse);
+}  // namespace
+
 extern "C" {
 
 zlh_status zlh_create(const zlh_solver_desc *desc, zlh_ctx **out) {
@@ -1200,6 +1216,7 @@ zlh_status zlh_upload_populations(zlh_ctx *ctx, int level, const double *f) {
     ZLH_TRY(hipMemcpyAsync(tmp.p, f, sizeof(double) * L.n * ctx->q, hipMemcpyHostToDevice, ctx->stream));
     L.cur = 0;
     if ((s = do_upload(ctx, level, tmp.as<double>()))) return s;
+    if ((s = refresh_transfer_rows(ctx, level))) return s;
     ZLH_TRY(hipStreamSynchronize(ctx->stream));
     return ZLH_OK;
 }
@@ -1253,11 +1270,29 @@ zlh_status transfer_checked(zlh_ctx *ctx, int fine_level, int parts) {
     if (!ctx->built) return ctx->fail(ZLH_ESTATE, "build before transfer");
     if (fine_level < 1 || fine_level > ctx->d.max_level) return ctx->fail(ZLH_EINVAL, "bad fine level %d", fine_level);
     // InformationFromGridOfDifferentLevel (lbm_solver_interface.cpp:224-238) exchanges after the
-    // fine level's even substeps, whose fused kernel defers the ghost targets this writes
-    if (ctx->lv[fine_level].last_tsl % 2 != 0)
-        return ctx->fail(ZLH_ESTATE, "transfer into level %d after its odd substep %d", fine_level,
-                         ctx->lv[fine_level].last_tsl);
+    // fine level's even substeps, whose fused kernel defers the ghost targets this writes.  With
+    // time interpolation every substep defers them, and after an odd one (the coarse level a
+    // step ahead) the coarse-to-fine part is the time-interpolated refill and the fine-to-coarse
+    // part has nothing to give.
+    if (ctx->lv[fine_level].last_tsl % 2 != 0) {
+        if (!ctx->time_interp || !(parts & kTransferC2F))
+            return ctx->fail(ZLH_ESTATE, "transfer %s level %d after its odd substep %d",
+                             (parts & kTransferC2F) ? "into" : "from", fine_level, ctx->lv[fine_level].last_tsl);
+        return launch_transfers(ctx, fine_level, kTransferC2F | kTransferMid);
+    }
     return launch_transfers(ctx, fine_level, parts);
+}
+
+// the synchronised coarse rows of every refined level whose coarse level is `coarse` (-1: all),
+// from the coarse state as it is now: the earlier state of the next time-interpolated refill
+zlh_status refresh_transfer_rows(zlh_ctx *ctx, int coarse) {
+    if (!ctx->time_interp) return ZLH_OK;
+    for (int l = 1; l <= ctx->d.max_level; ++l) {
+        if (coarse >= 0 && l - 1 != coarse) continue;
+        zlh_status s = launch_transfers(ctx, l, kTransferRows);
+        if (s) return s;
+    }
+    return ZLH_OK;
 }
 }  // namespace
 
@@ -1316,7 +1351,9 @@ zlh_status march_launches(zlh_ctx *ctx, int nsteps) {
             }
             if ((s = zlh_step_level(ctx, l, tsl[l]))) break;
             if (split && (e = hipEventRecord(ctx->ev_step[l], ctx->stream)) != hipSuccess) break;
-            if (l > 0 && tsl[l] % 2 == 0) {
+            // (with time interpolation, after odd substeps too: the refill reads the coarse level's
+            // latest state, which its next step overwrites, so the same ordering holds)
+            if (l > 0 && (tsl[l] % 2 == 0 || ctx->time_interp)) {
                 if (split && (e = hipStreamWaitEvent(ctx->stream, ctx->ev_step[l - 1], 0)) != hipSuccess) break;
                 // where the transfers into l and l + 1 touch common nodes of level l (interfaces
                 // that come within a stencil), both keep the serial order: the transfer into l
@@ -1448,6 +1485,30 @@ zlh_status zlh_set_interpolation_method(zlh_ctx *ctx, int level, int method) {
     if (method != ZLH_INTERP_LAGRANGIAN && method != ZLH_INTERP_LINEAR)
         return ctx->fail(ZLH_EINVAL, "bad interpolation method %d", method);
     ctx->interp_method[level] = method;
+    return ZLH_OK;
+}
+
+zlh_status zlh_set_time_interpolation(zlh_ctx *ctx, int enable) {
+    if (!ctx) return ZLH_EINVAL;
+    if (ctx->time_interp == (enable != 0)) return ZLH_OK;
+    ctx->time_interp = enable != 0;
+    ctx->drop_graphs();   // captured graphs hold the other transfer schedule
+    if (ctx->built) {
+        // the synchronised rows from the state as it is now (after a whole background step, the
+        // rows the last transfers computed)
+        zlh_status s = refresh_transfer_rows(ctx, -1);
+        if (s) return s;
+        ZLH_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    return ZLH_OK;
+}
+
+zlh_status zlh_set_moment_transform(zlh_ctx *ctx, int where) {
+    if (!ctx) return ZLH_EINVAL;
+    if (where != ZLH_TRANSFORM_VALU && where != ZLH_TRANSFORM_MFMA)
+        return ctx->fail(ZLH_EINVAL, "bad moment transform %d", where);
+    ctx->mrt_transform = where;
+    ctx->drop_graphs();   // captured graphs hold the other kernels
     return ZLH_OK;
 }
 
@@ -1671,6 +1732,20 @@ zlh_status zlh_mlp_set(zlh_ctx *ctx, const float *params, const float *mean, con
     }
     for (int h = 0; h < 2; ++h)
         for (int r = 0; r < 16; ++r) o.w3r[h][r] = (float)((double)w3[mfma_row(h, r)] / kSiluScale);
+    // operand images of the moment transform on the matrix cores (M and M^T, integers)
+    {
+        auto fill = [&o](auto S) {
+            constexpr int Q = decltype(S)::Q;
+            for (int s = 0; s < kMrtSteps; ++s)
+                for (int l = 0; l < 64; ++l) {
+                    const int i = l % 32, k = 2 * s + l / 32;
+                    o.mfwd[s][l] = (i < Q && k < Q) ? (float)S.m[i][k] : 0.f;
+                    o.mbwd[s][l] = (i < Q && k < Q) ? (float)S.m[k][i] : 0.f;
+                }
+        };
+        if (ctx->d.dims == 2) fill(MomentSpace<2>{});
+        else fill(MomentSpace<3>{});
+    }
     for (int i = 0; i < kMlpF; ++i) {
         o.mean[i] = m.mean[i];
         o.inv_std[i] = m.inv_std[i];

@@ -26,6 +26,7 @@ MLP_FP32, MLP_BF16_MFMA, MLP_FP16_MFMA = 0, 1, 2
 MLP_OPERANDS_BF16, MLP_OPERANDS_FP16 = 0, 1
 MLP_NOALLOC_MAX_SAMPLES = 65536   # ZLH_MLP_NOALLOC_MAX_SAMPLES
 INTERP_LAGRANGIAN, INTERP_LINEAR = 0, 1
+TRANSFORM_VALU, TRANSFORM_MFMA = 0, 1
 
 
 class SolverDesc(C.Structure):
@@ -49,7 +50,7 @@ EXPORTED = (
     "zlh_mlp_gradient_noalloc", "zlh_set_boundary_velocities", "zlh_set_interpolation_method",
     "zlh_download_node_forces", "zlh_transfer_from_coarse", "zlh_transfer_to_coarse",
     "zlh_mlp_train", "zlh_mlp_training_form", "zlh_mlp_eval", "zlh_closure_features", "zlh_graphs_enable",
-    "zlh_max_level_bricks",
+    "zlh_max_level_bricks", "zlh_set_time_interpolation", "zlh_set_moment_transform",
 )
 
 
@@ -104,6 +105,8 @@ def lib():
         L.zlh_set_boundary_profile.argtypes = [P, C.c_int, C.c_int]
         L.zlh_set_boundary_velocities.argtypes = [P, C.c_int, C.c_int, D(np.float64), C.c_int64]
         L.zlh_set_interpolation_method.argtypes = [P, C.c_int, C.c_int]
+        L.zlh_set_time_interpolation.argtypes = [P, C.c_int]
+        L.zlh_set_moment_transform.argtypes = [P, C.c_int]
         L.zlh_max_level_bricks.argtypes = [C.POINTER(SolverDesc), C.POINTER(C.c_int64)]
         _lib = L
     return _lib
@@ -160,11 +163,14 @@ class Solver:
     GridInfoLbmInteface pair of the reference, lbm_interface.h:140-474)."""
 
     def __init__(self, grid: "_grid.Grid", bc_profile=None, bc_velocities=None, interp_method=INTERP_LAGRANGIAN,
-                 **kw):
+                 time_interp=False, moment_transform=TRANSFORM_VALU, **kw):
         """bc_profile: {face: PROFILE_PARABOLIC} for non-equilibrium-extrapolation faces;
         bc_velocities: {(level, face): [n][3] velocities of the level's nodes on that face, in key
         order} (zlh_set_boundary_velocities); interp_method: INTERP_LAGRANGIAN or INTERP_LINEAR
-        ghost-node interpolation of every refined level (zlh_set_interpolation_method)."""
+        ghost-node interpolation of every refined level (zlh_set_interpolation_method);
+        time_interp: refill the ghost layers after odd substeps too (zlh_set_time_interpolation);
+        moment_transform: TRANSFORM_VALU or TRANSFORM_MFMA, where the FP32 learned-closure kernels
+        run the MRT transform (zlh_set_moment_transform)."""
         self.grid = grid
         self.dims = grid.dims
         self.q = 9 if grid.dims == 2 else 19
@@ -182,6 +188,8 @@ class Solver:
         for (level, face), ub in (bc_velocities or {}).items():
             ub = np.ascontiguousarray(ub, np.float64).reshape(-1, 3)
             self._chk(L.zlh_set_boundary_velocities(self.h, level, face, ub.ravel(), len(ub)))
+        self._chk(L.zlh_set_time_interpolation(self.h, 1 if time_interp else 0))
+        self._chk(L.zlh_set_moment_transform(self.h, moment_transform))
         for l, lev in enumerate(grid.levels):
             self._chk(L.zlh_set_level_nodes(self.h, l, _checked(lev.keys, np.uint64, lev.n, "level %d keys" % l),
                                             _checked(lev.flags, np.int32, lev.n, "level %d flags" % l), lev.n))
@@ -351,6 +359,14 @@ class Solver:
 
     def graphs_enable(self, on=True):
         self._chk(lib().zlh_graphs_enable(self.h, 1 if on else 0))
+
+    def set_time_interpolation(self, on=True):
+        """Refill the ghost layers after odd substeps too (zlh_set_time_interpolation)."""
+        self._chk(lib().zlh_set_time_interpolation(self.h, 1 if on else 0))
+
+    def set_moment_transform(self, where):
+        """TRANSFORM_VALU or TRANSFORM_MFMA (zlh_set_moment_transform)."""
+        self._chk(lib().zlh_set_moment_transform(self.h, where))
 
     def level_streams_enable(self, on=True):
         """Run each level's substeps on a stream of its own (default on); results are identical."""

@@ -132,6 +132,7 @@ int main(int argc, char **argv) {
                 std::string t; val >> t;
                 lbm->SetPrecision(t == "fp32" ? ZLH_FP32_SHIFTED : t == "bf16" ? ZLH_BF16_SHIFTED : ZLH_FP64);
             }
+            else if (key == "time_interpolation") { int v; val >> v; lbm->SetTimeInterpolation(v != 0); }
             else if (key == "mlp_operands") {
                 std::string t; val >> t;
                 lbm->SetMlpOperands(t == "bf16" ? ZLH_MLP_OPERANDS_BF16 : ZLH_MLP_OPERANDS_FP16);

@@ -334,6 +334,7 @@ void SolverLbmHip::BuildDevice() {
         ctx_ = nullptr;
         throw ZlhError("zlh_create failed (" + std::to_string(s) + ")");
     }
+    Check(zlh_set_time_interpolation(ctx_, time_interpolation_ ? 1 : 0), "zlh_set_time_interpolation");
     for (int l = 0; l <= gm.k0MaxLevel_; ++l) {
         amrproject::GridInfoInterface &gi = *gm.vec_ptr_grid_info_.at(l);
         std::vector<std::pair<uint64_t, int32_t>> nodes;
@@ -415,17 +416,22 @@ void SolverLbmHip::RunSolverOnGivenGrid(const amrproject::ETimeSteppingScheme, c
 // SolverLbmInterface::InformationFromGridOfDifferentLevel (lbm_solver_interface.cpp:224-238)
 // SolverLbmInterface::InformationFromGridOfDifferentLevel (lbm_solver_interface.cpp:224-238): after
 // the fine level's even substeps, the fine grid's TransferInfoFromCoarseGrid, then its
-// TransferInfoToCoarseGrid, with the coarse grid's information
+// TransferInfoToCoarseGrid, with the coarse grid's information; with time interpolation also
+// the (time-interpolated) TransferInfoFromCoarseGrid after odd substeps
 void SolverLbmHip::InformationFromGridOfDifferentLevel(const DefInt time_step_level,
                                                        const amrproject::SFBitsetAuxInterface &sfbitset_aux,
                                                        amrproject::GridInfoInterface *const ptr_grid_info) {
     const DefInt i_level = ptr_grid_info->GetGridLevel();
-    if (i_level > 0 && time_step_level % 2 == 0) {
-        amrproject::GridInfoInterface *coarse = ptr_grid_manager_->vec_ptr_grid_info_.at(i_level - 1).get();
+    if (i_level == 0) return;
+    amrproject::GridInfoInterface *coarse = ptr_grid_manager_->vec_ptr_grid_info_.at(i_level - 1).get();
+    if (time_step_level % 2 == 0) {
         ptr_grid_info->TransferInfoFromCoarseGrid(sfbitset_aux, amrproject::NodeBitStatus::kNodeStatusCoarse2FineGhost_,
                                                   *coarse);
         ptr_grid_info->TransferInfoToCoarseGrid(sfbitset_aux, amrproject::NodeBitStatus::kNodeStatusCoarse2FineGhost_,
                                                 coarse);
+    } else if (time_interpolation_) {
+        ptr_grid_info->TransferInfoFromCoarseGrid(sfbitset_aux, amrproject::NodeBitStatus::kNodeStatusCoarse2FineGhost_,
+                                                  *coarse);
     }
 }
 

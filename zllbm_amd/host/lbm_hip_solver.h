@@ -124,6 +124,9 @@ class SolverLbmHip : public amrproject::SolverInterface {
                                     const ELbmBoundaryConditionScheme scheme, const std::vector<DefReal> &values = {});
     // storage form of the populations on the device
     void SetPrecision(const int precision) { precision_ = precision; }
+    // refill fine ghost layers after odd substeps too, from the mean of the coarse level's states
+    // before and after its last step (zlh_set_time_interpolation; off is the reference's schedule)
+    void SetTimeInterpolation(const bool on) { time_interpolation_ = on; }
     // immersed-boundary markers of one level (coordinates in background units)
     void SetImmersedBoundary(DefInt level, const std::vector<std::array<DefReal, 3>> &xyz,
                              const std::vector<DefReal> &area, const std::vector<std::array<DefReal, 3>> &velocity,
@@ -131,6 +134,7 @@ class SolverLbmHip : public amrproject::SolverInterface {
 
     const DefInt k0NumQ_;
     bool bool_forces_ = false;
+    bool time_interpolation_ = false;
 
     // SolverInterface
     std::string GetSolverMethod() override;
