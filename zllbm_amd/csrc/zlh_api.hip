@@ -1097,12 +1097,15 @@ zlh_status zlh_create(const zlh_solver_desc *desc, zlh_ctx **out) {
 
 void zlh_destroy(zlh_ctx *ctx) {
     if (!ctx) return;
+    // every stream drains before the device buffers are freed
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (hipStream_t s : ctx->lstream)
+        if (s) (void)hipStreamSynchronize(s);
     for (auto &e : ctx->events) { (void)hipEventDestroy(e.start); (void)hipEventDestroy(e.stop); }
     ctx->drop_graphs();
     ctx->lv.clear();
     for (hipStream_t s : ctx->lstream)
-        if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); }
+        if (s) (void)hipStreamDestroy(s);
     for (auto *v : {&ctx->ev_step, &ctx->ev_xfer, &ctx->ev_join})
         for (hipEvent_t e : *v)
             if (e) (void)hipEventDestroy(e);
