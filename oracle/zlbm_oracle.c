@@ -848,6 +848,47 @@ static int zo_region_axis(int64_t start, int L, int64_t lo, int64_t hi, int per_
     return index_min;
 }
 
+/*
+ * One axis of FindNodesInPeriodicRegionCornerOverlap (grid/sfbitset_aux.cpp:581-850 (2D),
+ * 2893-3825 (3D)), the region walk of the immersed boundary in a periodic domain
+ * (immersed_boundary.cpp:250-256).  Unlike zo_region_axis, a walk reaching a periodic domain
+ * plane continues past its duplicate on the opposite side (lo -> hi - 1, hi -> lo + 1), and the
+ * duplicate is returned in twin[] at the entry of the plane node (-1 elsewhere): it receives the
+ * entry's spread force as well (immersed_boundary.cpp:341-363).  At a start node on a periodic
+ * maximum plane the reference gives the duplicate the weight of the next entry, and the
+ * duplicate rows of a region's first and last rows their neighbours' or none; here every
+ * duplicate takes its own entry's weight (the reference's weights at those places are not
+ * reproduced).  Returns the axis' bound on the valid half length.
+ */
+static int zo_region_axis_twin(int64_t start, int L, int64_t lo, int64_t hi, int per_lo, int per_hi,
+                               int64_t *seq, int64_t *twin) {
+    int index_min = L;
+    for (int k = 0; k < 2 * L; ++k) seq[k] = twin[k] = -1;
+    int64_t v = start;
+    for (int ix = 0; ix < L; ++ix) {            /* negative direction */
+        seq[L - ix - 1] = v;
+        if (v == lo) {
+            if (per_lo) { twin[L - ix - 1] = hi; v = hi; }
+            else { if (index_min > ix + 1) index_min = ix + 1; break; }
+        }
+        v = v - 1;
+    }
+    v = start;
+    if (v == hi) {
+        if (per_hi) { twin[L - 1] = lo; v = lo; }
+        else return 0;
+    }
+    for (int ix = 0; ix < L; ++ix) {            /* positive direction */
+        v = v + 1;
+        seq[L + ix] = v;
+        if (v == hi) {
+            if (per_hi) { twin[L + ix] = lo; v = lo; }
+            else { if (index_min > ix + 1) index_min = ix + 1; break; }
+        }
+    }
+    return index_min;
+}
+
 /* CalculateLagrangianInterpCoeff (grid/grid_interpolation.cpp:87-112) */
 static void zo_lagrange_coeff(int half, double *c0, double *c1) {
     int num = 2 * half, offset = half - 1;
@@ -1034,6 +1075,9 @@ static int zo_ib_level(zo_sim *s, int level) {
     double dx = 1. / (double)((uint32_t)1 << level);   /* background spacing 1 */
     unsigned char *touched = (unsigned char *)calloc(L->n ? L->n : 1, 1);
     int errs = 0;
+    /* DomainInfo::bool_periodic_domain_: any periodic face selects the overlap walk */
+    int periodic = 0;
+    for (int d = 1; d <= 2 * dims; ++d) periodic |= s->p.bc_scheme[d] == ZO_BC_PERIODIC;
     for (int64_t m = 0; m < ib->n; ++m) {      /* marker order: index order */
         double iref[3], dis[3];
         uint32_t idx[3] = {0, 0, 0};
@@ -1043,17 +1087,24 @@ static int zo_ib_level(zo_sim *s, int level) {
             dis[a] = iref[a] - idx[a];
         }
         int Ls = ib->stencil_dis;
-        int64_t seq[3][8];
+        int64_t seq[3][8], twin[3][8];
         int valid = Ls;
         for (int a = 0; a < dims; ++a) {
             int64_t lo = (int64_t)s->p.dom_min[a] << level, hi = (int64_t)s->p.dom_max[a] << level;
-            int v = zo_region_axis(idx[a], Ls, lo, hi, 0, 0, seq[a]);
+            int v = periodic ? zo_region_axis_twin(idx[a], Ls, lo, hi, zo_periodic_axis(s, a, 0),
+                                                   zo_periodic_axis(s, a, 1), seq[a], twin[a])
+                             : zo_region_axis(idx[a], Ls, lo, hi, 0, 0, seq[a]);
+            if (!periodic)
+                for (int k = 0; k < 2 * Ls; ++k) twin[a][k] = -1;
             if (v < valid) valid = v;
         }
+        for (int a = dims; a < 3; ++a)
+            for (int k = 0; k < 8; ++k) { seq[a][k] = 0; twin[a][k] = -1; }
         double (*stencil)(double) = valid == 2 ? zo_stencil_two : zo_stencil_one;
         int n1 = 2 * valid;
         int64_t nodes[512];
         double delta[512];
+        int entry_off[512][3];   /* the entry's offsets (ox, oy, oz) */
         int cnt = 0;
         double uref[3] = {0, 0, 0};
         int nz = dims == 3 ? n1 : 1;
@@ -1067,6 +1118,7 @@ static int zo_ib_level(zo_sim *s, int level) {
                     int ox = ix - valid;
                     double dxw = stencil(fabs(dis[0] - ox - 1));
                     int64_t c[3] = {seq[0][Ls + ox], seq[1][Ls + oy], dims == 3 ? seq[2][Ls + oz] : 0};
+                    entry_off[cnt][0] = ox; entry_off[cnt][1] = oy; entry_off[cnt][2] = dims == 3 ? oz : 0;
                     if (c[0] < 0 || c[1] < 0 || c[2] < 0) { ++errs; nodes[cnt] = -1; delta[cnt++] = 0; continue; }
                     uint64_t key = zo_encode(dims, (uint32_t)c[0], (uint32_t)c[1], (uint32_t)c[2]);
                     int64_t j = zo_find(L, key);
@@ -1096,6 +1148,39 @@ static int zo_ib_level(zo_sim *s, int level) {
             if (j < 0) continue;
             double dt = delta[k] * area;
             for (int a = 0; a < dims; ++a) L->force[j * 3 + a] += Fm[a] * dt;
+        }
+        if (periodic) {
+            /* duplicates of the stencil nodes on periodic planes: every combination of an entry's
+             * coordinate or its twin per axis, other than the entry itself, in entry order */
+            for (int k = 0; k < cnt; ++k) {
+                if (nodes[k] < 0) continue;
+                int64_t opt[3][2];
+                int nopt[3];
+                for (int a = 0; a < 3; ++a) {
+                    int o = entry_off[k][a];
+                    opt[a][0] = a < dims ? seq[a][Ls + o] : 0;
+                    nopt[a] = 1;
+                    if (a < dims && twin[a][Ls + o] >= 0) opt[a][nopt[a]++] = twin[a][Ls + o];
+                }
+                for (int cz = 0; cz < nopt[2]; ++cz)
+                    for (int cy = 0; cy < nopt[1]; ++cy)
+                        for (int cx = 0; cx < nopt[0]; ++cx) {
+                            if (cx == 0 && cy == 0 && cz == 0) continue;
+                            uint64_t key = zo_encode(dims, (uint32_t)opt[0][cx], (uint32_t)opt[1][cy],
+                                                     (uint32_t)opt[2][cz]);
+                            int64_t j = zo_find(L, key);
+                            if (j < 0) { ++errs; continue; }
+                            if (!touched[j]) {
+                                double F[3];
+                                L->force[j * 3] = L->force[j * 3 + 1] = L->force[j * 3 + 2] = 0.;
+                                zo_all_forces(s, L->force + j * 3, F);
+                                zo_macro_force(s, L->dt, L->f + j * q, F, &L->rho[j], L->u + j * 3);
+                                touched[j] = 1;
+                            }
+                            double dt = delta[k] * area;
+                            for (int a = 0; a < dims; ++a) L->force[j * 3 + a] += Fm[a] * dt;
+                        }
+            }
         }
     }
     free(touched);

@@ -128,6 +128,41 @@ def test_ib_cylinder_fp64(stencil):
     assert np.count_nonzero(np.abs(fo).sum(1)) > 0 and np.count_nonzero(np.abs(fs).sum(1)) == np.count_nonzero(np.abs(fo).sum(1))
 
 
+@pytest.mark.parametrize("dims,stencil", [(2, 2), (2, 1), (3, 2)])
+def test_ib_periodic_domain_fp64(dims, stencil):
+    """Immersed boundary in a periodic domain (the overlap walk, immersed_boundary.cpp:250-256,
+    341-363): markers whose stencils wrap across periodic planes, at corners and past a maximum
+    plane. Populations, marker forces and node forces (duplicated plane nodes included) against the
+    oracle after 10 steps."""
+    from tests.test_oracle_amr_ib import _periodic_field
+    cells = (16, 12) if dims == 2 else (12, 10, 8)
+    grid = g.uniform_grid(dims, cells)
+    kw = dict(nu0=0.02, bc=sc.all_bc(dims, g.BC_PERIODIC), collision=(hip.MRT,), has_force=True)
+    if dims == 2:
+        mk = np.array([[1.3, 6.2], [cells[0] + 0.7, 3.4], [8.0, cells[1] + 0.9], [1.2, 1.1],
+                       [cells[0] + 1.2, 5.5], [1.0, 7.3]])
+    else:
+        mk = np.array([[1.3, 5.2, 4.4], [cells[0] + 0.7, 1.4, 3.3], [6.0, cells[1] + 0.9, cells[2] + 0.6],
+                       [1.2, 1.1, 1.3], [cells[0] + 1.2, 5.5, 2.5]])
+    area = np.linspace(0.3, 0.7, len(mk))
+    phys = (grid.coords(0).astype(float) - 1.0) % np.asarray(cells, float)
+    init = _periodic_field(phys, cells)
+    o = Oracle(grid, **kw)
+    s = hip.Solver(grid, precision=hip.FP64, **kw)
+    o.set_ib(0, mk, area, np.zeros_like(mk), stencil_dis=stencil)
+    s.set_ib(0, mk, area, np.zeros_like(mk), stencil_dis=stencil)
+    s.build()
+    o.init_macro(0, *init)
+    s.init_macro(0, *init)
+    o.time_marching(10)
+    s.time_marching(10)
+    assert_close(s.f(0), o.f(0), rtol=1e-11)
+    np.testing.assert_allclose(s.ib_force(0), o.ib_force(0), rtol=1e-9, atol=1e-14)
+    fs, fo = s.node_forces(0)[:, :dims], o.node_force(0)
+    np.testing.assert_allclose(fs, fo, rtol=1e-9, atol=1e-15)
+    assert np.abs(fo).max() > 1e-4
+
+
 def test_ib_uniform_flow_force():
     grid = g.uniform_grid(2, (40, 24))
     U = (0.03, -0.01)

@@ -266,3 +266,67 @@ def test_time_interp_accuracy_against_uniform_fine():
     print("fine-level L2(u) vs uniform fine: reference schedule %.3e, time interpolation %.3e" % (err[False], err[True]))
     assert err[False] < 1.2e-2 and err[True] < 1.5e-2
     assert err[False] != err[True]
+
+
+def _periodic_field(phys, cells, seed=3):
+    """A smooth periodic density and velocity at physical coordinates phys in [0, cells)."""
+    rng = np.random.default_rng(seed)
+    dims = phys.shape[1]
+    u = np.zeros_like(phys)
+    for a in range(dims):
+        for _ in range(3):
+            k = rng.integers(1, 3, size=dims)
+            ph = rng.uniform(0, 2 * np.pi)
+            u[:, a] += 0.01 * np.sin((2 * np.pi * phys / np.asarray(cells, float)) @ k + ph)
+    rho = 1.0 + 0.01 * np.cos(2 * np.pi * phys[:, 0] / cells[0])
+    return rho, u
+
+
+@pytest.mark.parametrize("dims,stencil", [(2, 2), (2, 1), (3, 2)])
+def test_ib_periodic_translation_invariance(dims, stencil):
+    """Immersed boundary in a periodic domain (the overlap walk of
+    FindNodesInPeriodicRegionCornerOverlap, immersed_boundary.cpp:250-256, 341-363): markers whose
+    stencils wrap across the periodic planes, at corners, on a minimum plane and past a maximum
+    plane, against the same markers and flow translated by half the box, where no stencil wraps.
+    Marker forces agree, every node force agrees with the node it is translated to, and the
+    duplicated plane nodes carry equal forces."""
+    cells = (16, 12) if dims == 2 else (12, 10, 8)
+    grid = g.uniform_grid(dims, cells)
+    kw = dict(nu0=0.02, bc=sc.all_bc(dims, g.BC_PERIODIC), collision=(1,), has_force=True)
+    xyz = grid.coords(0).astype(float)
+    phys = (xyz - 1.0) % np.asarray(cells, float)
+    if dims == 2:
+        mk = np.array([[1.3, 6.2], [cells[0] + 0.7, 3.4], [8.0, cells[1] + 0.9], [1.2, 1.1],
+                       [cells[0] + 1.2, 5.5], [1.0, 7.3]])
+    else:
+        mk = np.array([[1.3, 5.2, 4.4], [cells[0] + 0.7, 1.4, 3.3], [6.0, cells[1] + 0.9, cells[2] + 0.6],
+                       [1.2, 1.1, 1.3], [cells[0] + 1.2, 5.5, 2.5]])
+    area = np.linspace(0.3, 0.7, len(mk))
+    shift = np.asarray(cells, float) / 2
+    out = []
+    for s in (np.zeros(dims), shift):
+        o = Oracle(grid, **kw)
+        o.init_macro(0, *_periodic_field((phys - s) % np.asarray(cells, float), cells))
+        m = (mk - 1.0 + s) % np.asarray(cells, float) + 1.0 if s.any() else mk
+        o.set_ib(0, m, area, np.zeros_like(m), stencil_dis=stencil)
+        o.advance_level(0, 1)
+        out.append((o.ib_force(0), o.node_force(0)))
+    (fa, na), (fb, nb) = out
+    np.testing.assert_allclose(fa, fb, rtol=1e-11, atol=1e-15)
+    # node of a at physical p <-> node of b at p + shift (either duplicate)
+    key_b = {tuple(np.round(p).astype(int)): i for i, p in enumerate(phys)}
+    idx = np.array([key_b[tuple(np.round((p + shift) % np.asarray(cells, float)).astype(int))] for p in phys])
+    np.testing.assert_allclose(na, nb[idx], rtol=1e-10, atol=1e-16)
+    assert np.abs(na).max() > 1e-4
+    # a node on a minimum plane and its duplicate on the maximum plane carry the same force
+    node_of = {tuple(c): i for i, c in enumerate(xyz.astype(int))}
+    for a in range(dims):
+        lo = np.nonzero(xyz[:, a] == 1)[0]
+        tw = xyz[lo].astype(int)
+        tw[:, a] = cells[a] + 1
+        hi = np.array([node_of[tuple(t)] for t in tw])
+        np.testing.assert_array_equal(na[lo], na[hi])
+        assert np.abs(na[lo]).max() > 0
+    # the spread force over physical nodes (each duplicate set counted once) totals sum F_m area_m
+    phys_once = np.all(xyz <= np.asarray(cells), axis=1)
+    np.testing.assert_allclose(na[phys_once].sum(0), (fa * area[:, None]).sum(0), rtol=1e-12, atol=1e-16)

@@ -366,6 +366,43 @@ inline int region_axis(int64_t start, int L, int64_t lo, int64_t hi, bool per_lo
     return index_min;
 }
 
+// one axis of FindNodesInPeriodicRegionCornerOverlap (grid/sfbitset_aux.cpp:581-850,
+// 2893-3825), the immersed boundary's region walk in a periodic domain
+// (immersed_boundary.cpp:250-256): a walk reaching a periodic domain plane continues past its
+// duplicate on the opposite side (lo -> hi - 1, hi -> lo + 1), and the duplicate is returned in
+// twin[] at the plane node's entry (-1 elsewhere); it receives that entry's spread force too
+// (immersed_boundary.cpp:341-363).  Every duplicate takes its own entry's weight; the reference's
+// other weights at a start node on a maximum plane and at the region's first and last rows are
+// not reproduced.
+inline int region_axis_twin(int64_t start, int L, int64_t lo, int64_t hi, bool per_lo, bool per_hi, int64_t *seq,
+                            int64_t *twin) {
+    int index_min = L;
+    for (int k = 0; k < 2 * L; ++k) seq[k] = twin[k] = -1;
+    int64_t v = start;
+    for (int ix = 0; ix < L; ++ix) {
+        seq[L - ix - 1] = v;
+        if (v == lo) {
+            if (per_lo) { twin[L - ix - 1] = hi; v = hi; }
+            else { index_min = std::min(index_min, ix + 1); break; }
+        }
+        v = v - 1;
+    }
+    v = start;
+    if (v == hi) {
+        if (per_hi) { twin[L - 1] = lo; v = lo; }
+        else return 0;
+    }
+    for (int ix = 0; ix < L; ++ix) {
+        v = v + 1;
+        seq[L + ix] = v;
+        if (v == hi) {
+            if (per_hi) { twin[L + ix] = lo; v = lo; }
+            else { index_min = std::min(index_min, ix + 1); break; }
+        }
+    }
+    return index_min;
+}
+
 // CalculateLagrangianInterpCoeff (grid/grid_interpolation.cpp:87-112)
 inline void lagrange_coeff(int half, double *c0, double *c1) {
     const int num = 2 * half, offset = half - 1;
@@ -492,9 +529,9 @@ inline zlh_status plan_ib(zlh_ctx *ctx, int level) {
     const IbMarkers &M = L.ib;
     if (M.n == 0) return ZLH_OK;
     const int dims = ctx->d.dims;
-    for (int d = 1; d <= 2 * dims; ++d)
-        if (ctx->d.bc_scheme[d] == ZLH_BC_PERIODIC)
-            return ctx->fail(ZLH_EINVAL, "immersed boundary in a periodic domain is not supported");
+    // DomainInfo::bool_periodic_domain_: any periodic face selects the overlap walk
+    bool periodic = false;
+    for (int d = 1; d <= 2 * dims; ++d) periodic = periodic || ctx->d.bc_scheme[d] == ZLH_BC_PERIODIC;
     const int Ls = M.stencil_dis;
     P.support = 1;
     for (int a = 0; a < dims; ++a) P.support *= 2 * Ls;
@@ -504,14 +541,19 @@ inline zlh_status plan_ib(zlh_ctx *ctx, int level) {
     const double dx = 1. / (double)(1u << level), dt = dx;
     for (int64_t m = 0; m < M.n; ++m) {
         double dis[3];
-        int64_t idx[3], seq[3][8];
+        int64_t idx[3], seq[3][8], twin[3][8];
         int valid = Ls;
+        for (int a = 0; a < 3; ++a)
+            for (int k = 0; k < 8; ++k) { seq[a][k] = 0; twin[a][k] = -1; }
         for (int a = 0; a < dims; ++a) {
             const double iref = M.xyz[m * 3 + a] / dx;
             idx[a] = (int64_t)(iref + 1e-6);
             dis[a] = iref - (double)idx[a];
             const int64_t lo = (int64_t)ctx->d.dom_min[a] << level, hi = (int64_t)ctx->d.dom_max[a] << level;
-            valid = std::min(valid, region_axis(idx[a], Ls, lo, hi, false, false, seq[a]));
+            const bool pl = ctx->d.bc_scheme[1 + 2 * a] == ZLH_BC_PERIODIC;
+            const bool ph = ctx->d.bc_scheme[2 + 2 * a] == ZLH_BC_PERIODIC;
+            valid = std::min(valid, periodic ? region_axis_twin(idx[a], Ls, lo, hi, pl, ph, seq[a], twin[a])
+                                             : region_axis(idx[a], Ls, lo, hi, false, false, seq[a]));
         }
         double (*stencil)(double) = valid == 2 ? stencil_two : stencil_one;
         const int n1 = 2 * valid, nz = dims == 3 ? n1 : 1;
@@ -535,6 +577,36 @@ inline zlh_status plan_ib(zlh_ctx *ctx, int level) {
                     L.aux_mark[j] |= kAuxRecorded | kAuxDeferAlways;   // forced before its collision
                 }
             }
+        }
+        if (periodic) {
+            // duplicates on periodic planes: every combination of an entry's coordinate or its
+            // twin per axis other than the entry itself, with the entry's weight
+            e = 0;
+            for (int iz = 0; iz < nz; ++iz)
+                for (int iy = 0; iy < n1; ++iy)
+                    for (int ix = 0; ix < n1; ++ix, ++e) {
+                        const int o[3] = {ix - valid, iy - valid, dims == 3 ? iz - valid : 0};
+                        int64_t opt[3][2];
+                        int nopt[3];
+                        for (int a = 0; a < 3; ++a) {
+                            opt[a][0] = a < dims ? seq[a][Ls + o[a]] : 0;
+                            nopt[a] = 1;
+                            if (a < dims && twin[a][Ls + o[a]] >= 0) opt[a][nopt[a]++] = twin[a][Ls + o[a]];
+                        }
+                        for (int cz = 0; cz < nopt[2]; ++cz)
+                            for (int cy = 0; cy < nopt[1]; ++cy)
+                                for (int cx = 0; cx < nopt[0]; ++cx) {
+                                    if (cx == 0 && cy == 0 && cz == 0) continue;
+                                    const int64_t c[3] = {opt[0][cx], opt[1][cy], opt[2][cz]};
+                                    const int64_t j = find_key(L.keys, encode_xyz(dims, c));
+                                    if (j < 0)
+                                        return ctx->fail(ZLH_EINVAL, "marker %lld duplicate node absent at level %d",
+                                                         (long long)m, level);
+                                    P.dup_node.push_back(j);
+                                    P.dup_entry.push_back(m * P.support + e);
+                                    L.aux_mark[j] |= kAuxRecorded | kAuxDeferAlways;
+                                }
+                    }
         }
         P.scale[m] = dims == 3 ? M.area[m] / dt / dx / dx : M.area[m] / dt / dx;
     }
@@ -716,13 +788,19 @@ inline zlh_status upload_ib(zlh_ctx *ctx, int level) {
     zlh_status s;
     T.support = P.support;
     T.stiffness = L.ib.stiffness;
-    std::vector<int32_t> ai(P.node.size(), -1), reset;
+    std::vector<int32_t> ai(P.node.size(), -1), dup_ai(P.dup_node.size(), -1), reset;
     std::vector<uint8_t> seen(L.naux, 0);
     for (size_t k = 0; k < P.node.size(); ++k) {
         if (P.node[k] < 0) continue;
         ai[k] = L.node_aux_h[P.node[k]];
         if (ai[k] < 0) return ctx->fail(ZLH_EDEVICE, "immersed boundary node not deferred");
         if (!seen[ai[k]]) { seen[ai[k]] = 1; reset.push_back(ai[k]); }
+    }
+    // duplicates on periodic planes are reset and receive spread terms, but interpolate nothing
+    for (size_t k = 0; k < P.dup_node.size(); ++k) {
+        dup_ai[k] = L.node_aux_h[P.dup_node[k]];
+        if (dup_ai[k] < 0) return ctx->fail(ZLH_EDEVICE, "immersed boundary node not deferred");
+        if (!seen[dup_ai[k]]) { seen[dup_ai[k]] = 1; reset.push_back(dup_ai[k]); }
     }
     // supported nodes in aux order, so the reset's and the spread's row accesses coalesce
     std::sort(reset.begin(), reset.end());
@@ -735,18 +813,24 @@ inline zlh_status upload_ib(zlh_ctx *ctx, int level) {
     std::vector<int32_t> pos(L.naux, -1), begin(reset.size() + 1, 0), marker;
     std::vector<double> coef;
     for (size_t r = 0; r < reset.size(); ++r) pos[reset[r]] = (int32_t)r;
-    for (int32_t a : ai)
-        if (a >= 0) ++begin[pos[a] + 1];
+    // terms: (node, marker, table entry) of every stencil entry and every duplicate, ordered by
+    // marker per node (a node takes at most one term per marker)
+    struct Term { int32_t a; int64_t m, k; };
+    std::vector<Term> terms;
+    for (size_t k = 0; k < ai.size(); ++k)
+        if (ai[k] >= 0) terms.push_back({ai[k], (int64_t)(k / (size_t)P.support), (int64_t)k});
+    for (size_t k = 0; k < dup_ai.size(); ++k)
+        terms.push_back({dup_ai[k], P.dup_entry[k] / P.support, P.dup_entry[k]});
+    std::stable_sort(terms.begin(), terms.end(), [](const Term &x, const Term &y) { return x.m < y.m; });
+    for (const Term &t : terms) ++begin[pos[t.a] + 1];
     for (size_t r = 0; r < reset.size(); ++r) begin[r + 1] += begin[r];
     marker.resize(begin.back());
     coef.resize(begin.back());
     std::vector<int32_t> fill(begin.begin(), begin.end() - 1);
-    for (size_t k = 0; k < ai.size(); ++k) {
-        if (ai[k] < 0) continue;
-        const int32_t e = fill[pos[ai[k]]]++;
-        const size_t m = k / (size_t)P.support;
-        marker[e] = (int32_t)m;
-        coef[e] = P.weight[k] * P.scale[m];
+    for (const Term &t : terms) {
+        const int32_t e = fill[pos[t.a]]++;
+        marker[e] = (int32_t)t.m;
+        coef[e] = P.weight[t.k] * P.scale[t.m];
     }
     if ((s = upload(ctx, T.spread_begin, begin))) return s;
     if ((s = upload(ctx, T.spread_marker, marker))) return s;

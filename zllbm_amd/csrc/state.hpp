@@ -103,6 +103,9 @@ struct IbPlan {
     std::vector<double> weight;
     std::vector<double> scale;
     int support = 0;
+    // periodic domains: duplicates of stencil nodes on periodic planes and the table entry
+    // (marker x support + entry) whose weight each takes in the spread
+    std::vector<int64_t> dup_node, dup_entry;
 };
 
 // Roles of a node on the deferred ("aux") path.  Its post-stream values are kept in the aux array
