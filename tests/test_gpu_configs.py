@@ -112,6 +112,24 @@ def test_config2_reduced_fp64_vs_oracle():
     assert np.max(np.abs(o.ib_force(2))) > 1e-4        # the cylinder is loaded
 
 
+def test_config2_generated_grid_fp64_vs_oracle():
+    """The cylinder channel (128 x 64 background cells, cylinder of 16 finest cells) on the grid of
+    the geometry-driven generator (tracking markers on level 2, kInAndOut extension with the
+    reference example's counts): 100 background steps, populations of every level and the
+    Lagrangian forces against the oracle."""
+    _threads()
+    W = workloads.cylinder_amr_generated(cells=(128, 64), diameter_fine=16)
+    assert all(n > 1000 for n in W.nodes)
+    s = gpu_solver(W, hip.FP64)
+    o = oracle_solver(W)
+    s.time_marching(100)
+    o.time_marching(100)
+    for l in range(3):
+        assert_close(s.f(l), o.f(l), rtol=1e-10)
+    np.testing.assert_allclose(s.ib_force(2), o.ib_force(2), rtol=1e-8, atol=1e-12)
+    assert np.max(np.abs(o.ib_force(2))) > 1e-4
+
+
 def test_config2_full_fp32_tracks_fp64():
     """The full 512 x 256 3-level cylinder channel: FP32 shifted populations against FP64 in
     lockstep over 500 background steps."""

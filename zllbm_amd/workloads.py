@@ -8,6 +8,8 @@
   exceeds the 80th percentile of |vorticity| over the box (percentile from a 64^3 sample).
   base 32, levels 4 is the 256^3-equivalent configuration.
 * ``cavity(n)``: D2Q9 SRT lid-driven cavity, n x n cells, Re 100 on the lid speed 0.1, FP64 at rest.
+* ``cylinder_amr_generated()``: the same channel and cylinder on a grid from the geometry-driven
+  generator (zllbm_amd.gridgen) with the reference example's extension counts.
 * ``cylinder_amr()``: D2Q9 MRT channel of 512 x 256 background cells with two nested refinement
   boxes around a cylinder of 32 finest cells diameter (101 direct-forcing markers on the finest
   level), Re 100 on the diameter and the mean inflow speed 0.05; parabolic inflow and outflow
@@ -147,6 +149,38 @@ def cylinder_amr(cells=(512, 256), diameter_fine=32, u_in=0.05, re=100.0):
     wl = Workload("D2Q9 MRT channel %dx%d, 3 levels, cylinder D=%d finest cells, Re %g, parabolic inflow" % (
         cells + (diameter_fine, re)), grid,
         dict(nu0=nu, bc=_channel_bc(2, u_peak), bc_profile=prof, collision=(1,), has_force=True), init)
+    wl.ib = IbSet(levels - 1, xy, area)
+    return wl
+
+
+def cylinder_amr_generated(cells=(512, 256), diameter_fine=32, u_in=0.05, re=100.0, levels=3,
+                           extend=((16, 8, 8, 8), 8)):
+    """The cylinder channel of cylinder_amr on a grid from the geometry-driven generator
+    (zllbm_amd.gridgen): the markers at the finest level are the tracking geometry, kInAndOut
+    extension with the reference example's counts (x +16 / -8, y +-8 outside, 8 inside;
+    examples/lbm/immersed_boundary/cylinder2d/cylinder2d.cpp:52-58), flood fill from the centre."""
+    from zllbm_amd import gridgen
+    u_peak = 1.5 * u_in
+    d_bg = diameter_fine / (1 << (levels - 1))
+    cx, cy = cells[0] / 4.0 + 1.0, cells[1] / 2.0 + 1.0
+    xy, area = _geometry().circle_markers((cx + 0.13, cy + 0.07), d_bg / 2.0, 1.0 / (1 << (levels - 1)))
+    (xp, xn, yp, yn), inner = extend
+    geo = gridgen.GeometrySpec(level=levels - 1, vertices=xy, extend_type="in_and_out", x_pos=[xp] * levels,
+                               x_neg=[xn] * levels, y_pos=[yp] * levels, y_neg=[yn] * levels,
+                               inner=[inner] * levels, flood_fill_origin=(cx + 0.13, cy + 0.07))
+    grid = gridgen.generate(2, cells, levels - 1, [geo]).grid
+    nu = u_in * d_bg / re
+    init = []
+    for l in range(levels):
+        y = grid.coords(l)[:, 1].astype(np.float64) / (1 << l)
+        t = (y - 1.0) / cells[1]
+        u = np.zeros((len(y), 2))
+        u[:, 0] = u_peak * 4.0 * t * (1.0 - t)
+        init.append((np.ones(len(y)), u))
+    prof = {g.XMIN: 1, g.XMAX: 1}
+    wl = Workload("D2Q9 MRT channel %dx%d, %d levels from the geometry-driven generator, cylinder D=%d finest cells, "
+                  "Re %g, parabolic inflow" % (cells + (levels, diameter_fine, re)), grid,
+                  dict(nu0=nu, bc=_channel_bc(2, u_peak), bc_profile=prof, collision=(1,), has_force=True), init)
     wl.ib = IbSet(levels - 1, xy, area)
     return wl
 
