@@ -391,3 +391,52 @@ def test_moment_transform_switch_takes_effect():
     d = np.abs(out[0] - out[1]).max()
     print("VALU vs matrix-core transform after 10 steps: max |df| %.2e" % d)
     assert 0 < d < 1e-5
+
+
+def test_learned_closure_a_posteriori():
+    """A-posteriori comparison at the training conditions: the 256^3 periodic box of the
+    decaying-turbulence workload (Fourier modes, finest-level viscosity, Smagorinsky Cs 0.05, FP32)
+    run 1,024 steps, box-filtered by 2, 4 and 8; coarse LES runs on 128^3, 64^3 and 32^3 from the
+    filtered initial field, with the learned closure, Smagorinsky Cs 0.1 and 0.05 and no model, to
+    the same time. Velocity L2 against the filtered fine field. Measured: learned 1.67e-2,
+    7.35e-2, 2.02e-1; Smagorinsky 0.1 1.69e-2, 7.29e-2, 2.01e-1; no model 1.56e-2, 7.08e-2,
+    2.00e-1 -- the forms differ by at most 8 % (3 % at filters 4 and 8) because the fine run is
+    nearly resolved here, and the learned closure stays within 2 % of the Smagorinsky form it
+    replaces."""
+    import os
+    from zllbm_amd import workloads
+    from zllbm_amd.fields import fourier_modes
+    base, levels, T = 32, 4, 1024
+    fine = base << (levels - 1)
+    nu_f = 1e-4 * (1 << (levels - 1)) * (1 << (levels - 1))
+    model = closure.ClosureModel.load(os.path.join(os.path.dirname(closure.__file__), "data", "closure_turbulence.npz"))
+    gf = g.uniform_grid(3, (fine,) * 3)
+    u0 = fourier_modes((base,) * 3).velocity((gf.coords(0).astype(np.float64) - 1.0) / (1 << (levels - 1)))
+    s = hip.Solver(gf, precision=hip.FP32_SHIFTED, collision=(hip.MRT,), nu0=nu_f, bc=workloads.periodic_bc(3),
+                   les=hip.LES_SMAGORINSKY, cs=0.05).build()
+    s.init_macro(0, np.ones(len(u0)), u0)
+    uf0 = closure.unique_field(gf, u0)
+    del u0
+    s.time_marching(T)
+    ufT = closure.unique_field(gf, s.macros(0)[1])
+    del s
+    for r in (2, 4, 8):
+        n = fine // r
+        gc = g.uniform_grid(3, (n,) * 3)
+        idx = tuple(((gc.coords(0).astype(np.int64) - 1) % n).T)
+        init_u, want = closure.box_filter(uf0, r)[idx], closure.box_filter(ufT, r)[idx]
+        err = {}
+        for name, kw in (("learned", dict(les=hip.LES_MLP, cs=0.1)), ("smag0.1", dict(les=hip.LES_SMAGORINSKY, cs=0.1)),
+                         ("smag0.05", dict(les=hip.LES_SMAGORINSKY, cs=0.05)), ("none", dict(les=hip.LES_NONE))):
+            c = hip.Solver(gc, precision=hip.FP32_SHIFTED, collision=(hip.MRT,), nu0=nu_f / r,
+                           bc=workloads.periodic_bc(3), **kw).build()
+            if kw["les"] == hip.LES_MLP:
+                c.mlp_set(model)
+            c.init_macro(0, np.ones(len(init_u)), init_u)
+            c.time_marching(T // r)
+            uc = c.macros(0)[1]
+            err[name] = float(np.sqrt(np.sum((uc - want) ** 2) / np.sum(want ** 2)))
+            del c
+        print("filter %d: velocity L2 against the filtered fine run %s" % (r, err))
+        assert err["learned"] < 1.02 * err["smag0.1"]
+        assert max(err.values()) < 1.1 * min(err.values())

@@ -630,7 +630,7 @@ void k_stream_collide_mlp(int64_t nbrick, int64_t naux, const uint64_t *__restri
     // the first pull goes out before the network is staged, so that their latencies overlap
     uint32_t miss = 0;
     if (b < nbrick) miss = pull_to_lds<D, T>(t, fin, b, threadIdx.x & 63, buf);
-    stage_mlp_ops(ops, p.mlp->ops);
+    stage_mlp_ops<((VAR & 15) >> 1) == 4, mrt_on_mfma<C, VAR>()>(ops, p.mlp->ops);
     if (b >= nbrick) return;                 // uniform per wave; no barrier follows
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // the first pull has landed
 #pragma unroll 1
@@ -710,7 +710,7 @@ void k_aux_collide(BrickLevel<T> lv, const int32_t *__restrict__ list, int64_t n
     const MlpOps *ops = nullptr;
     if constexpr (kWave) {
         __shared__ MlpOps s_ops;
-        stage_mlp_ops(s_ops, p.mlp->ops);
+        stage_mlp_ops<((VAR & 15) >> 1) == 4, mrt_on_mfma<C, VAR>()>(s_ops, p.mlp->ops);
         ops = &s_ops;
     }
     C F[3] = {C(p.const_force[0]), C(p.const_force[1]), C(p.const_force[2])};

@@ -4,6 +4,7 @@
 // inputs lie inside the training range and falls back to Smagorinsky elsewhere.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cstddef>
 #include <cstdint>
 #include <type_traits>
 
@@ -36,8 +37,6 @@ struct alignas(16) MlpOps {
     uint16_t w2a[3][64][8];
     uint16_t h1a[64][8];
     uint16_t h2a[3][64][8];
-    float mfwd[kMrtSteps][64];
-    float mbwd[kMrtSteps][64];
     float w3r[2][16];
     float mean[kMlpF];
     float inv_std[kMlpF];
@@ -46,6 +45,8 @@ struct alignas(16) MlpOps {
     float nu_scale;
     float b3;
     float pad[2];
+    float mfwd[kMrtSteps][64];   // last: only the kernels with the transform on the matrix cores stage them
+    float mbwd[kMrtSteps][64];
 };
 
 // device-resident closure: operand images, parameters (flat order W1, b1, W2, b2, w3, b3), input
@@ -359,12 +360,25 @@ __device__ __forceinline__ float mlp_tau_t_wave(const MlpOps &o, const float *pi
 }
 
 // copy of the operand images into LDS by one workgroup (followed by a barrier)
+// Copies into LDS the parts of the operand images a kernel reads: one operand form's network
+// images (BF16 w1a, w2a or FP16 h1a, h2a), the scalars, and with MRT the transform's images; the
+// other parts of dst stay unwritten and are never read.
+template <bool FP16, bool MRT>
 __device__ __forceinline__ void stage_mlp_ops(MlpOps &dst, const MlpOps &src) {
-    constexpr int n16 = (int)(sizeof(MlpOps) / 16);
-    static_assert(sizeof(MlpOps) % 16 == 0, "operand images are copied in 16-byte units");
+    static_assert(sizeof(MlpOps) % 16 == 0 && offsetof(MlpOps, w3r) % 16 == 0 && offsetof(MlpOps, mfwd) % 16 == 0,
+                  "operand images are copied in 16-byte units");
+    static_assert(offsetof(MlpOps, w2a) == offsetof(MlpOps, w1a) + sizeof(src.w1a) &&
+                  offsetof(MlpOps, h2a) == offsetof(MlpOps, h1a) + sizeof(src.h1a), "each form's images are adjacent");
+    constexpr int img0 = (int)((FP16 ? offsetof(MlpOps, h1a) : offsetof(MlpOps, w1a)) / 16);
+    constexpr int nimg = (int)((sizeof(src.w1a) + sizeof(src.w2a)) / 16);
+    constexpr int sc0 = (int)(offsetof(MlpOps, w3r) / 16);
+    constexpr int nsc = (int)(((MRT ? sizeof(MlpOps) : offsetof(MlpOps, mfwd)) - offsetof(MlpOps, w3r)) / 16);
     const uint4 *s = reinterpret_cast<const uint4 *>(&src);
     uint4 *d = reinterpret_cast<uint4 *>(&dst);
-    for (int i = threadIdx.x; i < n16; i += blockDim.x) d[i] = s[i];
+    for (int i = threadIdx.x; i < nimg + nsc; i += blockDim.x) {
+        const int j = i < nimg ? img0 + i : sc0 + (i - nimg);
+        d[j] = s[j];
+    }
     __syncthreads();
 }
 
@@ -372,7 +386,7 @@ __device__ __forceinline__ void stage_mlp_ops(MlpOps &dst, const MlpOps &src) {
 template <typename OT>
 __global__ __launch_bounds__(256) void k_mlp_eval_wave(const MlpDev *m, const float *X, int64_t n, float *y) {
     __shared__ MlpOps ops;
-    stage_mlp_ops(ops, m->ops);
+    stage_mlp_ops<std::is_same<OT, OperandsF16>::value, false>(ops, m->ops);
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     float xn[kMlpF];
     for (int i = 0; i < kMlpF; ++i) xn[i] = k < n ? X[k * kMlpF + i] : 0.f;
