@@ -81,6 +81,33 @@ def warm(s, warmup):
     s.time_marching(warmup)
 
 
+def cpp_drop_in(hip, W, model, mlp_operands, steps, warmup=4):
+    """The same step through the C++ plugin classes under the AmrManager::TimeMarching loop (the
+    example driver, a process of its own on the same GPU): wall ms per background step with whole
+    background steps replayed as graphs (SetGraphReplay, the default) and with each substep
+    launched by its own AdvancingAtCurrentTime call."""
+    import shutil
+    import tempfile
+    from zllbm_amd import host_driver
+    kw = dict(W.solver, les=hip.LES_MLP)
+    res = {"loop": "AmrManager::TimeMarching over GridInfoLbmHip / SolverLbmHip (zllbm_lbm_driver)",
+           "steps": steps, "warmup": warmup}
+    for replay in (True, False):
+        d = tempfile.mkdtemp(prefix="zllbm_driver_")
+        try:
+            proc, _ = host_driver.run(W.grid, kw, warmup + steps, d, collision="mrt", precision="fp32", model=model,
+                                      graph_replay=replay, init=W.init, warmup=warmup, output=False,
+                                      extra=("mlp_operands = " + mlp_operands,), timeout=600)
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+        if proc.returncode != 0:
+            raise RuntimeError("host driver failed: " + proc.stderr[-2000:])
+        ms = host_driver.ms_per_step(proc)
+        res["graph_replay_ms_per_step" if replay else "per_substep_ms_per_step"] = round(ms, 4)
+        log("C++ drop-in (%s): %.3f ms per step" % ("graph replay" if replay else "per-substep launches", ms))
+    return res
+
+
 def timed(s, W, warmup, steps, kernel_steps=20):
     """Wall time of `steps` background steps (graph replays) after `warmup`, then the finest
     level's fused-kernel time from per-launch events over `kernel_steps` direct-launch steps."""
@@ -259,6 +286,7 @@ def main():
         del sx
         log("FP32 learned, MRT transform on the matrix cores: %.3f ms per step, finest-level kernel %.4f ms" % (
             wallx / c_steps * 1e3, kx_ms / max(kx_n, 1)))
+        cpp = cpp_drop_in(hip, W, model, args.mlp_operands, c_steps)
         # accuracy: FP32 against FP64, same closure, in lockstep (the learned closure, and for
         # reference the plain Smagorinsky pair, whose difference is FP32 rounding alone)
         curve = l2_curve(hip, W, hip.LES_MLP, model, args.l2_steps, args.l2_every)
@@ -274,6 +302,7 @@ def main():
                                  "finest_kernel_ms": round(kx_ms / max(kx_n, 1), 4),
                                  "vector_units_finest_kernel_ms": round(avg_s * 1e3, 4),
                                  "kernel": "k_stream_collide_mlp (level %d), v_mfma_f32_32x32x2f32 transform" % finest},
+                             "cpp_drop_in": cpp,
                              "l2_u_vs_fp64_mean": float(np.mean([c[1] for c in curve])),
                              "l2_u_vs_fp64_final": curve[-1][1], "l2_rho_vs_fp64_final": curve[-1][2],
                              "l2_u_curve": [[t, round(u, 8)] for t, u, _ in curve],

@@ -1,24 +1,9 @@
-"""Helpers that run the C++ host driver (zllbm_amd/zllbm_lbm_driver) on a grid."""
-import os
-import subprocess
-
+"""Helpers for the C++ host driver's tests: the reference's checkpoint record order and files by
+its own loops; running the driver is zllbm_amd.host_driver's."""
 import numpy as np
 
 from zllbm_amd import grid as g
-
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-DRIVER = os.path.join(ROOT, "zllbm_amd", "zllbm_lbm_driver")
-_DIR = {1: "xmin", 2: "xmax", 3: "ymin", 4: "ymax", 5: "zmin", 6: "zmax"}
-_SCHEME = {1: "bounce_back", 2: "periodic", 3: "nee"}
-
-
-def write_closure(path, model):
-    """Closure file of the driver: parameters, mean, spread, minimum, maximum, output scale."""
-    vals = [np.asarray(a, np.float64).ravel() for a in (model.params, model.mean, model.std, model.fmin, model.fmax)]
-    with open(path, "w") as fh:
-        for v in vals:
-            fh.write(" ".join(repr(float(np.float32(x))) for x in v) + "\n")
-        fh.write(repr(float(model.nu_scale)) + "\n")
+from zllbm_amd.host_driver import DRIVER, run, ms_per_step, write_closure, write_params  # noqa: F401
 
 
 def read_checkpoint(path, q):
@@ -139,52 +124,3 @@ def reference_interface_nodes(grid):
                                     out += struct.pack("<Qi", k, index[(l, ind, layer)])
                                     break
     return out
-
-
-def write_params(path, kw, dims, collision="srt", precision="fp64", closure_path=None, extra=()):
-    lines = ["viscosity = %r" % kw["nu0"], "rho = 1.0",
-             "velocity = " + " ".join(["0.0"] * dims), "collision = " + collision, "precision = " + precision]
-    if kw.get("les") == 2:
-        lines += ["les = learned", "cs = %r" % kw.get("cs", 0.1), "closure = " + closure_path]
-    elif kw.get("les"):
-        lines += ["les = smagorinsky", "cs = %r" % kw.get("cs", 0.1), "les_tau_mode = %d" % kw.get("les_tau_mode", 1)]
-    for d, (scheme, val) in kw["bc"].items():
-        lines.append("bc.%s = %s %s" % (_DIR[d], _SCHEME[scheme], " ".join(repr(float(v)) for v in val)))
-    lines += list(extra)
-    with open(path, "w") as fh:
-        fh.write("\n".join(lines) + "\n")
-
-
-def run(grid, kw, steps, workdir, collision="srt", precision="fp64", model=None, checkpoint_in=None,
-        checkpoint_out=None, vtu_out=None, checkpoint_dir_in=None, checkpoint_dir_out=None, time_interp=False):
-    gpath = os.path.join(workdir, "grid.bin")
-    ppath = os.path.join(workdir, "params.txt")
-    opath = os.path.join(workdir, "out.bin")
-    cpath = os.path.join(workdir, "closure.txt")
-    g.write_grid_file(grid, gpath)
-    if model is not None:
-        write_closure(cpath, model)
-    extra = []
-    if checkpoint_in:
-        extra.append("checkpoint_in = " + checkpoint_in)
-    if checkpoint_out:
-        extra.append("checkpoint_out = " + checkpoint_out)
-    if vtu_out:
-        extra.append("vtu_out = " + vtu_out)
-    if checkpoint_dir_in:
-        extra.append("checkpoint_dir_in = " + checkpoint_dir_in)
-    if checkpoint_dir_out:
-        extra.append("checkpoint_dir_out = " + checkpoint_dir_out)
-    if time_interp:
-        extra.append("time_interpolation = 1")
-    write_params(ppath, kw, grid.dims, collision, precision, cpath, extra)
-    proc = subprocess.run([DRIVER, gpath, ppath, str(steps), opath], capture_output=True, text=True, timeout=300)
-    if proc.returncode != 0:
-        return proc, None
-    q = 9 if grid.dims == 2 else 19
-    out = []
-    with open(opath, "rb") as fh:
-        for _ in grid.levels:
-            n = int(np.fromfile(fh, dtype=np.int64, count=1)[0])
-            out.append(np.fromfile(fh, dtype=np.float64, count=n * q).reshape(n, q))
-    return proc, out

@@ -218,7 +218,8 @@ def test_vorticity_refined_turbulence_fp32_velocity_l2():
 
 
 def test_graph_replay_matches_direct_launches():
-    """Replayed two-step graphs and direct launches, each with the levels on streams of their own,
+    """Replayed graphs (of two background steps, and of one for an odd count) and direct
+    launches, each with the levels on streams of their own,
     against direct launches on one stream, over calls of odd and even length with manual substeps
     and downloads (synchronisations) in between: every replay bit-identical."""
     from zllbm_amd import workloads

@@ -50,6 +50,34 @@ def test_driver_time_interpolation_matches_oracle(tmp_path):
         assert np.max(np.abs(out[l] - ref)) < 1e-11 * np.max(np.abs(ref))
 
 
+@pytest.mark.gpu
+def test_driver_graph_replay_matches_per_substep(tmp_path):
+    """SetGraphReplay: the first AdvancingAtCurrentTime of each background step replays the whole
+    step (zlh_time_marching; graphs of one step, for both buffer parities of level 0) and the
+    finer levels' calls only check the order.  Its populations equal those of per-substep
+    launches bit for bit, from an initial field given in the init file, and the oracle's from
+    the same field; the driver reports the timed steps' wall time."""
+    grid, kw = sc.amr2d(levels=3)
+    init = [sc.level_fields(grid, l, 0.05) for l in range(grid.max_level + 1)]
+    outs = []
+    for replay in (True, False):
+        d = tmp_path / ("replay" if replay else "substeps")
+        d.mkdir()
+        proc, out = host_driver.run(grid, kw, 5, str(d), collision="mrt", graph_replay=replay, init=init, warmup=1)
+        assert proc.returncode == 0, proc.stderr
+        assert host_driver.ms_per_step(proc) > 0
+        outs.append(out)
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(a, b)
+    o = Oracle(grid, collision=(1,), **kw)
+    for l, (rho, u) in enumerate(init):
+        o.init_macro(l, rho, u)
+    o.time_marching(5)
+    for l in range(grid.max_level + 1):
+        ref = o.f(l)
+        assert np.max(np.abs(outs[0][l] - ref)) < 1e-11 * np.max(np.abs(ref))
+
+
 def _cavity3d(n=10, lid=0.05):
     from zllbm_amd import grid as g
     grid = g.uniform_grid(3, (n, n, n))

@@ -1395,9 +1395,9 @@ zlh_status march_launches(zlh_ctx *ctx, int nsteps) {
 
 // capture two background steps into an executable graph (the host-side state advances as if
 // they ran; the caller launches the graph at once)
-zlh_status capture_two_steps(zlh_ctx *ctx, hipGraphExec_t *exec) {
+zlh_status capture_steps(zlh_ctx *ctx, int nsteps, hipGraphExec_t *exec) {
     ZLH_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    const zlh_status s = march_launches(ctx, 2);
+    const zlh_status s = march_launches(ctx, nsteps);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
     if (s) {
@@ -1419,36 +1419,40 @@ zlh_status zlh_time_marching(zlh_ctx *ctx, int nsteps) {
     if (!ctx) return ZLH_EINVAL;
     if (!ctx->built) return ctx->fail(ZLH_ESTATE, "not built");
     if (nsteps < 0) return ctx->fail(ZLH_EINVAL, "negative step count");
-    // launch-bound multi-level steps replay a captured graph of two background steps; timing
-    // (per-launch events) runs the launches directly
+    // launch-bound multi-level steps replay captured graphs: of two background steps while two
+    // remain (level 0's buffer index comes back to where it was), of one step for an odd last
+    // one.  Timing (per-launch events) runs the launches directly.
     if (!ctx->use_graphs || ctx->timing) return march_launches(ctx, nsteps);
     int t = 0;
-    while (nsteps - t >= 2) {
-        // buffer index and the aux collision's pending node set of every level
-        uint64_t key = 0;
+    while (t < nsteps) {
+        const int n = nsteps - t >= 2 ? 2 : 1;
+        // steps per graph, and the buffer index and the aux collision's pending node set of every level
+        uint64_t key = n == 1 ? 1ull << 63 : 0;
         for (int l = 0; l <= ctx->d.max_level; ++l)
             key |= (uint64_t)(ctx->lv[l].cur & 1) << l | (uint64_t)(ctx->lv[l].pending & 3) << (16 + 2 * l);
         auto it = ctx->graph_exec.find(key);
         if (it == ctx->graph_exec.end()) {
             hipGraphExec_t g = nullptr;
-            zlh_status s = capture_two_steps(ctx, &g);   // advances the host state
+            zlh_status s = capture_steps(ctx, n, &g);   // advances the host state
             if (s) return s;
             ctx->graph_exec[key] = g;
             ZLH_TRY(hipGraphLaunch(g, ctx->stream));
         } else {
             ZLH_TRY(hipGraphLaunch(it->second, ctx->stream));
-            // host state after two steps: same buffer indices, post-stream data current, and each
-            // level's last substep the last of a background step (even on refined levels)
+            // host state after the steps: post-stream data current, each level's last substep the
+            // last of a background step (even on refined levels), and the buffer index flipped
+            // once per substep: refined levels take an even number, level 0 one per step
             for (int l = 0; l <= ctx->d.max_level; ++l) {
                 LevelState &L = ctx->lv[l];
                 L.fresh = false;
                 L.last_tsl = 1 << l;
                 L.pending = l > 0 ? kPendingTransfer : kPendingPlain;
             }
+            ctx->lv[0].cur ^= n & 1;
         }
-        t += 2;
+        t += n;
     }
-    return t < nsteps ? march_launches(ctx, nsteps - t) : ZLH_OK;
+    return ZLH_OK;
 }
 
 zlh_status zlh_graphs_enable(zlh_ctx *ctx, int enable) {

@@ -71,12 +71,12 @@ struct GridNode {
     virtual ~GridNode() {}
 };
 
-// one output variable of a grid's nodes (io/vtk_writer_manager.h, OutputNodeVariableInfoInterface):
-// its name and component count
-class OutputNodeVariableInfoInterface {
- public:
-    std::string output_name_;
+// one output variable of a grid's nodes (grid/grid_info_interface.h:93-101): its name and
+// component count
+struct OutputNodeVariableInfoInterface {
     DefInt variable_dims_ = 1;
+    std::string output_name_;
+    virtual void WriteNodeVariable(const GridNode &grid_node) {}
     virtual ~OutputNodeVariableInfoInterface() {}
 };
 
@@ -90,6 +90,7 @@ class GridInfoInterface {
  protected:
     DefInt i_level_ = 0;
     std::weak_ptr<SolverInterface> ptr_solver_;
+    SFBitsetAuxInterface *ptr_sfbitset_aux_ = nullptr;
     GridManagerInterface *ptr_parent_grid_manager_ = nullptr;
     DefInt k0NumFine2CoarseLayer_ = 3;
     DefInt k0NumCoarse2FineLayer_ = 2;
@@ -99,6 +100,8 @@ class GridInfoInterface {
     void SetGridLevel(const DefInt i_level) { i_level_ = i_level; }
     void SetPtrSolver(const std::weak_ptr<SolverInterface> &ptr_solver) { ptr_solver_ = ptr_solver; }
     std::weak_ptr<SolverInterface> GetPtrToSolver() const { return ptr_solver_; }
+    void SetPtrSFBitsetAux(SFBitsetAuxInterface *const p) { ptr_sfbitset_aux_ = p; }
+    SFBitsetAuxInterface *GetPtrSFBitsetAux() const { return ptr_sfbitset_aux_; }
     void SetPtrToParentGridManager(GridManagerInterface *p) { ptr_parent_grid_manager_ = p; }
     GridManagerInterface *GetPtrToParentGridManager() const { return ptr_parent_grid_manager_; }
     DefInt GetNumFine2CoarseLayer() const { return k0NumFine2CoarseLayer_; }
@@ -123,8 +126,7 @@ class GridInfoInterface {
     // node record sizes (grid_info_interface.h:376-377)
     virtual int GetSizeOfGridNodeInfoForMpiCommunication() const { return sizeof(DefInt); }
     virtual int GetSizeOfGridNodeInfoForCheckPoint() const { return GetSizeOfGridNodeInfoForMpiCommunication(); }
-    // output variables (grid_info_interface.h:278-281)
-    std::vector<std::string> output_variable_name_;
+    // output variables (grid_info_interface.h:278-279)
     std::vector<std::unique_ptr<OutputNodeVariableInfoInterface>> output_variables_;
     virtual void SetupOutputVariables() {}
 
@@ -178,19 +180,31 @@ class SolverCreatorInterface {
     virtual ~SolverCreatorInterface() {}
 };
 
-// The parts of GridManagerInterface the time step reads: dimensions, levels, the per-level
-// grid information and the solver (grid/grid_manager.h:75-190).
+// The parts of GridManagerInterface the time step reads (grid/grid_manager.h:80-160): dimensions,
+// levels, the per-level grid information, the solvers and the background node index range.  The
+// reference's class is abstract (GridManager2D / GridManager3D implement the index range); this
+// one stores the range, set with SetIndexRangeOfBackgroundNode.
 class GridManagerInterface {
  public:
     const DefInt k0GridDims_;
-    DefInt k0MaxLevel_ = 0;
-    std::vector<std::shared_ptr<GridInfoInterface>> vec_ptr_grid_info_;
     std::vector<std::shared_ptr<SolverInterface>> vec_ptr_solver_;
-    // background node index range (k0MinIndexOfBackgroundNode_ / k0MaxIndexOfBackgroundNode_)
-    std::vector<DefAmrLUint> k0MinIndexOfBackgroundNode_, k0MaxIndexOfBackgroundNode_;
+    std::vector<std::shared_ptr<GridInfoInterface>> vec_ptr_grid_info_;
     explicit GridManagerInterface(DefInt dims) : k0GridDims_(dims) {}
+    void SetMaxLevel(const DefInt max_level) { k0MaxLevel_ = max_level; }
     DefInt GetMaxLevel() const { return k0MaxLevel_; }
+    virtual std::vector<DefAmrLUint> GetMinIndexOfBackgroundNodeArrAsVec() const { return min_index_; }
+    virtual std::vector<DefAmrLUint> GetMaxIndexOfBackgroundNodeArrAsVec() const { return max_index_; }
+    void SetIndexRangeOfBackgroundNode(const std::vector<DefAmrLUint> &lo, const std::vector<DefAmrLUint> &hi) {
+        min_index_ = lo;
+        max_index_ = hi;
+    }
     virtual ~GridManagerInterface() {}
+
+ protected:
+    DefInt k0MaxLevel_ = 0;
+
+ private:
+    std::vector<DefAmrLUint> min_index_, max_index_;
 };
 
 // Coarse-to-fine multi time stepping (grid/grid_manager.cpp:997-1024): a level step is

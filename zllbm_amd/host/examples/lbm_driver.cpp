@@ -12,11 +12,18 @@
 //               checkpoint_dir_in / checkpoint_dir_out = <folder> (the same with the reference's
 //               checkpoint folder: loadings, grid_nodes, interface_nodes),
 //               vtu_out = <prefix> (VTU files of every level at the end),
-//               bc.xmin ... bc.zmax = <bounce_back|periodic|nee> [u v w])
+//               bc.xmin ... bc.zmax = <bounce_back|periodic|nee> [u v w],
+//               time_interpolation = <0|1>, graph_replay = <0|1> (default 1: SetGraphReplay),
+//               init = <file> (initial density and velocity: per level an int64 node count, then
+//               the densities and the velocity components node by node, as doubles, in key order),
+//               warmup = <steps> (of the background steps, the first ones, left out of the timing))
+//   stdout:     the solver method, "ms_per_step <t>" (wall time of the background steps after the
+//               warm-up, device synchronised at both ends) and "done <steps> steps"
 //   closure file (les = learned): whitespace-separated numbers, the 1409 network parameters,
 //               then 9 each of input mean, spread, training minimum and maximum, then the output
 //               scale
-//   output:     per level the node count and the post-stream populations in key order
+//   output:     per level the node count and the post-stream populations in key order ("-": none)
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -63,9 +70,10 @@ int main(int argc, char **argv) {
             dims == 2 ? SolverCreatorLbmD2Q9Hip().CreateSolver() : SolverCreatorLbmD3Q19Hip().CreateSolver();
         auto *lbm = dynamic_cast<SolverLbmHip *>(solver.get());
         amrproject::GridManagerInterface gm(dims);
-        gm.k0MaxLevel_ = nlev - 1;
-        gm.k0MinIndexOfBackgroundNode_.assign(dmin, dmin + dims);
-        gm.k0MaxIndexOfBackgroundNode_.assign(dmax, dmax + dims);
+        amrproject::SFBitsetAuxInterface sfbitset_aux;
+        gm.SetMaxLevel(nlev - 1);
+        gm.SetIndexRangeOfBackgroundNode(std::vector<DefAmrLUint>(dmin, dmin + dims),
+                                         std::vector<DefAmrLUint>(dmax, dmax + dims));
         gm.vec_ptr_solver_.push_back(solver);
         solver->SetPtrToGridManager(&gm);
         std::vector<std::vector<uint64_t>> level_keys(nlev);
@@ -74,6 +82,7 @@ int main(int argc, char **argv) {
             gi->SetGridLevel(l);
             gi->SetPtrSolver(solver);
             gi->SetPtrToParentGridManager(&gm);
+            gi->SetPtrSFBitsetAux(&sfbitset_aux);
             gi->InitialGridInfo(dims);
             int64_t n;
             rd(in, &n);
@@ -108,6 +117,7 @@ int main(int argc, char **argv) {
         }
         if (!in) throw ZlhError("truncated grid file");
         // parameters (ReadAndSetupSolverParameters keys, lbm_solver_interface.cpp:49-117)
+        int warmup = 0;
         std::ifstream pin(argv[2]);
         std::string line;
         while (std::getline(pin, line)) {
@@ -133,6 +143,21 @@ int main(int argc, char **argv) {
                 lbm->SetPrecision(t == "fp32" ? ZLH_FP32_SHIFTED : t == "bf16" ? ZLH_BF16_SHIFTED : ZLH_FP64);
             }
             else if (key == "time_interpolation") { int v; val >> v; lbm->SetTimeInterpolation(v != 0); }
+            else if (key == "graph_replay") { int v; val >> v; lbm->SetGraphReplay(v != 0); }
+            else if (key == "warmup") { val >> warmup; }
+            else if (key == "init") {
+                std::string path; val >> path;
+                std::ifstream iin(path, std::ios::binary);
+                for (int l = 0; l < nlev; ++l) {
+                    int64_t n;
+                    if (!rd(iin, &n) || n != (int64_t)level_keys[l].size())
+                        throw ZlhError("init file " + path + ": level " + std::to_string(l) + " node count");
+                    std::vector<double> rho(n), u((size_t)n * dims);
+                    if (!rd(iin, rho.data(), rho.size()) || !rd(iin, u.data(), u.size()))
+                        throw ZlhError("init file " + path + " is short");
+                    lbm->SetInitialMacros(l, rho, u);
+                }
+            }
             else if (key == "mlp_operands") {
                 std::string t; val >> t;
                 lbm->SetMlpOperands(t == "bf16" ? ZLH_MLP_OPERANDS_BF16 : ZLH_MLP_OPERANDS_FP16);
@@ -181,12 +206,22 @@ int main(int argc, char **argv) {
         if (!checkpoint_dir_in.empty()) lbm->ReadCheckPoint(checkpoint_dir_in);
         std::cout << solver->GetSolverMethod() << std::endl;
         const int steps = std::atoi(argv[3]);
-        for (int t = 0; t < steps; ++t) TimeMarching(&gm, (DefAmrLUint)t);
+        if (warmup < 0 || warmup > steps) throw ZlhError("warm-up of " + std::to_string(warmup) + " steps out of " +
+                                                         std::to_string(steps));
+        for (int t = 0; t < warmup; ++t) TimeMarching(&gm, (DefAmrLUint)t);
+        lbm->BuildDevice();
+        if (zlh_synchronize(lbm->Context()) != ZLH_OK) throw ZlhError("zlh_synchronize failed");
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int t = warmup; t < steps; ++t) TimeMarching(&gm, (DefAmrLUint)t);
+        if (zlh_synchronize(lbm->Context()) != ZLH_OK) throw ZlhError("zlh_synchronize failed");
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (steps > warmup) std::cout << "ms_per_step " << ms / (steps - warmup) << std::endl;
         if (!checkpoint_out.empty()) lbm->WriteCheckPointGridNodes(checkpoint_out);
         if (!checkpoint_dir_out.empty()) lbm->WriteCheckPoint(checkpoint_dir_out);
         if (!vtu_out.empty()) lbm->WriteVtu(vtu_out);
-        std::ofstream out(argv[4], std::ios::binary);
-        for (int l = 0; l < nlev; ++l) {
+        std::ofstream out;
+        if (std::strcmp(argv[4], "-") != 0) out.open(argv[4], std::ios::binary);
+        for (int l = 0; l < nlev && out.is_open(); ++l) {
             lbm->CopyPopulationsToHost(l);
             const int64_t n = (int64_t)level_keys[l].size();
             out.write(reinterpret_cast<const char *>(&n), sizeof(n));

@@ -58,9 +58,10 @@ void GridInfoLbmHip::AdvancingAtCurrentTime(const amrproject::ETimeSteppingSchem
                                             const DefInt time_step_level, const DefReal time_step_current,
                                             amrproject::MpiManager *const, amrproject::CriterionManager *const) {
     SolverLbmHip *s = solver_of(*this);
-    amrproject::SFBitsetAuxInterface aux;
-    s->RunSolverOnGivenGrid(time_scheme, time_step_level, time_step_current, aux, this);
-    s->InformationFromGridOfDifferentLevel(time_step_level, aux, this);
+    if (s->AdvanceWithinBackgroundStep(i_level_, time_step_level)) return;
+    if (ptr_sfbitset_aux_ == nullptr) throw ZlhError("pointer to SFBitsetAux is nullptr");
+    s->RunSolverOnGivenGrid(time_scheme, time_step_level, time_step_current, *ptr_sfbitset_aux_, this);
+    s->InformationFromGridOfDifferentLevel(time_step_level, *ptr_sfbitset_aux_, this);
 }
 
 std::unique_ptr<amrproject::GridNode> GridInfoLbmHip::GridNodeCreator() const {
@@ -300,7 +301,7 @@ void SolverLbmHip::BuildDevice() {
     amrproject::GridManagerInterface &gm = *ptr_grid_manager_;
     zlh_solver_desc d{};
     d.dims = k0SolverDims_;
-    d.max_level = gm.k0MaxLevel_;
+    d.max_level = gm.GetMaxLevel();
     d.precision = precision_;
     d.compressible = 1;
     d.nu0 = k0LbmViscosity_;
@@ -323,11 +324,13 @@ void SolverLbmHip::BuildDevice() {
         d.bc_scheme[kv.first] = static_cast<int>(kv.second.boundary_scheme_);
         for (int a = 0; a < 3; ++a) d.bc_value[kv.first][a] = kv.second.values_[a];
     }
+    const std::vector<DefAmrLUint> lo = gm.GetMinIndexOfBackgroundNodeArrAsVec(),
+                                   hi = gm.GetMaxIndexOfBackgroundNodeArrAsVec();
     for (int a = 0; a < k0SolverDims_; ++a) {
-        if (a >= (int)gm.k0MinIndexOfBackgroundNode_.size() || a >= (int)gm.k0MaxIndexOfBackgroundNode_.size())
+        if (a >= (int)lo.size() || a >= (int)hi.size())
             throw ZlhError("background index range not set on the grid manager");
-        d.dom_min[a] = gm.k0MinIndexOfBackgroundNode_[a];
-        d.dom_max[a] = gm.k0MaxIndexOfBackgroundNode_[a];
+        d.dom_min[a] = lo[a];
+        d.dom_max[a] = hi[a];
     }
     zlh_status s = zlh_create(&d, &ctx_);
     if (s != ZLH_OK) {
@@ -335,7 +338,7 @@ void SolverLbmHip::BuildDevice() {
         throw ZlhError("zlh_create failed (" + std::to_string(s) + ")");
     }
     Check(zlh_set_time_interpolation(ctx_, time_interpolation_ ? 1 : 0), "zlh_set_time_interpolation");
-    for (int l = 0; l <= gm.k0MaxLevel_; ++l) {
+    for (int l = 0; l <= gm.GetMaxLevel(); ++l) {
         amrproject::GridInfoInterface &gi = *gm.vec_ptr_grid_info_.at(l);
         std::vector<std::pair<uint64_t, int32_t>> nodes;
         nodes.reserve(gi.map_grid_node_.size());
@@ -381,7 +384,7 @@ void SolverLbmHip::BuildDevice() {
     }
     // initial populations: node mirrors with f_ set, else feq(rho0, u0)
     Check(zlh_init_feq(ctx_), "zlh_init_feq");
-    for (int l = 0; l <= gm.k0MaxLevel_; ++l) {
+    for (int l = 0; l <= gm.GetMaxLevel(); ++l) {
         amrproject::GridInfoInterface &gi = *gm.vec_ptr_grid_info_.at(l);
         std::vector<std::pair<uint64_t, const GridNodeLbmHip *>> nodes;
         bool any = false;
@@ -403,6 +406,49 @@ void SolverLbmHip::BuildDevice() {
         }
         Check(zlh_upload_populations(ctx_, l, f.data()), "zlh_upload_populations");
     }
+    for (auto &kv : init_macros_)
+        Check(zlh_init_macros(ctx_, kv.first, kv.second.first.data(), kv.second.second.data()), "zlh_init_macros");
+}
+
+void SolverLbmHip::SetGraphReplay(const bool on) {
+    if (cycle_pos_ != 0) throw ZlhError("graph replay switched inside a background step");
+    graph_replay_ = on;
+}
+
+void SolverLbmHip::SetInitialMacros(DefInt level, const std::vector<DefReal> &rho, const std::vector<DefReal> &velocity) {
+    if (!ptr_grid_manager_ || level < 0 || level > ptr_grid_manager_->GetMaxLevel())
+        throw ZlhError("initial macros: bad level " + std::to_string(level));
+    const size_t n = ptr_grid_manager_->vec_ptr_grid_info_.at(level)->map_grid_node_.size();
+    if (rho.size() != n || velocity.size() != n * (size_t)k0SolverDims_)
+        throw ZlhError("initial macros of level " + std::to_string(level) + ": expected " + std::to_string(n) +
+                       " densities and " + std::to_string(n * k0SolverDims_) + " velocity components");
+    init_macros_[level] = {rho, velocity};
+    if (ctx_) Check(zlh_init_macros(ctx_, level, rho.data(), velocity.data()), "zlh_init_macros");
+}
+
+bool SolverLbmHip::AdvanceWithinBackgroundStep(DefInt level, DefInt time_step_level) {
+    if (!graph_replay_) return false;
+    if (!ctx_) BuildDevice();
+    const DefInt max_level = ptr_grid_manager_->GetMaxLevel();
+    if (cycle_order_.empty()) {
+        amrproject::MultiTimeSteppingC2F scheme(max_level);
+        cycle_order_ = scheme.k0TimeSteppingOrder_;
+    }
+    if (cycle_pos_ == 0) {
+        if (level != 0 || time_step_level != 1)
+            throw ZlhError("graph replay: a background step begins with level 0's substep 1, not level " +
+                           std::to_string(level) + "'s substep " + std::to_string(time_step_level));
+        Check(zlh_time_marching(ctx_, 1), "zlh_time_marching");
+        cycle_tsl_.assign(max_level + 1, 0);
+    } else if (level != cycle_order_[cycle_pos_]) {
+        throw ZlhError("graph replay: substep of level " + std::to_string(level) + " where the coarse-to-fine order has level " +
+                       std::to_string(cycle_order_[cycle_pos_]));
+    }
+    if (time_step_level != ++cycle_tsl_[level])
+        throw ZlhError("graph replay: level " + std::to_string(level) + " substep " + std::to_string(time_step_level) +
+                       " where " + std::to_string(cycle_tsl_[level]) + " is next");
+    if (++cycle_pos_ == cycle_order_.size()) cycle_pos_ = 0;
+    return true;
 }
 
 // SolverLbmInterface::RunSolverOnGivenGrid (lbm_solver_interface.cpp:246-262)
@@ -413,7 +459,6 @@ void SolverLbmHip::RunSolverOnGivenGrid(const amrproject::ETimeSteppingScheme, c
     Check(zlh_step_level(ctx_, ptr_grid_info->GetGridLevel(), time_step_level), "zlh_step_level");
 }
 
-// SolverLbmInterface::InformationFromGridOfDifferentLevel (lbm_solver_interface.cpp:224-238)
 // SolverLbmInterface::InformationFromGridOfDifferentLevel (lbm_solver_interface.cpp:224-238): after
 // the fine level's even substeps, the fine grid's TransferInfoFromCoarseGrid, then its
 // TransferInfoToCoarseGrid, with the coarse grid's information; with time interpolation also
@@ -460,7 +505,7 @@ void SolverLbmHip::WriteCheckPointGridNodes(const std::string &file_name) {
     amrproject::GridManagerInterface &gm = *ptr_grid_manager_;
     const int dims = k0SolverDims_;
     std::vector<CheckpointOrder> order;
-    for (DefInt l = 0; l <= gm.k0MaxLevel_; ++l) {
+    for (DefInt l = 0; l <= gm.GetMaxLevel(); ++l) {
         CopyPopulationsToHost(l);
         const uint32_t mask = (1u << l) - 1u;
         for (auto &kv : gm.vec_ptr_grid_info_.at(l)->map_grid_node_) {
@@ -479,11 +524,11 @@ void SolverLbmHip::WriteCheckPointGridNodes(const std::string &file_name) {
     out.write(reinterpret_cast<const char *>(&count), sizeof(count));
     out.write(reinterpret_cast<const char *>(&payload), sizeof(payload));
     // node forces of every level in key order (GridNodeLbm::force_)
-    std::vector<std::vector<uint64_t>> level_keys(gm.k0MaxLevel_ + 1);
-    std::vector<std::vector<double>> level_force(gm.k0MaxLevel_ + 1);
+    std::vector<std::vector<uint64_t>> level_keys(gm.GetMaxLevel() + 1);
+    std::vector<std::vector<double>> level_force(gm.GetMaxLevel() + 1);
     if (nforce) {
         if (nforce > 3) throw ZlhError("checkpoint: more than 3 node forces");
-        for (DefInt l = 0; l <= gm.k0MaxLevel_; ++l) {
+        for (DefInt l = 0; l <= gm.GetMaxLevel(); ++l) {
             for (auto &kv : gm.vec_ptr_grid_info_.at(l)->map_grid_node_) level_keys[l].push_back(kv.first.to_ullong());
             std::sort(level_keys[l].begin(), level_keys[l].end());
             level_force[l].resize(level_keys[l].size() * 3);
@@ -542,8 +587,8 @@ struct CellLoad {
 
 std::map<uint64_t, CellLoad> cell_loads(const amrproject::GridManagerInterface &gm, int dims) {
     std::map<uint64_t, CellLoad> cells;
-    const size_t nbytes = (size_t)(gm.k0MaxLevel_ + 1 + 7) / 8;
-    for (DefInt l = 0; l <= gm.k0MaxLevel_; ++l) {
+    const size_t nbytes = (size_t)(gm.GetMaxLevel() + 1 + 7) / 8;
+    for (DefInt l = 0; l <= gm.GetMaxLevel(); ++l) {
         const auto &gi = *gm.vec_ptr_grid_info_.at(l);
         for (const auto &kv : gi.map_grid_node_) {
             CellLoad &c = cells[kv.first.to_ullong() >> (dims * l)];
@@ -588,7 +633,7 @@ std::vector<char> SolverLbmHip::CheckPointLoadings() const {
     std::vector<char> b;
     put(b, (uint32_t)total);
     put(b, (uint32_t)cells.size());
-    put(b, (int32_t)gm.k0MaxLevel_);
+    put(b, (int32_t)gm.GetMaxLevel());
     for (const auto &c : cells) {
         if (c.second.nodes > INT32_MAX || c.second.interface_nodes > INT32_MAX || c.second.load > INT32_MAX)
             throw ZlhError("checkpoint: a cell's counts exceed int32");
@@ -608,7 +653,7 @@ std::vector<char> SolverLbmHip::CheckPointInterfaceNodes() const {
     std::map<std::tuple<int32_t, int32_t, int32_t, int32_t, int32_t>, int32_t> index;
     std::vector<char> table;
     int32_t nidx = 0;
-    for (DefInt l = 0; l <= gm.k0MaxLevel_; ++l)
+    for (DefInt l = 0; l <= gm.GetMaxLevel(); ++l)
         for (const auto &it : gm.vec_ptr_grid_info_.at(l)->map_ptr_interface_layer_info_)
             for (const auto &kind : layer_kinds(*it.second))
                 for (int32_t layer = 0; layer < (int32_t)kind.second->size(); ++layer) {
@@ -625,7 +670,7 @@ std::vector<char> SolverLbmHip::CheckPointInterfaceNodes() const {
     put(b, nidx);
     b.insert(b.end(), table.begin(), table.end());
     for (const auto &c : cell_loads(gm, dims))
-        for (DefInt l = 0; l <= gm.k0MaxLevel_; ++l) {
+        for (DefInt l = 0; l <= gm.GetMaxLevel(); ++l) {
             if (!(c.second.levels[l / 8] >> (l % 8) & 1u)) continue;
             const auto &gi = *gm.vec_ptr_grid_info_.at(l);
             for (uint64_t key : cell_keys(dims, c.first, l))
@@ -671,10 +716,10 @@ void SolverLbmHip::ReadCheckPointGridNodes(const std::string &file_name) {
     if (!in || payload != (int32_t)(sizeof(DefInt) + sizeof(double) * (k0NumQ_ + nforce)))
         throw ZlhError("checkpoint " + file_name + ": payload size does not match this solver's lattice and forces");
     size_t nodes = 0;
-    for (DefInt l = 0; l <= gm.k0MaxLevel_; ++l) nodes += gm.vec_ptr_grid_info_.at(l)->map_grid_node_.size();
+    for (DefInt l = 0; l <= gm.GetMaxLevel(); ++l) nodes += gm.vec_ptr_grid_info_.at(l)->map_grid_node_.size();
     if (count != nodes) throw ZlhError("checkpoint " + file_name + ": node count differs from the grid");
     std::vector<char> rec(sizeof(uint64_t) + sizeof(int32_t) + payload);
-    std::vector<std::vector<uint64_t>> seen(gm.k0MaxLevel_ + 1);
+    std::vector<std::vector<uint64_t>> seen(gm.GetMaxLevel() + 1);
     for (uint32_t i = 0; i < count; ++i) {
         if (!in.read(rec.data(), (std::streamsize)rec.size())) throw ZlhError("checkpoint " + file_name + " is truncated");
         uint64_t key;
@@ -682,7 +727,7 @@ void SolverLbmHip::ReadCheckPointGridNodes(const std::string &file_name) {
         std::memcpy(&key, rec.data(), sizeof(key));
         std::memcpy(&level, rec.data() + 8, sizeof(level));
         std::memcpy(&flag, rec.data() + 12, sizeof(flag));
-        if (level < 0 || level > gm.k0MaxLevel_) throw ZlhError("checkpoint " + file_name + ": bad level");
+        if (level < 0 || level > gm.GetMaxLevel()) throw ZlhError("checkpoint " + file_name + ": bad level");
         auto &map = gm.vec_ptr_grid_info_.at(level)->map_grid_node_;
         auto it = map.find(DefSFBitset(key));
         if (it == map.end()) throw ZlhError("checkpoint " + file_name + ": node not in the grid");
@@ -700,7 +745,7 @@ void SolverLbmHip::ReadCheckPointGridNodes(const std::string &file_name) {
             throw ZlhError("checkpoint " + file_name + ": a node has two records");
     }
     if (!ctx_) return;   // BuildDevice uploads the mirrors
-    for (DefInt l = 0; l <= gm.k0MaxLevel_; ++l) {
+    for (DefInt l = 0; l <= gm.GetMaxLevel(); ++l) {
         auto &map = gm.vec_ptr_grid_info_.at(l)->map_grid_node_;
         std::vector<std::pair<uint64_t, const GridNodeLbmHip *>> nodes_l;
         for (auto &kv : map) nodes_l.emplace_back(kv.first.to_ullong(), dynamic_cast<const GridNodeLbmHip *>(kv.second.get()));
@@ -712,6 +757,8 @@ void SolverLbmHip::ReadCheckPointGridNodes(const std::string &file_name) {
         }
         Check(zlh_upload_populations(ctx_, l, f.data()), "zlh_upload_populations");
     }
+    for (auto &kv : init_macros_)
+        Check(zlh_init_macros(ctx_, kv.first, kv.second.first.data(), kv.second.second.data()), "zlh_init_macros");
 }
 
 void SolverLbmHip::WriteVtu(const std::string &prefix) {
@@ -719,7 +766,7 @@ void SolverLbmHip::WriteVtu(const std::string &prefix) {
     amrproject::GridManagerInterface &gm = *ptr_grid_manager_;
     const int dims = k0SolverDims_;
     std::vector<std::string> pieces;
-    for (DefInt l = 0; l <= gm.k0MaxLevel_; ++l) {
+    for (DefInt l = 0; l <= gm.GetMaxLevel(); ++l) {
         CopyPopulationsToHost(l);
         auto &map = gm.vec_ptr_grid_info_.at(l)->map_grid_node_;
         std::vector<uint64_t> keys;
@@ -800,8 +847,8 @@ void SolverLbmHip::WriteVtu(const std::string &prefix) {
 
 // AmrManager::TimeMarching (amr_manager.cpp:279-300)
 void TimeMarching(amrproject::GridManagerInterface *gm, DefAmrLUint time_step_background) {
-    amrproject::MultiTimeSteppingC2F scheme(gm->k0MaxLevel_);
-    std::vector<DefInt> time_step_level(gm->k0MaxLevel_ + 1, 0);
+    amrproject::MultiTimeSteppingC2F scheme(gm->GetMaxLevel());
+    std::vector<DefInt> time_step_level(gm->GetMaxLevel() + 1, 0);
     for (DefInt i_level : scheme.k0TimeSteppingOrder_) {
         ++time_step_level[i_level];
         const DefReal t = scheme.GetCurrentTimeStep(i_level, time_step_background, time_step_level[i_level]);

@@ -2,23 +2,42 @@
 // of include/zllbm_hip.h.  Mirrors the reference's solver manager SolverLbmInterface and grid
 // information GridInfoLbmInteface (lbm_project/lbm_interface.h:140-474): the same setters, the
 // same plugin virtuals, and the node maps kept as host mirrors filled on request.
+//
+// Interface declarations: by default those of amr_interface.h (this directory).  With
+// ZLLBM_HIP_REFERENCE_HEADERS defined, the reference's own headers instead, found on its include
+// paths (source/, source/amr_project, source/lbm_project), so that the plugin links into the
+// reference's AmrManager with one declaration of each interface class.  The code below uses only
+// members that both declare alike.
 #pragma once
 #include <array>
 #include <map>
 #include <memory>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
+#ifdef ZLLBM_HIP_REFERENCE_HEADERS
+#include "grid/grid_manager.h"                              // GridManagerInterface, MultiTimeSteppingC2F
+#include "grid/grid_info_interface.h"                       // GridInfoInterface, GridNode
+#include "solver_info_interface.h"                          // SolverInterface
+#include "collision_model/lbm_collision_models.h"           // ELbmCollisionOperatorType
+#include "boundary_conditions/lbm_boundary_conditions.h"    // ELbmBoundaryConditionScheme
+#include "les_models/lbm_les_models.h"                      // ELbmLesModelType
+#else
 #include "amr_interface.h"
+#endif
 #include "zllbm_hip.h"
 
 namespace rootproject {
 namespace lbmproject {
 
+#ifndef ZLLBM_HIP_REFERENCE_HEADERS
+// collision_model/lbm_collision_models.h:21-25, boundary_conditions/lbm_boundary_conditions.h:22-27,
+// les_models/lbm_les_models.h:20-23
 enum class ELbmCollisionOperatorType { kLbmSrt, kLbmMrt, kUndefined };
 enum class ELbmBoundaryConditionScheme { kUndefined = 0, kBounceBack = 1, kPeriodic = 2, kNonEqExtrapolation = 3 };
-// kLearned: the learned sub-grid closure of zlh_mlp_set (no counterpart in the reference)
-enum class ELbmLesModelType { kSmagorinsky, kLearned, kUndefined };
+enum class ELbmLesModelType { kSmagorinsky, kUndefined };
+#endif
 
 // Fatal errors of the device library surface as this exception (the reference's
 // LogManager::LogError terminates; callers that want that behaviour let it propagate).
@@ -69,6 +88,8 @@ class GridInfoLbmHip : public amrproject::GridInfoInterface {
     // node record sizes (lbm_grid_interface.cpp:737-749, lbm_grid_interface_mpi.cpp:77-90)
     int GetSizeOfGridNodeInfoForMpiCommunication() const override;
     int GetSizeOfGridNodeInfoForCheckPoint() const override;
+    // names of the output variables (GridInfoLbmInteface::output_variable_name_, lbm_interface.h:262)
+    std::set<std::string> output_variable_name_ = {"rho", "velocity"};
     // the names in output_variable_name_ ("rho", "velocity") as output variables
     // (SetupOutputVariables, lbm_grid_interface.cpp:614-660); WriteVtu writes them
     void SetupOutputVariables() override;
@@ -102,10 +123,10 @@ class SolverLbmHip : public amrproject::SolverInterface {
     // LES (SetLesModel, les_models/lbm_les_models.cpp:17-41) with the Smagorinsky constant and the
     // relaxation-time mode exposed
     void SetLesModel(const ELbmLesModelType type) {
-        les_ = type == ELbmLesModelType::kSmagorinsky ? ZLH_LES_SMAGORINSKY
-             : type == ELbmLesModelType::kLearned     ? ZLH_LES_MLP
-                                                      : ZLH_LES_NONE;
+        les_ = type == ELbmLesModelType::kSmagorinsky ? ZLH_LES_SMAGORINSKY : ZLH_LES_NONE;
     }
+    // "smagorinsky" or "learned" (the learned sub-grid closure of zlh_mlp_set, which has no
+    // counterpart in the reference's ELbmLesModelType)
     void SetLesModel(const std::string &type);
     // the network of the learned closure (zlh_mlp_set: 1409 parameters, then 9 each of input mean,
     // spread, training minimum and maximum) and its output scale; Smagorinsky (physical tau mode,
@@ -127,6 +148,17 @@ class SolverLbmHip : public amrproject::SolverInterface {
     // refill fine ghost layers after odd substeps too, from the mean of the coarse level's states
     // before and after its last step (zlh_set_time_interpolation; off is the reference's schedule)
     void SetTimeInterpolation(const bool on) { time_interpolation_ = on; }
+    // whole background steps as one graph replay (default on): the first AdvancingAtCurrentTime
+    // of a background step (level 0) runs every level's substeps and exchanges of that step
+    // through zlh_time_marching, and the calls for the finer levels' substeps that follow, in
+    // the coarse-to-fine order (MultiTimeSteppingC2F), only check that order.  Off: each call
+    // launches its own substep and exchanges.  Results are identical; between the calls of one
+    // background step the device state is that of its end.
+    void SetGraphReplay(const bool on);
+    // initial macroscopic fields of a level (density, velocity with dims components per node),
+    // node by node in ascending Morton key order; the populations start at their equilibrium
+    // (zlh_init_macros) once the device is built
+    void SetInitialMacros(DefInt level, const std::vector<DefReal> &rho, const std::vector<DefReal> &velocity);
     // immersed-boundary markers of one level (coordinates in background units)
     void SetImmersedBoundary(DefInt level, const std::vector<std::array<DefReal, 3>> &xyz,
                              const std::vector<DefReal> &area, const std::vector<std::array<DefReal, 3>> &velocity,
@@ -197,6 +229,9 @@ class SolverLbmHip : public amrproject::SolverInterface {
     // Nodes are written in Morton key order.
     void WriteVtu(const std::string &prefix);
     zlh_ctx *Context() const { return ctx_; }
+    // AdvancingAtCurrentTime's graph-replay route: true when the call's substep is (or was, by
+    // the background step's first call) advanced as part of a whole background step
+    bool AdvanceWithinBackgroundStep(DefInt level, DefInt time_step_level);
 
  private:
     void Check(zlh_status s, const char *what) const;
@@ -220,6 +255,11 @@ class SolverLbmHip : public amrproject::SolverInterface {
         double stiffness = 2.;
     };
     std::map<DefInt, Ib> ib_;
+    std::map<DefInt, std::pair<std::vector<DefReal>, std::vector<DefReal>>> init_macros_;
+    bool graph_replay_ = true;
+    std::vector<DefInt> cycle_order_;   // levels of one background step's substeps, in order
+    size_t cycle_pos_ = 0;              // next substep of the current background step (0: none begun)
+    std::vector<DefInt> cycle_tsl_;     // substeps of each level seen in the current background step
     zlh_ctx *ctx_ = nullptr;
 };
 
