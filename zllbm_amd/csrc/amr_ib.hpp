@@ -52,13 +52,28 @@ __global__ __launch_bounds__(256) void k_c2f_sources(int64_t n_src, int64_t n_fb
     const int64_t n = n_src + n_fb;
     if (r < n_src) {
         const int64_t j = src_ai[r];
-        double f[Q], feq[Q], rho, u[3];
+        if constexpr (SHIFT) {
+            // FP32 / BF16 storage: in FP32 on the stored deviations f - w, whose equilibrium
+            // feq - w needs no cancellation
+            float f[Q], rho, drho, u[3];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) f[q] = double(aux_c[q * naux_c + j]) + (SHIFT ? Lattice<D>::w[q] : 0.);
-        macro_feq_double<D>(f, compressible, rho, u, feq);
+            for (int q = 0; q < Q; ++q) f[q] = float(aux_c[q * naux_c + j]);
+            macro<D, true, float>(f, nullptr, 0.f, 0, compressible, rho, drho, u);
+            const float usq = u[0] * u[0] + u[1] * u[1] + (D == 3 ? u[2] * u[2] : 0.f);
+            const float tr = (float)tau_ratio;
 #pragma unroll
-        for (int q = 0; q < Q; ++q)
-            tile[threadIdx.x * Q + q] = T(feq[q] + tau_ratio * (f[q] - feq[q]) - (SHIFT ? Lattice<D>::w[q] : 0.));
+            for (int q = 0; q < Q; ++q) {
+                const float feq = feq_stored<D, true, float>(q, rho, drho, u[0], u[1], u[2], usq, compressible);
+                tile[threadIdx.x * Q + q] = T(__builtin_fmaf(tr, f[q] - feq, feq));
+            }
+        } else {
+            double f[Q], feq[Q], rho, u[3];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) f[q] = double(aux_c[q * naux_c + j]);
+            macro_feq_double<D>(f, compressible, rho, u, feq);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) tile[threadIdx.x * Q + q] = T(feq[q] + tau_ratio * (f[q] - feq[q]));
+        }
     } else if (r < n) {
         const int64_t fi = fb_ai[r - n_src];
         T v[Q];
@@ -235,6 +250,20 @@ __global__ void k_f2c(int64_t n, const int32_t *coarse_ai, const int32_t *fine_a
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int64_t i = fine_ai[p], j = coarse_ai[p];
+    if constexpr (SHIFT) {
+        // FP32 / BF16 storage: in FP32 on the stored deviations f - w
+        const float rho = (float)fine_macro[i * 4], ux = (float)fine_macro[i * 4 + 1];
+        const float uy = (float)fine_macro[i * 4 + 2], uz = D == 3 ? (float)fine_macro[i * 4 + 3] : 0.f;
+        const float drho = (float)(fine_macro[i * 4] - 1.);
+        const float usq = ux * ux + uy * uy + (D == 3 ? uz * uz : 0.f);
+        const float tr = (float)tau_ratio;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const float feq = feq_stored<D, true, float>(q, rho, drho, ux, uy, uz, usq, compressible);
+            coarse_f[q * naux_c + j] = T(__builtin_fmaf(tr, float(fine_f[q * naux_f + i]) - feq, feq));
+        }
+        return;
+    }
     const double rho = fine_macro[i * 4], ux = fine_macro[i * 4 + 1], uy = fine_macro[i * 4 + 2];
     const double uz = D == 3 ? fine_macro[i * 4 + 3] : 0.;
     const double usq = ux * ux + uy * uy + (D == 3 ? uz * uz : 0.);
