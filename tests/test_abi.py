@@ -89,8 +89,8 @@ def test_bench_refuses_bf16_with_learned_closure():
 
 
 def test_level_size_limit_without_device():
-    """The FP32 learned closure's fused kernel forms 32-bit byte offsets of its pulls, so a level
-    holds at most 2^32 / (Q 64 4) bricks with it (883,011 in D3Q19); other settings keep the
+    """The FP32 learned closure's fused kernel forms 32-bit byte offsets of its pulls and of its
+    deferred-node stores, so a level holds fewer than 2^32 / (Q 64 4) bricks with it (883,011 in D3Q19); other settings keep the
     32-bit slot-index bound.  zlh_build refuses larger levels with this limit."""
     from zllbm_amd import grid as g
     grid = g.uniform_grid(3, (4, 4, 4))
@@ -99,9 +99,11 @@ def test_level_size_limit_without_device():
     assert hip.max_level_bricks(hip.make_desc(grid, nu0=0.01, precision=hip.FP32_SHIFTED)) == general
     assert hip.max_level_bricks(hip.make_desc(grid, nu0=0.01, precision=hip.FP64, les=hip.LES_MLP)) == general
     mlp = hip.max_level_bricks(hip.make_desc(grid, nu0=0.01, precision=hip.FP32_SHIFTED, les=hip.LES_MLP))
-    assert mlp == 2 ** 32 // (19 * 64 * 4) == 883011
-    # the largest admitted level's last pull offset stays below 2^32
+    assert mlp == (2 ** 32 - 1) // (19 * 64 * 4) == 883011
+    # the largest admitted level's last pull offset stays below 2^32, and so does the spare slot
+    # of its deferred-node stores (Q naux 4 with every node deferred)
     assert (mlp - 1) * 19 * 256 + 19 * 256 - 1 < 2 ** 32
+    assert 19 * (mlp * 64) * 4 < 2 ** 32
     grid2 = g.uniform_grid(2, (4, 4))
     assert hip.max_level_bricks(hip.make_desc(grid2, nu0=0.01, precision=hip.FP32_SHIFTED,
                                               les=hip.LES_MLP)) == min(general, 2 ** 32 // (9 * 256))

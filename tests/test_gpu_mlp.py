@@ -1,6 +1,7 @@
 """Learned closure on the device: gradients and training against the FP64 reference, the
 closure fused into the collision against the oracle, and the gain over Smagorinsky."""
 import functools
+import os
 
 import numpy as np
 import pytest
@@ -10,6 +11,8 @@ from oracle.zlbm_oracle import Oracle
 from zllbm_amd import closure, grid as g, hip
 from tests import scenarios as sc
 from tests.test_gpu_parity import _macros, assert_close
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 pytestmark = pytest.mark.gpu
 
@@ -368,6 +371,50 @@ def test_fp32_learned_closure_on_refined_grid(operands):
     l2 = np.sqrt(num / den)
     print("FP32 + closure (operands %d) vs FP64 on 3 levels: L2(u) %.2e" % (operands, l2))
     assert l2 < 1e-4
+
+
+def test_closure_operand_rounding_over_a_long_run():
+    """The trained closure on the 3-level vorticity-refined grid over 400 lockstep steps, against
+    the FP64 solver with the same network, measured in units of plain FP32 rounding (FP32
+    Smagorinsky against FP64 Smagorinsky): FP16 network operands drift 2.1 units (bound 3),
+    BF16 operands 4.8 units (bound 10), so the operands' rounding stays of the order of the
+    storage rounding and FP16 stays the closer form."""
+    from zllbm_amd import workloads
+    model = closure.ClosureModel.load(os.path.join(ROOT, "zllbm_amd", "data", "closure_turbulence.npz"))
+    W = workloads.decaying_turbulence(24, 3, buffer=2, les=2)
+
+    def make(precision, les, operands=hip.MLP_OPERANDS_FP16):
+        kw = dict(W.solver, les=les)
+        if les == hip.LES_MLP:
+            kw["mlp_operands"] = operands
+        s = hip.Solver(W.grid, precision=precision, **kw).build()
+        if les == hip.LES_MLP:
+            s.mlp_set(model)
+        for l, (rho, u) in enumerate(W.init):
+            s.init_macro(l, rho, u)
+        return s
+
+    def drift(a, b, steps=400):
+        a.time_marching(steps)
+        b.time_marching(steps)
+        num = den = 0.0
+        for l in range(3):
+            ua, ub = a.macros(l)[1], b.macros(l)[1]
+            num += np.sum((ua - ub) ** 2)
+            den += np.sum(ub ** 2)
+        return np.sqrt(num / den)
+
+    plain = drift(make(hip.FP32_SHIFTED, hip.LES_SMAGORINSKY), make(hip.FP64, hip.LES_SMAGORINSKY))
+    ref = make(hip.FP64, hip.LES_MLP)
+    fp16 = drift(make(hip.FP32_SHIFTED, hip.LES_MLP, hip.MLP_OPERANDS_FP16), ref)
+    ref = make(hip.FP64, hip.LES_MLP)
+    bf16 = drift(make(hip.FP32_SHIFTED, hip.LES_MLP, hip.MLP_OPERANDS_BF16), ref)
+    print("L2(u) after 400 steps vs FP64: Smagorinsky FP32 %.2e, closure FP16 operands %.2e, BF16 operands %.2e"
+          % (plain, fp16, bf16))
+    assert 0 < plain < 1e-3
+    assert fp16 <= 3 * plain
+    assert bf16 <= 10 * plain
+    assert fp16 < bf16
 
 
 def test_moment_transform_switch_takes_effect():

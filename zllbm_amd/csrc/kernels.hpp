@@ -546,7 +546,7 @@ __device__ __forceinline__ PullLane<D> pull_lane(int lane) {
 // direct-to-LDS dword load per direction (gfx950 global_load_lds_dword: lane i lands at
 // buf + 4 i).  Every lane loads from a valid address of fin: an absent neighbour brick is
 // replaced by b, a deferred node's missing source loads whatever slot the stencil names, and a
-// lane without a node loads its own slot; the consumer drops those values (a deferred node's
+// lane without a node loads the slot its stencil names inside brick b; the consumer drops those values (a deferred node's
 // missing directions keep the boundary / transfer values already in the aux array).  Addresses are 32-bit byte offsets
 // from the direction's base fin + 64 q.  Returns the lane's missing-source mask (deferred nodes).
 template <int D, typename T>
@@ -561,32 +561,54 @@ __device__ __forceinline__ uint32_t pull_to_lds(const BrickTables &t, const T *_
     // opaque per pull: the direction's LDS destinations are formed here, not hoisted out of the
     // caller's loop into 19 long-lived scalars
     asm volatile("" : "+s"(buf));
-    int32_t nb[NN];
+    // byte offsets of the neighbour bricks (uniform: scalar multiplies)
+    constexpr uint32_t kBrickBytes = (uint32_t)(Q * 64 * sizeof(T));
+    uint32_t nbo[NN];
 #pragma unroll
     for (int k = 0; k < NN; ++k) {
         const int32_t v = t.nbr[b * NN + k];
-        nb[k] = v < 0 ? (int32_t)b : v;
+        nbo[k] = (uint32_t)(v < 0 ? (int32_t)b : v) * kBrickBytes;
     }
-    // lanes without a node load their own slot (a line the brick's own pull fetches anyway)
+    // a lane crosses into a neighbour brick only if it holds a node: a lane without one loads
+    // the slot its stencil names inside its own brick, a line the brick's own pull fetches anyway
     const bool has_node = (t.occ[b] >> lane) & 1ull;
-    const uint32_t own = __umul24((uint32_t)b, (uint32_t)(Q * 64 * sizeof(T))) + 4u * (uint32_t)lane;
+    bool clo[3], chi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        clo[a] = P.lo[a] && has_node;
+        chi[a] = P.hi[a] && has_node;
+    }
+    constexpr int base = D == 3 ? 13 : 4;
+    // the own brick's offset in a vector register, so that each selection below takes one
+    // uniform operand and this one
+    uint32_t self = nbo[base];
+    asm volatile("" : "+v"(self));
     // addresses by selection, no branches: a branch between two of these loads makes the
-    // compiler wait on the vector memory counter there
+    // compiler wait on the vector memory counter there.  D2Q9 / D3Q19 directions cross at most
+    // two axes: the brick is chosen along the first axis, then the second.
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        constexpr int base = D == 3 ? 13 : 4;
         const int cx = L::cx[q], cy = L::cy[q], cz = D == 3 ? L::cz[q] : 0;
-        const bool ex = cx == 1 ? P.lo[0] : (cx == -1 ? P.hi[0] : false);
-        const bool ey = cy == 1 ? P.lo[1] : (cy == -1 ? P.hi[1] : false);
-        const bool ez = cz == 1 ? P.lo[2] : (cz == -1 ? P.hi[2] : false);
+        const bool ex = cx == 1 ? clo[0] : (cx == -1 ? chi[0] : false);
+        const bool ey = cy == 1 ? clo[1] : (cy == -1 ? chi[1] : false);
+        const bool ez = cz == 1 ? clo[2] : (cz == -1 ? chi[2] : false);
         const int dx = -cx, dy = -3 * cy, dz = -9 * cz;
-        const int sb = ez ? (ey ? (ex ? nb[base + dx + dy + dz] : nb[base + dy + dz])
-                                : (ex ? nb[base + dx + dz] : nb[base + dz]))
-                          : (ey ? (ex ? nb[base + dx + dy] : nb[base + dy])
-                                : (ex ? nb[base + dx] : nb[base]));
-        const uint32_t off = has_node ? __umul24((uint32_t)sb, (uint32_t)(Q * 64 * sizeof(T))) +
-                                            (P.s4[0][1 - cx] | P.s4[1][1 - cy] | P.s4[2][1 - cz])
-                                      : own;
+        uint32_t sel;
+        if (cx != 0 && cy != 0)
+            sel = ey ? (ex ? nbo[base + dx + dy] : nbo[base + dy]) : (ex ? nbo[base + dx] : self);
+        else if (cx != 0 && cz != 0)
+            sel = ez ? (ex ? nbo[base + dx + dz] : nbo[base + dz]) : (ex ? nbo[base + dx] : self);
+        else if (cy != 0 && cz != 0)
+            sel = ez ? (ey ? nbo[base + dy + dz] : nbo[base + dz]) : (ey ? nbo[base + dy] : self);
+        else if (cx != 0)
+            sel = ex ? nbo[base + dx] : self;
+        else if (cy != 0)
+            sel = ey ? nbo[base + dy] : self;
+        else if (cz != 0)
+            sel = ez ? nbo[base + dz] : self;
+        else
+            sel = self;
+        const uint32_t off = sel + (P.s4[0][1 - cx] | P.s4[1][1 - cy] | P.s4[2][1 - cz]);
         // an opaque scalar base, so that the load takes the scalar-base + 32-bit-offset form
         const char *qbase = reinterpret_cast<const char *>(fin + q * 64);
         asm volatile("" : "+s"(qbase));
@@ -650,13 +672,31 @@ void k_stream_collide_mlp(int64_t nbrick, int64_t naux, const uint64_t *__restri
         if ((am >> lane) & 1ull) {
             // deferred node: post-stream values to the aux array, except the missing sources
             // (their boundary / transfer values are there already): those go to the spare slot
-            // at index Q naux, so the stores need no branch.  32-bit element indices (the host
-            // keeps Q naux + 1 below 2^31).
-            const uint32_t ai = (uint32_t)(t.auxbase[b] + __popcll(am & ((1ull << lane) - 1ull)));
-            const uint32_t spare = (uint32_t)(Q * naux);
+            // at index Q naux.  Stores take a uniform base and a 32-bit byte offset (the host
+            // keeps Q naux 4 below 2^32); a wavefront whose deferred lanes miss no source (a
+            // uniform branch) stores without the per-direction selection.
+            const uint32_t ai4 = 4u * (uint32_t)(t.auxbase[b] + __popcll(am & ((1ull << lane) - 1ull)));
+            using gchar = __attribute__((address_space(1))) char;
+            using gT = __attribute__((address_space(1))) T;
+            gchar *base = (gchar *)aux_f;
+            asm volatile("" : "+s"(base));
+            // (each offset is made opaque so that the store keeps the base + 32-bit offset form)
+            if (__builtin_amdgcn_ballot_w64(miss != 0u) == 0ull) {
 #pragma unroll
-            for (int q = 0; q < Q; ++q)
-                aux_f[((miss >> q) & 1u) ? spare : (uint32_t)(q * naux) + ai] = T(f[q]);
+                for (int q = 0; q < Q; ++q) {
+                    uint32_t o = ai4 + (uint32_t)(4 * q * naux);
+                    asm volatile("" : "+v"(o));
+                    *(gT *)(base + o) = T(f[q]);
+                }
+            } else {
+                const uint32_t spare4 = (uint32_t)(4 * Q * naux);
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    uint32_t o = ((miss >> q) & 1u) ? spare4 : ai4 + (uint32_t)(4 * q * naux);
+                    asm volatile("" : "+v"(o));
+                    *(gT *)(base + o) = T(f[q]);
+                }
+            }
         }
         // the buffer's reads have returned before it is refilled
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");

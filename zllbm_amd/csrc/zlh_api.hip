@@ -1121,10 +1121,11 @@ zlh_status zlh_max_level_bricks(const zlh_solver_desc *desc, int64_t *max_bricks
     // slot and aux indices are 32-bit: 64 nbrick 19 < 2^31
     int64_t m = (((int64_t)1 << 31) / 19 - 1) / 64;
     // the FP32 learned closure's fused kernel pulls through 32-bit byte offsets
-    // nbrick Q 64 4 (pull_to_lds in kernels.hpp): nbrick Q 256 <= 2^32
+    // nbrick Q 64 4 (pull_to_lds in kernels.hpp) and stores deferred nodes through 32-bit byte
+    // offsets up to Q naux 4 <= nbrick Q 256: nbrick Q 256 < 2^32
     if (desc->les == ZLH_LES_MLP && desc->precision == ZLH_FP32_SHIFTED) {
         const int64_t q = desc->dims == 2 ? 9 : 19;
-        m = std::min<int64_t>(m, ((int64_t)1 << 32) / (q * 64 * 4));
+        m = std::min<int64_t>(m, (((int64_t)1 << 32) - 1) / (q * 64 * 4));
     }
     *max_bricks = m;
     return ZLH_OK;
