@@ -23,6 +23,7 @@
 #include <cstdint>
 #include "lattice.h"
 #include "mlp.hpp"
+#include "mrt_d3q19.hpp"
 
 namespace zlh {
 
@@ -193,7 +194,8 @@ __device__ __forceinline__ void neq_stress(const C (&f)[Lattice<D>::Q], C rho, C
 
 // MRT collision in moment space (LbmMrtCollisionOpt::CollisionOperator,
 // lbm_collision_model_mrt.cpp:114-143: f* = f - M^-1 S M (f - feq) + M^-1 (I - S/2) M F_i dt).
-// The moments m = M f come from pair sums/differences of opposite populations; M feq, M F_i and
+// The moments m = M f come from pair sums/differences of opposite populations (for D3Q19 in the
+// factored form of mrt_d3q19.hpp, which shares sums between rows); M feq, M F_i and
 // the stress sum_q c c (f - feq) are closed forms in the moments (MomentSpace), so no per-
 // direction equilibrium is formed; the post-collision moments return through
 // f* = M^T diag(1/|M_k|^2) m*.  Stored (shifted) populations map to shifted moments
@@ -213,6 +215,8 @@ __device__ __forceinline__ void collide_mrt(C (&f)[Lattice<D>::Q], const C *F, c
     C mm[Q];
     if constexpr (mrt_on_mfma<C, VAR>()) {
         moments_mfma<Q>(*ops, f, mm);
+    } else if constexpr (D == 3) {
+        mrt19_moments<C>(f, mm);
     } else {
         C ps[NP], pd[NP];
 #pragma unroll
@@ -324,6 +328,10 @@ __device__ __forceinline__ void collide_mrt(C (&f)[Lattice<D>::Q], const C *F, c
         if (L::viscous_row[k]) post(k, s_nu, true);
     if constexpr (mrt_on_mfma<C, VAR>()) {
         populations_mfma<Q>(*ops, y, f);
+        return;
+    }
+    if constexpr (D == 3) {
+        mrt19_populations<C>(y, f);
         return;
     }
     // back to populations: even rows give the pair mean, odd rows the pair difference
