@@ -110,19 +110,14 @@ constexpr size_t kGatherLdsBudget = 20 * 1024;
 constexpr int kGatherMaxRows = 768;   // staged rows of one group (row offsets fit the word's 16 bits)
 constexpr int64_t kGatherChunk = 2048;   // ghost nodes sorted by contributor count together
 template <int D, typename T, bool SHIFT>
-__global__ __launch_bounds__(256) void k_c2f_gather(int64_t n_groups, const GatherGroup *__restrict__ groups,
-                                                    const int32_t *__restrict__ tgt_ai,
-                                                    const int32_t *__restrict__ begin,
-                                                    const uint32_t *__restrict__ contrib,
-                                                    const int32_t *__restrict__ rows, const T *__restrict__ src_f,
-                                                    T *__restrict__ aux_f, int64_t naux) {
+__device__ __forceinline__ void gather_group(int64_t gi, unsigned char *lds_raw, const GatherGroup *__restrict__ groups,
+                                             const int32_t *__restrict__ tgt_ai, const int32_t *__restrict__ begin,
+                                             const uint32_t *__restrict__ contrib, const int32_t *__restrict__ rows,
+                                             const T *__restrict__ src_f, T *__restrict__ aux_f, int64_t naux) {
     constexpr int Q = Lattice<D>::Q;
     constexpr int NQ = kGatherDirs;
     constexpr int PARTS = (Q + NQ - 1) / NQ;
     constexpr int QP = PARTS * NQ;   // LDS row stride: every thread reads NQ entries, pads included
-    extern __shared__ unsigned char lds_raw[];
-    const int64_t gi = xcd_group(n_groups);   // groups are in spatial order: an XCD takes a range
-    if (gi < 0) return;
     const GatherGroup G = groups[gi];
     const int32_t nr = G.r1 - G.r0, nc = G.c1 - G.c0;
     T *rowbuf = reinterpret_cast<T *>(lds_raw);                                  // [nr][QP]
@@ -240,15 +235,27 @@ __global__ __launch_bounds__(256) void k_c2f_gather(int64_t n_groups, const Gath
     }
 }
 
+template <int D, typename T, bool SHIFT>
+__global__ __launch_bounds__(256) void k_c2f_gather(int64_t n_groups, const GatherGroup *__restrict__ groups,
+                                                    const int32_t *__restrict__ tgt_ai,
+                                                    const int32_t *__restrict__ begin,
+                                                    const uint32_t *__restrict__ contrib,
+                                                    const int32_t *__restrict__ rows, const T *__restrict__ src_f,
+                                                    T *__restrict__ aux_f, int64_t naux) {
+    extern __shared__ unsigned char lds_raw[];
+    const int64_t gi = xcd_group(n_groups);   // groups are in spatial order: an XCD takes a range
+    if (gi < 0) return;
+    gather_group<D, T, SHIFT>(gi, lds_raw, groups, tgt_ai, begin, contrib, rows, src_f, aux_f, naux);
+}
+
 // NodeInfoFine2Coarse (lbm_grid_interface.cpp:594-610) + PostStreamFine2Coarse
 // (lbm_collision_models.cpp:259-267): rho, u of the fine node's last collision
 template <int D, typename T, bool SHIFT>
-__global__ void k_f2c(int64_t n, const int32_t *coarse_ai, const int32_t *fine_ai, const T *fine_f, int64_t naux_f,
-                      const double *fine_macro, double tau_ratio, int compressible, T *coarse_f, int64_t naux_c) {
+__device__ __forceinline__ void f2c_node(int64_t p, const int32_t *coarse_ai, const int32_t *fine_ai, const T *fine_f,
+                                         int64_t naux_f, const double *fine_macro, double tau_ratio, int compressible,
+                                         T *coarse_f, int64_t naux_c) {
     using L = Lattice<D>;
     constexpr int Q = L::Q;
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
     const int64_t i = fine_ai[p], j = coarse_ai[p];
     if constexpr (SHIFT) {
         // FP32 / BF16 storage: in FP32 on the stored deviations f - w
@@ -274,6 +281,45 @@ __global__ void k_f2c(int64_t n, const int32_t *coarse_ai, const int32_t *fine_a
                                         : L::w[q] * (rho + 3. * cu + 4.5 * cu * cu - 1.5 * usq);
         const double ff = double(fine_f[q * naux_f + i]) + (SHIFT ? L::w[q] : 0.);
         coarse_f[q * naux_c + j] = T(feq + tau_ratio * (ff - feq) - (SHIFT ? L::w[q] : 0.));
+    }
+}
+
+template <int D, typename T, bool SHIFT>
+__global__ void k_f2c(int64_t n, const int32_t *coarse_ai, const int32_t *fine_ai, const T *fine_f, int64_t naux_f,
+                      const double *fine_macro, double tau_ratio, int compressible, T *coarse_f, int64_t naux_c) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    f2c_node<D, T, SHIFT>(p, coarse_ai, fine_ai, fine_f, naux_f, fine_macro, tau_ratio, compressible, coarse_f, naux_c);
+}
+
+// The gather and the fine-to-coarse copy of one exchange in one launch, where the host found
+// them independent (no fine node is both a gather target and a fine-to-coarse source; the
+// coarse-to-fine source rows are taken before either): blocks below nb_gather (a multiple of 8)
+// are the gather's, in its XCD-aware order; the blocks after them take 256 fine-to-coarse nodes
+// each.
+template <int D, typename T, bool SHIFT>
+__global__ __launch_bounds__(256) void k_c2f_gather_f2c(int64_t nb_gather, int64_t n_groups,
+                                                        const GatherGroup *__restrict__ groups,
+                                                        const int32_t *__restrict__ tgt_ai,
+                                                        const int32_t *__restrict__ begin,
+                                                        const uint32_t *__restrict__ contrib,
+                                                        const int32_t *__restrict__ rows, const T *__restrict__ src_f,
+                                                        T *__restrict__ aux_f, int64_t naux, int64_t n_f2c,
+                                                        const int32_t *coarse_ai, const int32_t *fine_ai,
+                                                        const double *fine_macro, double tau_ratio, int compressible,
+                                                        T *coarse_f, int64_t naux_c) {
+    extern __shared__ unsigned char lds_raw[];
+    const int64_t bid = blockIdx.x;
+    if (bid < nb_gather) {
+        const int64_t per = nb_gather >> 3;
+        const int64_t gi = (bid & 7) * per + (bid >> 3);
+        if (gi >= n_groups) return;
+        gather_group<D, T, SHIFT>(gi, lds_raw, groups, tgt_ai, begin, contrib, rows, src_f, aux_f, naux);
+    } else {
+        const int64_t p = (bid - nb_gather) * 256 + threadIdx.x;
+        if (p < n_f2c)
+            f2c_node<D, T, SHIFT>(p, coarse_ai, fine_ai, aux_f, naux, fine_macro, tau_ratio, compressible, coarse_f,
+                                  naux_c);
     }
 }
 
@@ -789,6 +835,14 @@ inline zlh_status upload_transfer(zlh_ctx *ctx, int level) {
     if ((s = dalloc(ctx, X.src_mid, ctx->tsize() * ctx->q * std::max<int64_t>(X.n_src + X.n_fb, 1)))) return s;
     if ((s = upload(ctx, X.f2c_coarse_ai, f2c_c))) return s;
     if ((s = upload(ctx, X.f2c_fine_ai, f2c_f))) return s;
+    {
+        std::vector<int32_t> a(tgt2.begin(), tgt2.end()), b(f2c_f.begin(), f2c_f.end());
+        std::sort(a.begin(), a.end());
+        std::sort(b.begin(), b.end());
+        std::vector<int32_t> common;
+        std::set_intersection(a.begin(), a.end(), b.begin(), b.end(), std::back_inserter(common));
+        X.f2c_apart = common.empty();
+    }
     F.xfer_writes = tgt;
     F.xfer_reads = fb_ai;
     F.xfer_reads.insert(F.xfer_reads.end(), f2c_f.begin(), f2c_f.end());
@@ -908,8 +962,20 @@ zlh_status launch_transfers_t(zlh_ctx *ctx, int level, int parts) {
                                    C.naux, X.fb_fine_ai.as<int32_t>(), F.aux_f_d.as<T>(), F.naux, c2f, comp,
                                    nullptr, rows);
         }
+        const int64_t nb_gather = (X.n_groups + 7) / 8 * 8;
+        if ((parts & kTransferC2F) && (parts & kTransferF2C) && X.n_f2c > 0 && X.f2c_apart) {
+            // one launch for both halves (independent: see k_c2f_gather_f2c)
+            hipLaunchKernelGGL((k_c2f_gather_f2c<D, T, SHIFT>), dim3((unsigned)(nb_gather + blocks(X.n_f2c))), dim3(256),
+                               X.lds_bytes, ctx->stream, nb_gather, X.n_groups, X.groups.as<GatherGroup>(),
+                               X.c2f_target.as<int32_t>(), X.c2f_begin.as<int32_t>(), X.c2f_contrib.as<uint32_t>(),
+                               X.group_rows.as<int32_t>(), rows, F.aux_f_d.as<T>(), F.naux, X.n_f2c,
+                               X.f2c_coarse_ai.as<int32_t>(), X.f2c_fine_ai.as<int32_t>(), F.aux_macro_d.as<double>(),
+                               f2c, comp, C.aux_f_d.as<T>(), C.naux);
+            ZLH_TRY(hipGetLastError());
+            return ZLH_OK;
+        }
         if (parts & kTransferC2F)
-            hipLaunchKernelGGL((k_c2f_gather<D, T, SHIFT>), dim3((unsigned)((X.n_groups + 7) / 8 * 8)), dim3(256),
+            hipLaunchKernelGGL((k_c2f_gather<D, T, SHIFT>), dim3((unsigned)nb_gather), dim3(256),
                                X.lds_bytes, ctx->stream, X.n_groups, X.groups.as<GatherGroup>(),
                                X.c2f_target.as<int32_t>(), X.c2f_begin.as<int32_t>(), X.c2f_contrib.as<uint32_t>(),
                                X.group_rows.as<int32_t>(), rows, F.aux_f_d.as<T>(), F.naux);

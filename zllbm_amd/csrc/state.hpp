@@ -61,6 +61,8 @@ struct TransferTables {
                                 // of src_f and the current ones)
     DevBuf f2c_coarse_ai;       // [n_f2c]
     DevBuf f2c_fine_ai;         // [n_f2c] fine aux index of the coincident node
+    bool f2c_apart = false;     // no fine node is both a gather target and a fine-to-coarse
+                                // source: the gather and the copy may share one launch
 };
 
 // Direct-forcing immersed boundary of one level (FsiImmersedBoundary::CalculateBodyForce,
