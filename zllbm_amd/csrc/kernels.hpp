@@ -532,20 +532,19 @@ struct PullLane {
 
 template <int D>
 __device__ __forceinline__ PullLane<D> pull_lane(int lane) {
-    using B = Brick<D>;
-    constexpr int E = B::edge;
     PullLane<D> P;
+    const uint32_t l4 = 4u * (uint32_t)lane;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const int l = a < D ? B::lane_coord(lane, a) : 0;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const int sc = (l - (1 - k)) & (E - 1);
-            const int lo = a == 0 ? B::lane_of(sc, 0, 0) : (a == 1 ? B::lane_of(0, sc, 0) : B::lane_of(0, 0, sc));
-            P.s4[a][k] = a < D ? 4u * (uint32_t)lo : 0u;
-        }
-        P.lo[a] = a < D && l == 0;
-        P.hi[a] = a < D && l == E - 1;
+        constexpr uint32_t kNone = 0u;
+        const uint32_t m4 = a < D ? lane_axis_mask4<D>(a) : kNone;
+        const uint32_t own4 = l4 & m4;
+        // k = 1 - c: the source coordinate along a is l - 1 (k = 0), l (1) or l + 1 (2)
+        P.s4[a][0] = a < D ? lane_axis_step4(own4, m4, -1) : 0u;
+        P.s4[a][1] = own4;
+        P.s4[a][2] = a < D ? lane_axis_step4(own4, m4, 1) : 0u;
+        P.lo[a] = a < D && own4 == 0u;
+        P.hi[a] = a < D && own4 == m4;
     }
     return P;
 }

@@ -170,4 +170,21 @@ template <> struct Brick<3> {
     }
 };
 
+// The lane bits of one axis ("dilated" coordinate: the axis' coordinate bits at their interleaved
+// positions) and their step by one along that axis, modulo the brick edge, without
+// de-interleaving: a borrow or carry passes through the other axes' bits when they are forced to
+// 0 (decrement) or 1 (increment).  Everything is scaled by 4 (byte offsets of 4-byte lanes).
+template <int D>
+__host__ __device__ constexpr uint32_t lane_axis_mask4(int axis) {
+    uint32_t m = 0;
+    for (int bit = 0; bit < 6; ++bit)
+        if (axis < D && Brick<D>::lane_coord(1 << bit, axis) != 0) m |= 1u << bit;
+    return 4u * m;
+}
+// 4 x the lane bits of the coordinate c - 1 (step < 0) or c + 1 (step > 0) along the axis whose
+// mask is m4, from own4 = 4 x (the lane's bits of that axis)
+__host__ __device__ inline uint32_t lane_axis_step4(uint32_t own4, uint32_t m4, int step) {
+    return step < 0 ? ((own4 - 4u) & m4) : (((own4 | ~m4) + 4u) & m4);
+}
+
 }  // namespace zlh
