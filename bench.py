@@ -194,6 +194,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--one-stream", action="store_true",
                     help="time marching on one stream (levels do not overlap; for per-kernel traces)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="direct launches instead of replayed graphs of two background steps")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (for counter calibration)")
     args = ap.parse_args()
@@ -237,6 +239,8 @@ def main():
     s = make_solver(hip, W, hip.BF16_SHIFTED if bf16 else hip.FP32_SHIFTED, les, model)
     if args.one_stream:
         s.level_streams_enable(False)
+    if args.no_graphs:
+        s.graphs_enable(False)
     log("aux nodes per level", [s.info(l)[1] for l in range(len(W.nodes))])
     wall, k_ms, k_n, _ = timed(s, W, args.warmup, args.steps)
     mlups = W.updates_per_step * args.steps / wall / 1e6
