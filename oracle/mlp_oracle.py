@@ -138,9 +138,10 @@ def forward_lowp(p, X, rnd):
     """The network with low-precision operands and wide accumulation, as the matrix-core form
     evaluates it. W1, b1 and b2 are first scaled by c = -log2(e) (in FP64, stored as FP32), so
     both layers' pre-activations are t = c z and an activation is t / (1 + 2^t) = c silu(z).
-    Inputs, the scaled W1 / b1 / b2, W2 and the layer-1 activations are rounded by `rnd` (the
-    biases enter the products as weights of a constant input 1); w3 / c, b3 and the
-    accumulations are exact. Up to those roundings this is forward()."""
+    Inputs, the scaled W1 / b1, W2 and the layer-1 activations are rounded by `rnd` (b1 enters the
+    products as the weight of a constant input 1); the scaled b2 (the layer-2 accumulators'
+    starting value, FP32), w3 / c, b3 and the accumulations are exact. Up to those roundings this
+    is forward()."""
     W1, b1, W2, b2, w3, b3 = unpack(p)
     c = SILU_SCALE
 
@@ -150,7 +151,7 @@ def forward_lowp(p, X, rnd):
     with np.errstate(over="ignore"):          # 2^t -> inf gives the activation 0, as on the device
         T1 = rnd(X) @ rnd(sc(W1)).T + rnd(sc(b1))
         H1 = rnd(T1 / (1.0 + np.exp2(T1)))
-        T2 = H1 @ rnd(W2).T + rnd(sc(b2))
+        T2 = H1 @ rnd(W2).T + sc(b2)
         return softplus((T2 / (1.0 + np.exp2(T2))) @ (w3 / c) + b3)
 
 

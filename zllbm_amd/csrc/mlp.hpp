@@ -23,9 +23,9 @@ __host__ __device__ constexpr int mfma_row(int h, int r) { return (r >> 2) * 8 +
 //   w1a[l][e]    = layer-1 A operand, K slot k = 8 (l / 32) + e of row l % 32: W1[l % 32][k] for
 //                  the 9 features, b1[l % 32] in slot 9 (its B slot holds 1), zero above
 //   w2a[s][l][e] = layer-2 A operand, K step s < 2: W2[l % 32][mfma_row(l / 32, 8 s + e)] (the
-//                  layer-1 accumulator order, so activations stay in their registers); step 2
-//                  holds b2[l % 32] in slot 0 (its B slot holds 1)
+//                  layer-1 accumulator order, so activations stay in their registers)
 //   w3r[h][r]    = w3[mfma_row(h, r)] (FP32)
+//   b2r[h][r]    = b2[mfma_row(h, r)] (FP32): the layer-2 accumulators start from it
 // plus the input normalisation, training range, output scale and b3, so that the fused kernel
 // reads nothing of the network from global memory after its first store; and the operand images
 // of the MRT moment transform on the matrix cores (moments_mfma, populations_mfma):
@@ -34,10 +34,11 @@ __host__ __device__ constexpr int mfma_row(int h, int r) { return (r >> 2) * 8 +
 constexpr int kMrtSteps = 10;   // K steps of two: (19 + 1) / 2 for D3Q19, 5 used for D2Q9
 struct alignas(16) MlpOps {
     uint16_t w1a[64][8];
-    uint16_t w2a[3][64][8];
+    uint16_t w2a[2][64][8];
     uint16_t h1a[64][8];
-    uint16_t h2a[3][64][8];
+    uint16_t h2a[2][64][8];
     float w3r[2][16];
+    float b2r[2][16];
     float mean[kMlpF];
     float inv_std[kMlpF];
     float fmin[kMlpF];
@@ -308,17 +309,18 @@ __device__ __forceinline__ float mlp_forward_wave(const MlpOps &o, const float *
     }
     // SiLU; accumulator r of this lane is hidden unit mfma_row(h, r), which is also the K slot
     // 8 h + r % 8 of layer-2 K step r / 8, so the activations stay in place
-    vec g0[3], g1[3];
+    vec g0[2], g1[2];
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         g0[r >> 3][r & 7] = (elem)silu_scaled(z0[r]);
         g1[r >> 3][r & 7] = (elem)silu_scaled(z1[r]);
     }
+    // layer 2: the accumulators start from the (FP32, scaled) bias of their rows
+    f32x16_t y0, y1;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) g0[2][e] = g1[2][e] = (elem)(h == 0 && e == 0 ? 1.f : 0.f);
-    f32x16_t y0 = zero, y1 = zero;
+    for (int r = 0; r < 16; ++r) y0[r] = y1[r] = o.b2r[h][r];
 #pragma unroll
-    for (int st = 0; st < 3; ++st) {
+    for (int st = 0; st < 2; ++st) {
         const vec a2 = OT::load(o, OT::w2(o, st, lane));
         y0 = OT::mfma(a2, g0[st], y0);
         y1 = OT::mfma(a2, g1[st], y1);

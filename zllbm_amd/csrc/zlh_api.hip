@@ -1735,12 +1735,13 @@ zlh_status zlh_mlp_set(zlh_ctx *ctx, const float *params, const float *mean, con
                 o.w2a[st][l][e] = bf16_bits(a2);
                 o.h2a[st][l][e] = f16_bits(a2);
             }
-            o.w2a[2][l][e] = bf16_bits(k == 0 ? scaled(b2[i]) : 0.f);
-            o.h2a[2][l][e] = f16_bits(k == 0 ? scaled(b2[i]) : 0.f);
         }
     }
     for (int h = 0; h < 2; ++h)
-        for (int r = 0; r < 16; ++r) o.w3r[h][r] = (float)((double)w3[mfma_row(h, r)] / kSiluScale);
+        for (int r = 0; r < 16; ++r) {
+            o.w3r[h][r] = (float)((double)w3[mfma_row(h, r)] / kSiluScale);
+            o.b2r[h][r] = scaled(b2[mfma_row(h, r)]);
+        }
     // operand images of the moment transform on the matrix cores (M and M^T, integers)
     {
         auto fill = [&o](auto S) {
