@@ -92,6 +92,9 @@ def cpp_drop_in(hip, W, model, mlp_operands, steps, warmup=4):
     kw = dict(W.solver, les=hip.LES_MLP)
     res = {"loop": "AmrManager::TimeMarching over GridInfoLbmHip / SolverLbmHip (zllbm_lbm_driver)",
            "steps": steps, "warmup": warmup}
+    if not os.path.exists(host_driver.DRIVER):
+        res["skipped"] = "zllbm_amd/zllbm_lbm_driver is not built (__graft_entry__.build())"
+        return res
     for replay in (True, False):
         d = tempfile.mkdtemp(prefix="zllbm_driver_")
         try:
@@ -100,9 +103,12 @@ def cpp_drop_in(hip, W, model, mlp_operands, steps, warmup=4):
                                       extra=("mlp_operands = " + mlp_operands,), timeout=600)
         finally:
             shutil.rmtree(d, ignore_errors=True)
-        if proc.returncode != 0:
-            raise RuntimeError("host driver failed: " + proc.stderr[-2000:])
-        ms = host_driver.ms_per_step(proc)
+        ms = host_driver.ms_per_step(proc) if proc.returncode == 0 else None
+        if ms is None:
+            # reported in the line, not fatal to it: the ctypes measurement above stands alone
+            res["error"] = "host driver exit %d: %s" % (proc.returncode, proc.stderr[-500:])
+            log("C++ drop-in failed:", res["error"])
+            return res
         res["graph_replay_ms_per_step" if replay else "per_substep_ms_per_step"] = round(ms, 4)
         log("C++ drop-in (%s): %.3f ms per step" % ("graph replay" if replay else "per-substep launches", ms))
     return res
