@@ -95,6 +95,9 @@ def cpp_drop_in(hip, W, model, mlp_operands, steps, warmup=4):
     if not os.path.exists(host_driver.DRIVER):
         res["skipped"] = "zllbm_amd/zllbm_lbm_driver is not built (__graft_entry__.build())"
         return res
+    if getattr(W, "ib", None) is not None:
+        res["skipped"] = "the example driver loads no immersed-boundary markers"
+        return res
     for replay in (True, False):
         d = tempfile.mkdtemp(prefix="zllbm_driver_")
         try:
