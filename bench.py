@@ -95,14 +95,13 @@ def cpp_drop_in(hip, W, model, mlp_operands, steps, warmup=4):
     if not os.path.exists(host_driver.DRIVER):
         res["skipped"] = "zllbm_amd/zllbm_lbm_driver is not built (__graft_entry__.build())"
         return res
-    if getattr(W, "ib", None) is not None:
-        res["skipped"] = "the example driver loads no immersed-boundary markers"
-        return res
     for replay in (True, False):
         d = tempfile.mkdtemp(prefix="zllbm_driver_")
         try:
+            ib = getattr(W, "ib", None)
+            sets = [(ib.level, ib.xyz, ib.area, np.zeros_like(ib.xyz), 2, 2.0)] if ib is not None else None
             proc, _ = host_driver.run(W.grid, kw, warmup + steps, d, collision="mrt", precision="fp32", model=model,
-                                      graph_replay=replay, init=W.init, warmup=warmup, output=False,
+                                      graph_replay=replay, init=W.init, warmup=warmup, ib=sets, output=False,
                                       extra=("mlp_operands = " + mlp_operands,), timeout=600)
         finally:
             shutil.rmtree(d, ignore_errors=True)

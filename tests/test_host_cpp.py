@@ -78,6 +78,24 @@ def test_driver_graph_replay_matches_per_substep(tmp_path):
         assert np.max(np.abs(outs[0][l] - ref)) < 1e-11 * np.max(np.abs(ref))
 
 
+@pytest.mark.gpu
+def test_driver_immersed_boundary_matches_oracle(tmp_path):
+    """SetImmersedBoundary through the driver's marker file: the D2Q9 channel with an immersed
+    cylinder (101 markers, stencil 2) after 20 background steps, against the oracle with the same
+    markers, under graph replay."""
+    grid, kw, (xy, area) = sc.cylinder_channel()
+    kw = dict(kw)
+    kw.pop("has_force", None)
+    proc, out = host_driver.run(grid, kw, 20, str(tmp_path), collision="srt",
+                                ib=[(0, xy, area, np.zeros_like(xy), 2, 2.0)])
+    assert proc.returncode == 0, proc.stderr
+    o = Oracle(grid, collision=(0,), has_force=True, **kw)
+    o.set_ib(0, xy, area, np.zeros_like(xy), stencil_dis=2)
+    o.time_marching(20)
+    ref = o.f(0)
+    assert np.max(np.abs(out[0] - ref)) < 1e-10 * np.max(np.abs(ref))
+
+
 def _cavity3d(n=10, lid=0.05):
     from zllbm_amd import grid as g
     grid = g.uniform_grid(3, (n, n, n))

@@ -16,7 +16,10 @@
 //               time_interpolation = <0|1>, graph_replay = <0|1> (default 1: SetGraphReplay),
 //               init = <file> (initial density and velocity: per level an int64 node count, then
 //               the densities and the velocity components node by node, as doubles, in key order),
-//               warmup = <steps> (of the background steps, the first ones, left out of the timing))
+//               warmup = <steps> (of the background steps, the first ones, left out of the timing),
+//               ib = <file> (immersed-boundary markers, SetImmersedBoundary: records of int32 level,
+//               int32 stencil, double stiffness, int64 n, then n x 3 coordinates, n areas and
+//               n x 3 velocities, as doubles))
 //   stdout:     the solver method, "ms_per_step <t>" (wall time of the background steps after the
 //               warm-up, device synchronised at both ends) and "done <steps> steps"
 //   closure file (les = learned): whitespace-separated numbers, the 1409 network parameters,
@@ -144,6 +147,27 @@ int main(int argc, char **argv) {
             }
             else if (key == "time_interpolation") { int v; val >> v; lbm->SetTimeInterpolation(v != 0); }
             else if (key == "graph_replay") { int v; val >> v; lbm->SetGraphReplay(v != 0); }
+            else if (key == "ib") {
+                // records {int32 level, int32 stencil, double stiffness, int64 n, n x 3 coordinates,
+                // n areas, n x 3 velocities} until the end of the file
+                std::string path; val >> path;
+                std::ifstream bin(path, std::ios::binary);
+                int32_t level, stencil;
+                double stiffness;
+                int64_t n;
+                while (rd(bin, &level) && rd(bin, &stencil) && rd(bin, &stiffness) && rd(bin, &n)) {
+                    std::vector<double> xyz((size_t)n * 3), area((size_t)n), vel((size_t)n * 3);
+                    if (n < 0 || !rd(bin, xyz.data(), xyz.size()) || !rd(bin, area.data(), area.size()) ||
+                        !rd(bin, vel.data(), vel.size()))
+                        throw ZlhError("immersed-boundary file " + path + " is short");
+                    std::vector<std::array<DefReal, 3>> x((size_t)n), v((size_t)n);
+                    for (int64_t i = 0; i < n; ++i) {
+                        x[i] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+                        v[i] = {vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]};
+                    }
+                    lbm->SetImmersedBoundary(level, x, area, v, stencil, stiffness);
+                }
+            }
             else if (key == "warmup") { val >> warmup; }
             else if (key == "init") {
                 std::string path; val >> path;

@@ -34,6 +34,26 @@ def write_init(path, init):
             np.ascontiguousarray(u, dtype="<f8").tofile(fh)
 
 
+def write_ib(path, sets):
+    """Immersed-boundary file of the driver: per marker set (level, coordinates (n, dims), areas,
+    velocities (n, dims), stencil, stiffness) one record of int32 level, int32 stencil, double
+    stiffness, int64 n, then the coordinates and velocities padded to 3 components and the areas."""
+    with open(path, "wb") as fh:
+        for level, xyz, area, vel, stencil, stiffness in sets:
+            area = np.ascontiguousarray(area, dtype="<f8")
+            n = len(area)
+
+            def pad3(a):
+                a = np.asarray(a, np.float64).reshape(n, -1)
+                return np.ascontiguousarray(np.pad(a, ((0, 0), (0, 3 - a.shape[1]))), dtype="<f8")
+            np.array([level, stencil], dtype="<i4").tofile(fh)
+            np.array([stiffness], dtype="<f8").tofile(fh)
+            np.array([n], dtype="<i8").tofile(fh)
+            pad3(xyz).tofile(fh)
+            area.tofile(fh)
+            pad3(vel).tofile(fh)
+
+
 def write_params(path, kw, dims, collision="srt", precision="fp64", closure_path=None, extra=()):
     """Parameter file of the driver from solver keywords (nu0, bc, les, cs, les_tau_mode)."""
     lines = ["viscosity = %r" % kw["nu0"], "rho = 1.0",
@@ -51,9 +71,10 @@ def write_params(path, kw, dims, collision="srt", precision="fp64", closure_path
 
 def run(grid, kw, steps, workdir, collision="srt", precision="fp64", model=None, checkpoint_in=None,
         checkpoint_out=None, vtu_out=None, checkpoint_dir_in=None, checkpoint_dir_out=None, time_interp=False,
-        graph_replay=None, init=None, warmup=None, output=True, extra=(), timeout=300):
+        graph_replay=None, init=None, warmup=None, ib=None, output=True, extra=(), timeout=300):
     """Runs the driver for `steps` background steps; returns the finished process and, when it
-    succeeded and `output` is set, the post-stream populations of every level (key order)."""
+    succeeded and `output` is set, the post-stream populations of every level (key order).  `ib`
+    lists marker sets (level, coordinates, areas, velocities, stencil, stiffness)."""
     gpath = os.path.join(workdir, "grid.bin")
     ppath = os.path.join(workdir, "params.txt")
     opath = os.path.join(workdir, "out.bin") if output else "-"
@@ -76,6 +97,10 @@ def run(grid, kw, steps, workdir, collision="srt", precision="fp64", model=None,
     if init is not None:
         write_init(ipath, init)
         extra.append("init = " + ipath)
+    if ib:
+        bpath = os.path.join(workdir, "ib.bin")
+        write_ib(bpath, ib)
+        extra.append("ib = " + bpath)
     write_params(ppath, kw, grid.dims, collision, precision, cpath, extra)
     proc = subprocess.run([DRIVER, gpath, ppath, str(steps), opath], capture_output=True, text=True, timeout=timeout)
     if proc.returncode != 0 or not output:
