@@ -41,13 +41,13 @@ __device__ __forceinline__ void macro_feq_double(const double *f, int compressib
 // computed now and the row `prev` of the last synchronised transfer, i.e. of the coarse state
 // before and after the coarse level's last step; fallback rows stay current copies.
 template <int D, typename T, bool SHIFT, bool MID>
-__global__ __launch_bounds__(256) void k_c2f_sources(int64_t n_src, int64_t n_fb, const int32_t *src_ai,
+__global__ __launch_bounds__(256) void k_c2f_sources(int64_t r_begin, int64_t n_src, int64_t n_fb, const int32_t *src_ai,
                                                      const T *aux_c, int64_t naux_c, const int32_t *fb_ai,
                                                      const T *aux_f, int64_t naux_f, double tau_ratio,
                                                      int compressible, const T *__restrict__ prev, T *src_f) {
     constexpr int Q = Lattice<D>::Q;
     __shared__ T tile[256 * Q];   // row-major; the odd stride Q keeps the row writes conflict-free
-    const int64_t r0 = (int64_t)blockIdx.x * 256;
+    const int64_t r0 = r_begin + (int64_t)blockIdx.x * 256;
     const int64_t r = r0 + threadIdx.x;
     const int64_t n = n_src + n_fb;
     if (r < n_src) {
@@ -934,8 +934,13 @@ inline zlh_status upload_ib(zlh_ctx *ctx, int level) {
 // in that order (InformationFromGridOfDifferentLevel, lbm_solver_interface.cpp:224-238);
 // kTransferMid makes the coarse-to-fine part the time-interpolated refill after an odd substep
 // (rows into src_mid, src_f kept as the synchronised rows), kTransferRows only computes the
-// synchronised rows (the earlier state of the next time-interpolated refill)
-constexpr int kTransferC2F = 1, kTransferF2C = 2, kTransferMid = 4, kTransferRows = 8;
+// synchronised rows (the earlier state of the next time-interpolated refill).
+// kTransferCoarseRows forms only the rescaled coarse rows (they depend on the coarse level's
+// state alone, so the coarse level's stream forms them right after its step), and
+// kTransferRowsReady makes a coarse-to-fine part that finds them in place and forms only the
+// fallback rows before the gather (not with kTransferMid)
+constexpr int kTransferC2F = 1, kTransferF2C = 2, kTransferMid = 4, kTransferRows = 8, kTransferCoarseRows = 16,
+              kTransferRowsReady = 32;
 template <int D, typename T, bool SHIFT>
 zlh_status launch_transfers_t(zlh_ctx *ctx, int level, int parts) {
     LevelState &F = ctx->lv[level], &C = ctx->lv[level - 1];
@@ -947,20 +952,32 @@ zlh_status launch_transfers_t(zlh_ctx *ctx, int level, int parts) {
     const double dtc = 1. / (double)(1u << (level - 1)), dtf = 1. / (double)(1u << level);
     const double c2f = 0.5 * (nu * 3. / dtc * 2. + 0.5) / (nu * 3. / dtc + 0.5);
     const double f2c = 2. * (nu * 3. / dtf / 2 + 0.5) / (nu * 3. / dtf + 0.5);
+    if ((parts & kTransferCoarseRows) && X.n_c2f > 0) {
+        // the coarse rows alone, rows [0, n_src)
+        if (X.n_src > 0)
+            hipLaunchKernelGGL((k_c2f_sources<D, T, SHIFT, false>), dim3(blocks(X.n_src)), dim3(256), 0, ctx->stream,
+                               int64_t(0), X.n_src, int64_t(0), X.src_coarse_ai.as<int32_t>(), C.aux_f_d.as<T>(),
+                               C.naux, X.fb_fine_ai.as<int32_t>(), F.aux_f_d.as<T>(), F.naux, c2f, comp, nullptr,
+                               X.src_f.as<T>());
+        ZLH_TRY(hipGetLastError());
+        return ZLH_OK;
+    }
     if ((parts & (kTransferC2F | kTransferRows)) && X.n_c2f > 0) {
         const bool mid = (parts & kTransferMid) != 0;
         T *rows = mid ? X.src_mid.as<T>() : X.src_f.as<T>();
-        if (X.n_src + X.n_fb > 0) {
+        // with kTransferRowsReady the coarse rows are in place: only the fallback rows remain
+        const int64_t r_begin = (parts & kTransferRowsReady) ? X.n_src : 0;
+        if (X.n_src + X.n_fb > r_begin) {
             if (mid)
-                hipLaunchKernelGGL((k_c2f_sources<D, T, SHIFT, true>), dim3(blocks(X.n_src + X.n_fb)), dim3(256), 0,
-                                   ctx->stream, X.n_src, X.n_fb, X.src_coarse_ai.as<int32_t>(), C.aux_f_d.as<T>(),
-                                   C.naux, X.fb_fine_ai.as<int32_t>(), F.aux_f_d.as<T>(), F.naux, c2f, comp,
-                                   X.src_f.as<T>(), rows);
+                hipLaunchKernelGGL((k_c2f_sources<D, T, SHIFT, true>), dim3(blocks(X.n_src + X.n_fb - r_begin)),
+                                   dim3(256), 0, ctx->stream, r_begin, X.n_src, X.n_fb, X.src_coarse_ai.as<int32_t>(),
+                                   C.aux_f_d.as<T>(), C.naux, X.fb_fine_ai.as<int32_t>(), F.aux_f_d.as<T>(), F.naux,
+                                   c2f, comp, X.src_f.as<T>(), rows);
             else
-                hipLaunchKernelGGL((k_c2f_sources<D, T, SHIFT, false>), dim3(blocks(X.n_src + X.n_fb)), dim3(256), 0,
-                                   ctx->stream, X.n_src, X.n_fb, X.src_coarse_ai.as<int32_t>(), C.aux_f_d.as<T>(),
-                                   C.naux, X.fb_fine_ai.as<int32_t>(), F.aux_f_d.as<T>(), F.naux, c2f, comp,
-                                   nullptr, rows);
+                hipLaunchKernelGGL((k_c2f_sources<D, T, SHIFT, false>), dim3(blocks(X.n_src + X.n_fb - r_begin)),
+                                   dim3(256), 0, ctx->stream, r_begin, X.n_src, X.n_fb, X.src_coarse_ai.as<int32_t>(),
+                                   C.aux_f_d.as<T>(), C.naux, X.fb_fine_ai.as<int32_t>(), F.aux_f_d.as<T>(), F.naux,
+                                   c2f, comp, nullptr, rows);
         }
         const int64_t nb_gather = (X.n_groups + 7) / 8 * 8;
         if ((parts & kTransferC2F) && (parts & kTransferF2C) && X.n_f2c > 0 && X.f2c_apart) {

@@ -437,7 +437,7 @@ struct zlh_ctx {
     int mrt_transform = ZLH_TRANSFORM_VALU;   // FP32 learned-closure kernels: ZLH_TRANSFORM_*
     bool level_streams = true;
     std::vector<hipStream_t> lstream;
-    std::vector<hipEvent_t> ev_step, ev_xfer, ev_join;
+    std::vector<hipEvent_t> ev_step, ev_xfer, ev_join, ev_rows;
     hipEvent_t ev_fork = nullptr;
     void drop_graphs() {
         for (auto &g : graph_exec) (void)hipGraphExecDestroy(g.second);
@@ -1078,12 +1078,14 @@ zlh_status zlh_create(const zlh_solver_desc *desc, zlh_ctx **out) {
     ctx->ev_step.assign(nl, nullptr);
     ctx->ev_xfer.assign(nl, nullptr);
     ctx->ev_join.assign(nl, nullptr);
+    ctx->ev_rows.assign(nl, nullptr);
     if (hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess) return ZLH_EDEVICE;
     for (int l = 0; l < nl; ++l)
         if (hipStreamCreate(&ctx->lstream[l]) != hipSuccess ||
             hipEventCreateWithFlags(&ctx->ev_step[l], hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&ctx->ev_xfer[l], hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&ctx->ev_join[l], hipEventDisableTiming) != hipSuccess)
+            hipEventCreateWithFlags(&ctx->ev_join[l], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&ctx->ev_rows[l], hipEventDisableTiming) != hipSuccess)
             return ZLH_EDEVICE;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -1107,7 +1109,7 @@ void zlh_destroy(zlh_ctx *ctx) {
     ctx->lv.clear();
     for (hipStream_t s : ctx->lstream)
         if (s) (void)hipStreamDestroy(s);
-    for (auto *v : {&ctx->ev_step, &ctx->ev_xfer, &ctx->ev_join})
+    for (auto *v : {&ctx->ev_step, &ctx->ev_xfer, &ctx->ev_join, &ctx->ev_rows})
         for (hipEvent_t e : *v)
             if (e) (void)hipEventDestroy(e);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
@@ -1336,6 +1338,14 @@ zlh_status march_launches(zlh_ctx *ctx, int nsteps) {
     // coarse level's step overlaps the finer levels' substeps.  Timed runs keep one stream, so
     // that per-launch events time a kernel alone.
     const bool split = ctx->level_streams && !ctx->timing && nl > 1;
+    // With level streams (and no time interpolation), level l - 1's stream forms the coarse rows
+    // of the next transfer into l right after its step and the transfer into l - 1 (the coarse
+    // state those rows read is then final: nothing else writes level l - 1 before that transfer
+    // into l, whose own fine-to-coarse copy comes after them); the transfer into l waits for
+    // them (ev_rows[l], which also follows ev_step[l - 1]) and forms only the fallback rows, so
+    // the coarse rows leave the finer level's chain.  Level l - 1's step itself waits for the
+    // previous transfer into l, whose gather reads the rows it overwrites.
+    const bool early_rows = split && !ctx->time_interp;
     hipStream_t main = ctx->stream;
     if (split) {
         ZLH_TRY(hipEventRecord(ctx->ev_fork, main));
@@ -1359,7 +1369,9 @@ zlh_status march_launches(zlh_ctx *ctx, int nsteps) {
             // (with time interpolation, after odd substeps too: the refill reads the coarse level's
             // latest state, which its next step overwrites, so the same ordering holds)
             if (l > 0 && (tsl[l] % 2 == 0 || ctx->time_interp)) {
-                if (split && (e = hipStreamWaitEvent(ctx->stream, ctx->ev_step[l - 1], 0)) != hipSuccess) break;
+                if (split && (e = hipStreamWaitEvent(ctx->stream, early_rows ? ctx->ev_rows[l] : ctx->ev_step[l - 1],
+                                                     0)) != hipSuccess)
+                    break;
                 // where the transfers into l and l + 1 touch common nodes of level l (interfaces
                 // that come within a stencil), both keep the serial order: the transfer into l
                 // after the latest transfer into l + 1, and the transfer into l after the latest
@@ -1371,12 +1383,17 @@ zlh_status march_launches(zlh_ctx *ctx, int nsteps) {
                 if (split && l >= 2 && ctx->lv[l - 1].xfer_overlap && xfer_recorded[l - 1] &&
                     (e = hipStreamWaitEvent(ctx->stream, ctx->ev_xfer[l - 1], 0)) != hipSuccess)
                     break;
-                if ((s = zlh_transfer_levels(ctx, l))) break;
+                if ((s = transfer_checked(ctx, l, kTransferC2F | kTransferF2C | (early_rows ? kTransferRowsReady : 0))))
+                    break;
                 if (split) {
                     if ((e = hipEventRecord(ctx->ev_xfer[l], ctx->stream)) != hipSuccess) break;
                     after_xfer[l - 1] = 1;
                     xfer_recorded[l] = 1;
                 }
+            }
+            if (early_rows && l < nl - 1) {
+                if ((s = launch_transfers(ctx, l + 1, kTransferCoarseRows))) break;
+                if ((e = hipEventRecord(ctx->ev_rows[l + 1], ctx->stream)) != hipSuccess) break;
             }
         }
     }
