@@ -217,13 +217,20 @@ def test_vorticity_refined_turbulence_fp32_velocity_l2():
     assert l2 < 1e-5
 
 
-def test_graph_replay_matches_direct_launches():
+@pytest.mark.parametrize("base,levels", [
+    (24, 3),
+    pytest.param(24, 5, marks=pytest.mark.skip(
+        reason="with the two coarsest levels mapped onto one shared stream this 5-level case crashed the "
+               "host inside zlh_time_marching; that mapping was taken out and its cause is not shown, so "
+               "the case stays off until it is")),
+])
+def test_graph_replay_matches_direct_launches(base, levels):
     """Replayed graphs (of two background steps, and of one for an odd count) and direct
     launches, each with the levels on streams of their own,
     against direct launches on one stream, over calls of odd and even length with manual substeps
     and downloads (synchronisations) in between: every replay bit-identical."""
     from zllbm_amd import workloads
-    W = workloads.decaying_turbulence(24, 3, buffer=2)
+    W = workloads.decaying_turbulence(base, levels, buffer=2)
     runs = []
     for graphs, streams in ((True, True), (False, True), (False, False)):
         s = hip.Solver(W.grid, precision=hip.FP32_SHIFTED, **W.solver).build()
@@ -236,13 +243,13 @@ def test_graph_replay_matches_direct_launches():
     for n in (2, 3, 4, 1, 2, 2):
         for s in runs:
             s.time_marching(n)
-        for l in range(3):
+        for l in range(levels):
             f = ref.f(l)
             for s in runs[:-1]:
                 np.testing.assert_array_equal(s.f(l), f)
         if n == 1:          # a finest-level substep outside the graph flips that level's buffers
             for s in runs:
-                s.step_level(2, 1)
+                s.step_level(levels - 1, 1)
 
 
 def test_graph_replay_with_immersed_boundary():
